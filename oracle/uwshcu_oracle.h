@@ -1,0 +1,102 @@
+/*
+ * uwshcu_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial FP64 CPU restatement of the UW shallow-cumulus column physics
+ * (compute_uwshcu_inv / compute_uwshcu and helpers, uwshcu.F90:251-5454;
+ * saturation tables wv_saturation.F90:64-204,482-609,734-751; gffgch.F90).
+ *
+ * This library is the checker for the HIP product path.  Only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark may load
+ * it.  Nothing in the shipped library links or calls it.
+ */
+#ifndef UWSHCU_ORACLE_H
+#define UWSHCU_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NCNST_MAX 32
+
+/* Physical constants and tables.  Field meaning follows the module variables of
+ * uwshcu.F90:25-37 (xlv..rovcp, rpen) and wv_saturation.F90:48-60. */
+typedef struct orc_consts {
+    /* uwshcu module (init_uwshcu, uwshcu.F90:230-239) */
+    double xlv, xlf, xls, cp, zvir, r, g, ep2, p00, rovcp;
+    /* tunables: rpen is a namelist value (uwshcu.F90:37,95); rle and rkm are
+     * compile-time parameters in the reference (uwshcu.F90:1036-1038) */
+    double rpen, rle, rkm;
+    /* wv_saturation module (gestbl, wv_saturation.F90:125-140) */
+    double tmin, tmax, ttrice, epsqs, rgasv, hlatf, hlatv, cpws, tmelt;
+    int    icephs;
+    double pcf[5];
+    double estbl[250];
+    /* constituents: minimum values and 'wet' (1) / 'dry' (0) type, 1-based index */
+    double qmin[ORC_NCNST_MAX + 1];
+    int    wet[ORC_NCNST_MAX + 1];
+    int    ixnumliq, ixnumice;   /* 1-based constituent indices of NUMLIQ/NUMICE */
+} orc_consts;
+
+/* Fill CESM default constants (shr_const values) and build the Goff-Gratch table. */
+void orc_default_consts(orc_consts *c);
+
+/* Primitives, exposed for unit tests. */
+void   orc_gffgch(const orc_consts *c, double t, double *es, int itype);
+void   orc_gestbl(orc_consts *c, double tmn, double tmx, double trice, int ip);
+double orc_estblf(const orc_consts *c, double td);
+void   orc_qsat(const orc_consts *c, double t, double p, double *es, double *qs, double *gam);
+void   orc_conden(const orc_consts *c, double p, double thl, double qt,
+                  double *th, double *qv, double *ql, double *qi, double *rvls, int *id_check);
+void   orc_roots(double a, double b, double c, double *r1, double *r2, int *status);
+void   orc_slope(int mkx, const double *field, const double *p0, double *out);
+double orc_qsinvert(const orc_consts *c, double qt, double thl, double psfc);
+double orc_compute_alpha(double del_CIN, double ke);
+double orc_compute_mumin2(double mulcl, double rmaxfrac, double mulow);
+double orc_compute_ppen(double wtwb, double D, double bogbot, double bogtop, double rho0j, double dpen);
+void   orc_fluxbelowinv(const orc_consts *c, double cbmf, const double *ps0, int mkx, int kinv, double dt,
+                        double xsrc, double xmean, double xtopin, double xbotin, double *xflx);
+void   orc_positive_moisture_single(double xlv, double xls, int mkx, double dt, double qvmin, double qlmin,
+                                    double qimin, const double *dp, double *qv, double *ql, double *qi,
+                                    double *s, double *qvten, double *qlten, double *qiten, double *sten);
+double orc_single_cin(const orc_consts *c, double pbot, double thv0bot, double ptop, double thv0top,
+                      double thvubot, double thvutop);
+void   orc_getbuoy(const orc_consts *c, double pbot, double thv0bot, double ptop, double thv0top,
+                   double thvubot, double thvutop, double *plfc, double *cin);
+/* Wet bulb for one (column,level) point; returns doit code (0 not converged, 1 skipped,
+ * 2 converged, 4 too cold); *enerr = relative enthalpy error for converged points. */
+int    orc_findsp_point(const orc_consts *c, double q, double t, double p,
+                        double *tsp, double *qsp, double *enerr);
+
+/* Number of integer diagnostics per column written by orc_compute_uwshcu_inv. */
+#define ORC_NDIAG 12
+/* diag[] layout per column:
+ *  0 exit code (0 convective, >0 reason of the final exit; see oracle source)
+ *  1 kinv 2 klcl 3 klfc 4 krel 5 kbup 6 kpen  (UW bottom-up indices of the final pass)
+ *  7 path: 0 exit, 1 iteration-1 result kept (del_CIN<=0), 2 implicit-CIN result
+ *  8 limit_ufrc 9 limit_ppen 10 limit_emf 11 iteration in which the column exited */
+
+/*
+ * Same argument list and layout as compute_uwshcu_inv (uwshcu.F90:251-265):
+ * Fortran column-major arrays with leading dimension mix (= pcols), CAM level
+ * order (index 1 = model top).  Constituents are tr0(mix,mkx,ncnst).
+ * Returns 0, or a negative code for conditions the reference treats as fatal:
+ *   -1 findsp not converging, -2 findsp enthalpy error, -3 mu > mumin2,
+ *   -4 unsupported sizes.  *bad_col receives the 0-based offending column.
+ */
+int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int ncnst, double dt,
+    const double *ps0_inv, const double *zs0_inv, const double *p0_inv, const double *z0_inv,
+    const double *dp0_inv, const double *u0_inv, const double *v0_inv, const double *qv0_inv,
+    const double *ql0_inv, const double *qi0_inv, const double *t0_inv, const double *s0_inv,
+    const double *tr0_inv, const double *tke_inv, const double *cldfrct_inv,
+    const double *concldfrct_inv, const double *pblh, double *cush,
+    double *umf_inv, double *slflx_inv, double *qtflx_inv, double *flxprc1_inv, double *flxsnow1_inv,
+    double *qvten_inv, double *qlten_inv, double *qiten_inv, double *sten_inv, double *uten_inv,
+    double *vten_inv, double *trten_inv, double *qrten_inv, double *qsten_inv, double *precip,
+    double *snow, double *evapc_inv, double *cufrc_inv, double *qcu_inv, double *qlu_inv,
+    double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
+    const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
