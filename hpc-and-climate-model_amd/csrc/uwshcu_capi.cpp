@@ -1,0 +1,460 @@
+// uwshcu_capi.cpp -- C ABI host layer: contexts, tables, workspace, launches.
+//
+// Replaces the per-call allocate/copy/free sequence of the reference C entry point
+// (bxcompute_uwshcu_inv.cpp:6208-6493, 6887-7043) with a persistent context:
+// device constants and table, a status word and a workspace that grows on
+// demand.  The level flip and the wet-bulb computation run inside the kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "uwshcu.h"
+#include "uwshcu_kernels.h"
+
+using uwshcu::FwdArgs;
+
+struct uwshcu_ctx {
+    int device = 0;
+    uwshcu_consts consts{};
+    double estbl[UWSHCU_ESTBL_LEN]{};
+    uwshcu_consts* d_consts = nullptr;
+    double* d_estbl = nullptr;
+    int* d_err = nullptr;
+    void* d_ws = nullptr;
+    size_t ws_bytes = 0;
+    size_t ws_limit = (size_t)24 << 30;
+    // staging for the host-pointer entry points
+    double* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    int* d_diag = nullptr;
+    size_t diag_bytes = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+};
+
+namespace {
+
+// gffgch.F90:1-127 (Goff & Gratch 1946).  itype < 0 selects a -itype degree
+// transition between water and ice below tmelt.
+double goff_gratch(double t, int itype, double tmelt) {
+    double tr = 0.0, eswtr = 0.0, es;
+    if (itype < 0) {
+        tr = std::fabs((double)itype);
+        itype = 1;
+    }
+    const bool ice_only = (t < (tmelt - tr) && itype == 1);
+    if (!ice_only) {
+        const double ps = 1013.246, ts = 373.16;
+        const double e1 = 11.344 * (1.0 - t / ts);
+        const double e2 = -3.49149 * (ts / t - 1.0);
+        const double f1 = -7.90298 * (ts / t - 1.0);
+        const double f2 = 5.02808 * std::log10(ts / t);
+        const double f3 = -1.3816 * (std::pow(10.0, e1) - 1.0) / 10000000.0;
+        const double f4 = 8.1328 * (std::pow(10.0, e2) - 1.0) / 1000.0;
+        const double f5 = std::log10(ps);
+        const double f = f1 + f2 + f3 + f4 + f5;
+        es = std::pow(10.0, f) * 100.0;
+        eswtr = es;
+        if (t >= tmelt || itype == 0) return es;
+    }
+    const double t0 = tmelt;
+    const double term1 = 2.01889049 / (t0 / t);
+    const double term2 = 3.56654 * std::log(t0 / t);
+    const double term3 = 20.947031 * (t0 / t);
+    es = 575.185606e10 * std::exp(-(term1 + term2 + term3));
+    if (t < (tmelt - tr)) return es;
+    const double weight = std::fmin((tmelt - t) / tr, 1.0);
+    return weight * es + (1.0 - weight) * eswtr;
+}
+
+bool hip_ok(hipError_t e) { return e == hipSuccess; }
+
+int ensure_ws(uwshcu_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->ws_bytes) return UWSHCU_OK;
+    if (ctx->d_ws) {
+        if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+        (void)hipFree(ctx->d_ws);
+        ctx->d_ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    if (!hip_ok(hipMalloc(&ctx->d_ws, bytes))) return UWSHCU_ERR_NOMEM;
+    ctx->ws_bytes = bytes;
+    return UWSHCU_OK;
+}
+
+struct Chunk {
+    const double *ps0, *zs0, *p0, *z0, *dp0, *u0, *v0, *qv0, *ql0, *qi0, *t0, *s0, *tr0, *tke, *pblh, *dpdry0;
+    double* cush;
+    double *umf, *slflx, *qtflx, *flxprc1, *flxsnow1, *qvten, *qlten, *qiten, *sten, *uten, *vten, *trten;
+    double *qrten, *qsten, *precip, *snow, *evapc, *cufrc, *qcu, *qlu, *qiu, *cbmf, *qc, *rliq, *cnt, *cnb, *wu;
+};
+
+int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,
+               double dt, const Chunk& ch, int* diag, hipStream_t stream) {
+    if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
+        return UWSHCU_ERR_ARG;
+    if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED) return UWSHCU_ERR_ARG;
+    if (ncol == 0) return UWSHCU_OK;
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, precision);
+    // batch of columns that shares one workspace
+    long long nb = (long long)(ctx->ws_limit / per_col);
+    if (nb > ncol) nb = ncol;
+    nb = (nb / 64) * 64;
+    if (nb < 64) nb = 64;
+    const long long wld = nb;
+    int rc = ensure_ws(ctx, per_col * (size_t)wld);
+    if (rc) return rc;
+
+    FwdArgs a{};
+    a.ncol = ncol;
+    a.mkx = pver;
+    a.ncnst = ncnst;
+    a.ldc = pcols;
+    a.wld = wld;
+    a.dt = dt;
+    uwshcu_default_params(&a.par);
+    if (par) a.par = *par;
+    a.ps0 = ch.ps0; a.zs0 = ch.zs0; a.p0 = ch.p0; a.z0 = ch.z0; a.dp0 = ch.dp0; a.dpdry0 = ch.dpdry0;
+    a.u0 = ch.u0; a.v0 = ch.v0; a.qv0 = ch.qv0; a.ql0 = ch.ql0; a.qi0 = ch.qi0; a.t0 = ch.t0; a.s0 = ch.s0;
+    a.tr0 = ch.tr0; a.tke = ch.tke; a.pblh = ch.pblh; a.cush = ch.cush;
+    a.umf = ch.umf; a.slflx = ch.slflx; a.qtflx = ch.qtflx; a.flxprc1 = ch.flxprc1; a.flxsnow1 = ch.flxsnow1;
+    a.qvten = ch.qvten; a.qlten = ch.qlten; a.qiten = ch.qiten; a.sten = ch.sten; a.uten = ch.uten;
+    a.vten = ch.vten; a.trten = ch.trten; a.qrten = ch.qrten; a.qsten = ch.qsten; a.precip = ch.precip;
+    a.snow = ch.snow; a.evapc = ch.evapc; a.cufrc = ch.cufrc; a.qcu = ch.qcu; a.qlu = ch.qlu; a.qiu = ch.qiu;
+    a.cbmf = ch.cbmf; a.qc = ch.qc; a.rliq = ch.rliq; a.cnt = ch.cnt; a.cnb = ch.cnb; a.wu = ch.wu;
+    a.diag = diag;
+    a.err = ctx->d_err;
+    a.ws = ctx->d_ws;
+    for (long long c0 = 0; c0 < ncol; c0 += nb) {
+        const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
+        a.col0 = (int)c0;
+        hipError_t e = (precision == UWSHCU_FP64)
+                           ? uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream)
+                           : uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream);
+        if (!hip_ok(e)) return UWSHCU_ERR_HIP;
+    }
+    return UWSHCU_OK;
+}
+
+std::mutex g_dropin_mu;
+uwshcu_ctx* g_dropin_ctx = nullptr;
+uwshcu_params g_dropin_par{10.0, 0.1, 14.0};
+int g_dropin_status = 0, g_dropin_badcol = -1;
+
+}  // namespace
+
+extern "C" {
+
+int uwshcu_default_consts(uwshcu_consts* c) {
+    if (!c) return UWSHCU_ERR_ARG;
+    std::memset(c, 0, sizeof(*c));
+    // CESM shr_const_mod values, combined in the order shr_const_mod combines them
+    const double boltz = 1.38065e-23, avogad = 6.02214e26;
+    const double rgas = avogad * boltz;
+    const double mwdair = 28.966, mwwv = 18.016;
+    const double rdair = rgas / mwdair, rwv = rgas / mwwv;
+    c->xlv = 2.501e6;
+    c->xlf = 3.337e5;
+    c->xls = c->xlv + c->xlf;
+    c->cp = 1.00464e3;
+    c->zvir = rwv / rdair - 1.0;
+    c->r = rdair;
+    c->g = 9.80616;
+    c->ep2 = mwwv / mwdair;
+    c->p00 = 1.e5;
+    c->rovcp = c->r / c->cp;
+    c->tmin = 173.16;      // esinti.F90:44
+    c->tmax = 375.16;      // esinti.F90:46
+    c->ttrice = 20.0;      // esinti.F90:47
+    c->icephs = 1;         // esinti.F90:48
+    c->epsqs = mwwv / mwdair;
+    c->rgasv = rwv;
+    c->hlatf = 3.337e5;
+    c->hlatv = 2.501e6;
+    c->cpws = 1.00464e3;
+    c->tmelt = 273.15;
+    c->ixnumliq = 4;
+    c->ixnumice = 5;
+    // degree-5 fit of es(water)-es(ice), wv_saturation.F90:178-182
+    c->pcf[0] = 5.04469588506e-01;
+    c->pcf[1] = -5.47288442819e+00;
+    c->pcf[2] = -3.67471858735e-01;
+    c->pcf[3] = -8.95963532403e-03;
+    c->pcf[4] = -7.78053686625e-05;
+    for (int m = 0; m <= UWSHCU_NCNST_MAX; ++m) {
+        c->qmin[m] = 0.0;
+        c->wet[m] = 1;
+    }
+    c->qmin[1] = 1.e-12;   // Q (initindx.F90:75); CLDLIQ/CLDICE/NUMLIQ/NUMICE 0 (microp_driver.F90:112-120)
+    return UWSHCU_OK;
+}
+
+int uwshcu_build_estbl(const uwshcu_consts* c, double* estbl) {
+    if (!c || !estbl) return UWSHCU_ERR_ARG;
+    // wv_saturation.F90:141-169
+    const int lentbl = (int)(c->tmax - c->tmin + 2.000001);
+    if (lentbl > UWSHCU_ESTBL_LEN) return UWSHCU_ERR_ARG;
+    int itype;
+    if (c->icephs) itype = (c->ttrice != 0.0) ? (int)(-c->ttrice) : 1;
+    else itype = 0;
+    double t = c->tmin - 1.0;
+    for (int n = 0; n < lentbl; ++n) {
+        t = t + 1.0;
+        estbl[n] = goff_gratch(t, itype, c->tmelt);
+    }
+    for (int n = lentbl; n < UWSHCU_ESTBL_LEN; ++n) estbl[n] = -99999.0;
+    return UWSHCU_OK;
+}
+
+int uwshcu_default_params(uwshcu_params* p) {
+    if (!p) return UWSHCU_ERR_ARG;
+    p->rpen = 10.0;
+    p->rle = 0.1;
+    p->rkm = 14.0;
+    return UWSHCU_OK;
+}
+
+size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision) {
+    const size_t elem = (precision == UWSHCU_FP64) ? sizeof(double) : sizeof(float);
+    return (size_t)uwshcu::ws_arrays(ncnst) * (size_t)(pver + 1) * elem;
+}
+
+int uwshcu_ctx_create(const uwshcu_consts* c, const double* estbl, int device, uwshcu_ctx** out) {
+    if (!out) return UWSHCU_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (!hip_ok(hipGetDeviceCount(&ndev)) || ndev < 1) return UWSHCU_ERR_NO_DEVICE;
+    if (device < 0 && !hip_ok(hipGetDevice(&device))) return UWSHCU_ERR_HIP;
+    if (device >= ndev) return UWSHCU_ERR_ARG;
+    uwshcu_ctx* ctx = new uwshcu_ctx();
+    ctx->device = device;
+    if (c) ctx->consts = *c;
+    else uwshcu_default_consts(&ctx->consts);
+    if (estbl) std::memcpy(ctx->estbl, estbl, sizeof(ctx->estbl));
+    else uwshcu_build_estbl(&ctx->consts, ctx->estbl);
+    bool ok = hip_ok(hipSetDevice(device)) && hip_ok(hipMalloc((void**)&ctx->d_consts, sizeof(uwshcu_consts))) &&
+              hip_ok(hipMalloc((void**)&ctx->d_estbl, sizeof(ctx->estbl))) &&
+              hip_ok(hipMalloc((void**)&ctx->d_err, 2 * sizeof(int))) &&
+              hip_ok(hipMemcpy(ctx->d_consts, &ctx->consts, sizeof(uwshcu_consts), hipMemcpyHostToDevice)) &&
+              hip_ok(hipMemcpy(ctx->d_estbl, ctx->estbl, sizeof(ctx->estbl), hipMemcpyHostToDevice)) &&
+              hip_ok(uwshcu::launch_clear_status(ctx->d_err, nullptr)) && hip_ok(hipStreamCreate(&ctx->stream)) &&
+              hip_ok(hipDeviceSynchronize());
+    if (!ok) {
+        uwshcu_ctx_destroy(ctx);
+        return UWSHCU_ERR_HIP;
+    }
+    *out = ctx;
+    return UWSHCU_OK;
+}
+
+int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
+    if (!ctx) return UWSHCU_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->d_consts) (void)hipFree(ctx->d_consts);
+    if (ctx->d_estbl) (void)hipFree(ctx->d_estbl);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->d_ws) (void)hipFree(ctx->d_ws);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return UWSHCU_OK;
+}
+
+int uwshcu_ctx_set_workspace_limit(uwshcu_ctx* ctx, size_t bytes) {
+    if (!ctx || bytes == 0) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->ws_limit = bytes;
+    return UWSHCU_OK;
+}
+
+int uwshcu_forward_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver,
+                          int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
+                          const double* z0, const double* dp0, const double* u0, const double* v0,
+                          const double* qv0, const double* ql0, const double* qi0, const double* t0,
+                          const double* s0, const double* tr0, const double* tke, const double* cldfrc,
+                          const double* concldfrc, const double* pblh, double* cush, double* umf, double* slflx,
+                          double* qtflx, double* flxprc1, double* flxsnow1, double* qvten, double* qlten,
+                          double* qiten, double* sten, double* uten, double* vten, double* trten, double* qrten,
+                          double* qsten, double* precip, double* snow, double* evapc, double* cufrc, double* qcu,
+                          double* qlu, double* qiu, double* cbmf, double* qc, double* rliq, double* cnt,
+                          double* cnb, const double* dpdry0, double* wu, int* diag, void* stream) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!ctx) return UWSHCU_ERR_ARG;
+    const void* req[] = {ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tke, pblh, cush, umf, slflx,
+                         qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, qrten, qsten, precip,
+                         snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu};
+    for (const void* p : req)
+        if (!p) return UWSHCU_ERR_ARG;
+    if (ncnst > 0 && (!tr0 || !trten)) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Chunk ch{ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tr0, tke, pblh, dpdry0, cush,
+             umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
+             qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
+    return run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream);
+}
+
+int uwshcu_status(uwshcu_ctx* ctx, void* stream, int* bad_col) {
+    if (!ctx) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    int h[2] = {0, -1};
+    if (!hip_ok(hipMemcpyAsync(h, ctx->d_err, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream)) ||
+        !hip_ok(hipStreamSynchronize((hipStream_t)stream)))
+        return UWSHCU_ERR_HIP;
+    if (!hip_ok(uwshcu::launch_clear_status(ctx->d_err, (hipStream_t)stream))) return UWSHCU_ERR_HIP;
+    if (bad_col) *bad_col = h[0] ? h[1] : -1;
+    return h[0];
+}
+
+int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver,
+                        int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
+                        const double* z0, const double* dp0, const double* u0, const double* v0, const double* qv0,
+                        const double* ql0, const double* qi0, const double* t0, const double* s0,
+                        const double* tr0, const double* tke, const double* cldfrc, const double* concldfrc,
+                        const double* pblh, double* cush, double* umf, double* slflx, double* qtflx,
+                        double* flxprc1, double* flxsnow1, double* qvten, double* qlten, double* qiten,
+                        double* sten, double* uten, double* vten, double* trten, double* qrten, double* qsten,
+                        double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
+                        double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
+                        const double* dpdry0, double* wu, int* diag, int* bad_col) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
+        return UWSHCU_ERR_ARG;
+    if (bad_col) *bad_col = -1;
+    if (ncol == 0) return UWSHCU_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    const size_t L = (size_t)pcols * pver, I = (size_t)pcols * (pver + 1), S = (size_t)pcols;
+    const size_t T = (size_t)pcols * pver * ncnst;
+    // staging layout: inputs then outputs, each a full (pcols, ...) array
+    const size_t n_in = 3 * I + 11 * L + T + 2 * S;
+    const size_t n_out = 6 * I + 14 * L + T + 6 * S;
+    const size_t bytes = (n_in + n_out) * sizeof(double);
+    if (bytes > ctx->stage_bytes) {
+        if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+        ctx->d_stage = nullptr;
+        ctx->stage_bytes = 0;
+        if (!hip_ok(hipMalloc((void**)&ctx->d_stage, bytes))) return UWSHCU_ERR_NOMEM;
+        ctx->stage_bytes = bytes;
+    }
+    int* ddiag = nullptr;
+    if (diag) {
+        const size_t db = (size_t)UWSHCU_NDIAG * pcols * sizeof(int);
+        if (db > ctx->diag_bytes) {
+            if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+            ctx->d_diag = nullptr;
+            ctx->diag_bytes = 0;
+            if (!hip_ok(hipMalloc((void**)&ctx->d_diag, db))) return UWSHCU_ERR_NOMEM;
+            ctx->diag_bytes = db;
+        }
+        ddiag = ctx->d_diag;
+    }
+    double* base = ctx->d_stage;
+    size_t off = 0;
+    auto take = [&](size_t n) { double* p = base + off; off += n; return p; };
+    hipStream_t st = ctx->stream;
+    bool ok = true;
+    auto h2d = [&](const double* h, size_t n) {
+        double* d = take(n);
+        if (n) ok = ok && hip_ok(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
+        return (const double*)d;
+    };
+    Chunk ch{};
+    ch.ps0 = h2d(ps0, I); ch.zs0 = h2d(zs0, I); ch.tke = h2d(tke, I);
+    ch.p0 = h2d(p0, L); ch.z0 = h2d(z0, L); ch.dp0 = h2d(dp0, L); ch.dpdry0 = h2d(dpdry0, L);
+    ch.u0 = h2d(u0, L); ch.v0 = h2d(v0, L); ch.qv0 = h2d(qv0, L); ch.ql0 = h2d(ql0, L); ch.qi0 = h2d(qi0, L);
+    ch.t0 = h2d(t0, L); ch.s0 = h2d(s0, L);
+    ch.tr0 = h2d(tr0, T);
+    ch.pblh = h2d(pblh, S);
+    ch.cush = const_cast<double*>(h2d(cush, S));
+    ch.umf = take(I); ch.slflx = take(I); ch.qtflx = take(I); ch.flxprc1 = take(I); ch.flxsnow1 = take(I);
+    ch.wu = take(I);
+    ch.qvten = take(L); ch.qlten = take(L); ch.qiten = take(L); ch.sten = take(L); ch.uten = take(L);
+    ch.vten = take(L); ch.qrten = take(L); ch.qsten = take(L); ch.evapc = take(L); ch.cufrc = take(L);
+    ch.qcu = take(L); ch.qlu = take(L); ch.qiu = take(L); ch.qc = take(L);
+    ch.trten = take(T);
+    ch.precip = take(S); ch.snow = take(S); ch.cbmf = take(S); ch.rliq = take(S); ch.cnt = take(S); ch.cnb = take(S);
+    if (!ok) return UWSHCU_ERR_HIP;
+    int rc = run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, ddiag, st);
+    if (rc) return rc;
+    auto d2h = [&](double* h, const double* d, size_t n) {
+        if (n) ok = ok && hip_ok(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    };
+    d2h(cush, ch.cush, S);
+    d2h(umf, ch.umf, I); d2h(slflx, ch.slflx, I); d2h(qtflx, ch.qtflx, I); d2h(flxprc1, ch.flxprc1, I);
+    d2h(flxsnow1, ch.flxsnow1, I); d2h(wu, ch.wu, I);
+    d2h(qvten, ch.qvten, L); d2h(qlten, ch.qlten, L); d2h(qiten, ch.qiten, L); d2h(sten, ch.sten, L);
+    d2h(uten, ch.uten, L); d2h(vten, ch.vten, L); d2h(qrten, ch.qrten, L); d2h(qsten, ch.qsten, L);
+    d2h(evapc, ch.evapc, L); d2h(cufrc, ch.cufrc, L); d2h(qcu, ch.qcu, L); d2h(qlu, ch.qlu, L);
+    d2h(qiu, ch.qiu, L); d2h(qc, ch.qc, L);
+    d2h(trten, ch.trten, T);
+    d2h(precip, ch.precip, S); d2h(snow, ch.snow, S); d2h(cbmf, ch.cbmf, S); d2h(rliq, ch.rliq, S);
+    d2h(cnt, ch.cnt, S); d2h(cnb, ch.cnb, S);
+    if (diag && ok)
+        ok = hip_ok(hipMemcpyAsync(diag, ddiag, (size_t)UWSHCU_NDIAG * pcols * sizeof(int), hipMemcpyDeviceToHost, st));
+    int h[2] = {0, -1};
+    ok = ok && hip_ok(hipMemcpyAsync(h, ctx->d_err, sizeof(h), hipMemcpyDeviceToHost, st)) &&
+         hip_ok(hipStreamSynchronize(st)) && hip_ok(uwshcu::launch_clear_status(ctx->d_err, st));
+    if (!ok) return UWSHCU_ERR_HIP;
+    if (bad_col) *bad_col = h[0] ? h[1] : -1;
+    return h[0];
+}
+
+void bxcompute_uwshcu_inv_(int* pcols, int* pver, int* ncol, int* pcnst, double* ztodt, double* ps0_inv,
+                           double* zs0_inv, double* p0_inv, double* z0_inv, double* dp0_inv, double* u0_inv,
+                           double* v0_inv, double* qv0_inv, double* ql0_inv, double* qi0_inv, double* t0_inv,
+                           double* s0_inv, double* tr0_inv, double* tke_inv, double* cldfrct_inv,
+                           double* concldfrct_inv, double* pblh, double* cush, double* umf_inv,
+                           double* slflx_inv, double* qtflx_inv, double* flxprc1_inv, double* flxsnow1_inv,
+                           double* qvten_inv, double* qlten_inv, double* qiten_inv, double* sten_inv,
+                           double* uten_inv, double* vten_inv, double* trten_inv, double* qrten_inv,
+                           double* qsten_inv, double* precip, double* snow, double* evapc_inv, double* cufrc_inv,
+                           double* qcu_inv, double* qlu_inv, double* qiu_inv, double* cbmf, double* qc_inv,
+                           double* rliq, double* cnt_inv, double* cnb_inv, int* lchnk, double* dpdry0_inv,
+                           double* wu_inv) {
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (!g_dropin_ctx) {
+        int rc = uwshcu_ctx_create(nullptr, nullptr, -1, &g_dropin_ctx);
+        if (rc) {
+            std::fprintf(stderr, "bxcompute_uwshcu_inv_: context creation failed (%d)\n", rc);
+            g_dropin_status = rc;
+            return;
+        }
+    }
+    int bad = -1;
+    int rc = uwshcu_forward_host(g_dropin_ctx, UWSHCU_FP64, &g_dropin_par, *pcols, *ncol, *pver, *pcnst, *ztodt,
+                                 ps0_inv, zs0_inv, p0_inv, z0_inv, dp0_inv, u0_inv, v0_inv, qv0_inv, ql0_inv,
+                                 qi0_inv, t0_inv, s0_inv, tr0_inv, tke_inv, cldfrct_inv, concldfrct_inv, pblh, cush,
+                                 umf_inv, slflx_inv, qtflx_inv, flxprc1_inv, flxsnow1_inv, qvten_inv, qlten_inv,
+                                 qiten_inv, sten_inv, uten_inv, vten_inv, trten_inv, qrten_inv, qsten_inv, precip,
+                                 snow, evapc_inv, cufrc_inv, qcu_inv, qlu_inv, qiu_inv, cbmf, qc_inv, rliq, cnt_inv,
+                                 cnb_inv, dpdry0_inv, wu_inv, nullptr, &bad);
+    g_dropin_status = rc;
+    g_dropin_badcol = bad;
+    if (rc)
+        std::fprintf(stderr, "bxcompute_uwshcu_inv_: chunk %d failed with code %d at column %d\n",
+                     lchnk ? *lchnk : -1, rc, bad);
+}
+
+int uwshcu_set_dropin_params(const uwshcu_params* par) {
+    if (!par) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    g_dropin_par = *par;
+    return UWSHCU_OK;
+}
+
+int uwshcu_dropin_last_status(int* bad_col) {
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (bad_col) *bad_col = g_dropin_badcol;
+    return g_dropin_status;
+}
+
+}  // extern "C"

@@ -1,0 +1,1281 @@
+// uwshcu_column.h -- one UW shallow-cumulus column per GPU thread.
+//
+// Follows compute_uwshcu (uwshcu.F90:1218-4549) stage by stage for one column i,
+// with the level flip of compute_uwshcu_inv (uwshcu.F90:380-414, 434-469) folded
+// into the indexing: UW level k (1 = lowest layer, interfaces 0..mkx) maps to row
+// mkx-k of the caller's CAM-ordered (pcols, pver[+1]) arrays.
+//
+// Memory layout.  Inputs and outputs are the caller's column-fastest arrays with
+// leading dimension ldc, so a wavefront touches 64 consecutive doubles per level.
+// Outputs double as working arrays where the reference's working array ends up in
+// an output.  Other per-level state lives in a workspace of (ws_arrays * (mkx+1))
+// rows of wld elements (column index fastest).
+//
+// Template parameter R is the type of the bulk arithmetic and of the workspace:
+// double for the FP64 path, float for the mixed path.  The saturation adjustment
+// (conden), the LCL inversion (qsinvert), the CIN integrals and the implicit-CIN
+// weight, the findsp wet bulb and the positive-moisture fix-up always run in FP64.
+#ifndef UWSHCU_COLUMN_H
+#define UWSHCU_COLUMN_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "uwshcu_thermo.h"
+
+// UWSHCU_BOUNDS_CHECK (host builds of the test harness only) asserts every index.
+#ifdef UWSHCU_BOUNDS_CHECK
+#include <cassert>
+#define UW_CHECK(cond) assert(cond)
+#else
+#define UW_CHECK(cond) ((void)0)
+#endif
+
+namespace uwshcu {
+
+// Workspace arrays (each mkx+1 rows).  A_* hold the environment of the input
+// state, B_* the environment re-derived from the iteration-1 provisional state
+// (uwshcu.F90:4285-4327), used only for the second CIN evaluation.
+enum WsArray : int {
+    A_THL0, A_QT0, A_SSTHL0, A_SSQT0, A_SSU0, A_SSV0, A_THV0BOT, A_THV0TOP, A_THVL0BOT, A_THVL0TOP,
+    A_EXN0, A_EXNS0,
+    B_THL0, B_QT0, B_SSTHL0, B_SSQT0, B_THV0BOT, B_THV0TOP, B_THVL0BOT, B_THVL0TOP,
+    W_EMF, W_THLU, W_QTU, W_UU, W_VU, W_THVU, W_UFRC, W_REI, W_FER, W_FDR, W_DWTEN, W_DITEN,
+    W_THLUE, W_QTUE, W_UUE, W_VUE, W_UFLX, W_VFLX, W_QLSINK, W_QISINK, W_NLSINK, W_NISINK,
+    W_QCL, W_QCI, W_QW0, W_QVSTAR, W_TRFD, W_TRFU,
+    WS_FIXED
+};
+// Per-constituent blocks follow: SSTR0, TRU, TRUE, TRFLX (ncnst arrays each).
+UWSHCU_HD inline int ws_arrays(int ncnst) { return WS_FIXED + 4 * ncnst; }
+
+struct FwdArgs {
+    int ncol, mkx, ncnst, ldc;
+    int col0;            // first column of this launch; workspace column = i - col0
+    long long wld;
+    double dt;
+    uwshcu_params par;
+    const double *ps0, *zs0, *p0, *z0, *dp0, *dpdry0, *u0, *v0, *qv0, *ql0, *qi0, *t0, *s0, *tr0, *tke, *pblh;
+    double* cush;
+    double *umf, *slflx, *qtflx, *flxprc1, *flxsnow1, *qvten, *qlten, *qiten, *sten, *uten, *vten, *trten;
+    double *qrten, *qsten, *precip, *snow, *evapc, *cufrc, *qcu, *qlu, *qiu, *cbmf, *qc, *rliq, *cnt, *cnb, *wu;
+    int* diag;          // optional [UWSHCU_NDIAG][ldc]
+    int* err;           // [2]: first fatal code, column
+    void* ws;           // workspace, R elements
+};
+
+// Exit reasons reported in diag row 0 (same numbering as the oracle).
+enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
+       EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
+
+template <class R>
+struct Column {
+    const FwdArgs& a;
+    const Thermo& T;
+    const uwshcu_consts& C;
+    const int i, iw, mkx, KS, ncnst;
+    const size_t ldc, wld;
+    R* __restrict__ ws;
+
+    UWSHCU_HD Column(const FwdArgs& args, const Thermo& th, int col)
+        : a(args), T(th), C(*th.c), i(col), iw(col - args.col0), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
+          ldc((size_t)args.ldc), wld((size_t)args.wld), ws(reinterpret_cast<R*>(args.ws)) {}
+
+    // caller arrays: UW level k <-> CAM row mkx-k (layers k=1..mkx and interfaces k=0..mkx)
+    UWSHCU_HD inline size_t row(int k) const { return (size_t)(mkx - k) * ldc + i; }
+#ifdef UWSHCU_BOUNDS_CHECK
+    UWSHCU_HD inline bool iface(const double* p) const {
+        return p == a.ps0 || p == a.zs0 || p == a.tke || p == a.umf || p == a.slflx || p == a.qtflx ||
+               p == a.flxprc1 || p == a.flxsnow1 || p == a.wu;
+    }
+#endif
+    UWSHCU_HD inline double in(const double* p, int k) const {
+        UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
+        return p[row(k)];
+    }
+    UWSHCU_HD inline double& out(double* p, int k) const {
+        UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
+        return p[row(k)];
+    }
+    UWSHCU_HD inline double trin(int m, int k) const {   // m 1-based
+        UW_CHECK(m >= 1 && m <= ncnst && k >= 1 && k <= mkx);
+        return a.tr0[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
+    }
+    UWSHCU_HD inline double& trout(int m, int k) const {
+        UW_CHECK(m >= 1 && m <= ncnst && k >= 1 && k <= mkx);
+        return a.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
+    }
+    UWSHCU_HD inline R& W(int arr, int k) const {
+        UW_CHECK(arr >= 0 && arr < WS_FIXED && k >= 0 && k <= mkx && iw >= 0 && (long long)iw < (long long)wld);
+        return ws[((size_t)arr * KS + k) * wld + iw]; }
+    UWSHCU_HD inline R& WT(int blk, int m, int k) const {   // blk 0..3, m 1-based
+        UW_CHECK(blk >= 0 && blk < 4 && m >= 1 && m <= ncnst && k >= 0 && k <= mkx && iw >= 0 && (long long)iw < (long long)wld);
+        return ws[((size_t)(WS_FIXED + blk * ncnst + (m - 1)) * KS + k) * wld + iw];
+    }
+
+    UWSHCU_HD inline R thv_of(double thj, double qvj, double qlj, double qij) const {
+        return R(thj * (1.0 + C.zvir * qvj - qlj - qij));
+    }
+
+    // minmod slopes (uwshcu.F90:4831-4861) of field(k) -> W(dst, k), k = 1..mkx
+    template <class F>
+    UWSHCU_HD inline void slope(F field, int dst) const {
+        R below = (R(field(2)) - R(field(1))) / (R(in(a.p0, 2)) - R(in(a.p0, 1)));
+        for (int k = 2; k <= mkx; ++k) {
+            const R above = (R(field(k)) - R(field(k - 1))) / (R(in(a.p0, k)) - R(in(a.p0, k - 1)));
+            W(dst, k - 1) = (above > R(0)) ? fmax(R(0), fmin(above, below)) : fmin(R(0), fmax(above, below));
+            below = above;
+        }
+        W(dst, mkx) = W(dst, mkx - 1);
+    }
+    template <class F>
+    UWSHCU_HD inline void slope_tr(F field, int m) const {
+        R below = (R(field(2)) - R(field(1))) / (R(in(a.p0, 2)) - R(in(a.p0, 1)));
+        for (int k = 2; k <= mkx; ++k) {
+            const R above = (R(field(k)) - R(field(k - 1))) / (R(in(a.p0, k)) - R(in(a.p0, k - 1)));
+            WT(0, m, k - 1) = (above > R(0)) ? fmax(R(0), fmin(above, below)) : fmin(R(0), fmax(above, below));
+            below = above;
+        }
+        WT(0, m, mkx) = WT(0, m, mkx - 1);
+    }
+
+    // Environment at layer interfaces (uwshcu.F90:1275-1299, 4303-4327) for set base eb
+    // (A_THL0 or B_THL0), from thl0/qt0 already stored in that set.  Returns false on a
+    // conden failure.
+    UWSHCU_HD bool interface_env(int eb) const {
+        const int thl = eb, qt = eb + 1, ssthl = eb + 2, ssqt = eb + 3;
+        const int thvb = eb == A_THL0 ? A_THV0BOT : B_THV0BOT, thvt = thvb + 1;
+        const int thvlb = eb == A_THL0 ? A_THVL0BOT : B_THVL0BOT, thvlt = thvlb + 1;
+        for (int k = 1; k <= mkx; ++k) {
+            double th, qv, ql, qi;
+            const double pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
+            const R thl0bot = W(thl, k) + W(ssthl, k) * R(pb - pm);
+            const R qt0bot = W(qt, k) + W(ssqt, k) * R(pb - pm);
+            if (T.conden(pb, thl0bot, qt0bot, th, qv, ql, qi)) return false;
+            W(thvb, k) = thv_of(th, qv, ql, qi);
+            W(thvlb, k) = thl0bot * (R(1) + R(C.zvir) * qt0bot);
+            const R thl0top = W(thl, k) + W(ssthl, k) * R(pt - pm);
+            const R qt0top = W(qt, k) + W(ssqt, k) * R(pt - pm);
+            if (T.conden(pt, thl0top, qt0top, th, qv, ql, qi)) return false;
+            W(thvt, k) = thv_of(th, qv, ql, qi);
+            W(thvlt, k) = thl0top * (R(1) + R(C.zvir) * qt0top);
+        }
+        return true;
+    }
+
+    // Zero the column's outputs and the working arrays the reference zeroes before a
+    // pass (uwshcu.F90:1333-1408, 1817-1869).
+    UWSHCU_HD void zero_pass_state() const {
+        for (int k = 0; k <= mkx; ++k) {
+            out(a.umf, k) = 0.0;
+            out(a.wu, k) = 0.0;
+            out(a.slflx, k) = 0.0;
+            out(a.qtflx, k) = 0.0;
+            W(W_EMF, k) = R(0);
+            W(W_UFRC, k) = R(0);
+            W(W_UFLX, k) = R(0);
+            W(W_VFLX, k) = R(0);
+            W(W_THLU, k) = R(0);
+            W(W_QTU, k) = R(0);
+            W(W_UU, k) = R(0);
+            W(W_VU, k) = R(0);
+            W(W_THVU, k) = R(0);
+            for (int m = 1; m <= ncnst; ++m) {
+                WT(1, m, k) = R(0);
+                WT(3, m, k) = R(0);
+            }
+        }
+        for (int k = 1; k <= mkx; ++k) {
+            out(a.qvten, k) = 0.0; out(a.qlten, k) = 0.0; out(a.qiten, k) = 0.0; out(a.sten, k) = 0.0;
+            out(a.uten, k) = 0.0; out(a.vten, k) = 0.0; out(a.qrten, k) = 0.0; out(a.qsten, k) = 0.0;
+            out(a.evapc, k) = 0.0; out(a.cufrc, k) = 0.0; out(a.qcu, k) = 0.0; out(a.qlu, k) = 0.0;
+            out(a.qiu, k) = 0.0; out(a.qc, k) = 0.0;
+            W(W_FER, k) = R(0); W(W_FDR, k) = R(0); W(W_DWTEN, k) = R(0); W(W_DITEN, k) = R(0);
+            W(W_QCL, k) = R(0); W(W_QCI, k) = R(0);
+            for (int m = 1; m <= ncnst; ++m) trout(m, k) = 0.0;
+        }
+    }
+
+    UWSHCU_HD void write_exit(int mk) const {
+        // uwshcu.F90:4448-4482; flxprc1/flxsnow1 keep their initial zero (uwshcu.F90:1090-1091)
+        for (int k = 0; k <= mk; ++k) {
+            out(a.umf, k) = 0.0; out(a.wu, k) = 0.0; out(a.slflx, k) = 0.0; out(a.qtflx, k) = 0.0;
+            out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0;
+        }
+        for (int k = 1; k <= mk; ++k) {
+            out(a.qvten, k) = 0.0; out(a.qlten, k) = 0.0; out(a.qiten, k) = 0.0; out(a.sten, k) = 0.0;
+            out(a.uten, k) = 0.0; out(a.vten, k) = 0.0; out(a.qrten, k) = 0.0; out(a.qsten, k) = 0.0;
+            out(a.evapc, k) = 0.0; out(a.cufrc, k) = 0.0; out(a.qcu, k) = 0.0; out(a.qlu, k) = 0.0;
+            out(a.qiu, k) = 0.0; out(a.qc, k) = 0.0;
+            for (int m = 1; m <= ncnst; ++m) trout(m, k) = 0.0;
+        }
+        a.precip[i] = 0.0; a.snow[i] = 0.0; a.cush[i] = -1.0; a.cbmf[i] = 0.0; a.rliq[i] = 0.0;
+        a.cnt[i] = (double)(mk + 1) - 1.0;        // cnt = 1 flipped: mkx+1-1
+        a.cnb[i] = (double)(mk + 1) - (double)mk; // cnb = mkx flipped
+    }
+
+    UWSHCU_HD void run() const;
+};
+
+template <class R>
+UWSHCU_HD void Column<R>::run() const {
+    // uwshcu.F90:1040-1068 parameters
+    const R rkfre = R(1), rmaxfrac = R(0.10), mumin1 = R(0.906), rbuoy = R(1), rdrag = R(1);
+    const R epsvarw = R(5.e-4), PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
+    const R xlv = R(C.xlv), xls = R(C.xls), cp = R(C.cp), zvir = R(C.zvir), r = R(C.r), g = R(C.g);
+    const R rle = R(a.par.rle), rkm = R(a.par.rkm), rpen = R(a.par.rpen);
+    const R dt = R(a.dt);
+    const double dtd = a.dt;
+    const int ixl = C.ixnumliq, ixi = C.ixnumice;
+    const bool has_nl = ixl >= 1 && ixl <= ncnst, has_ni = ixi >= 1 && ixi <= ncnst;
+    auto trz = [&](int m, int k) -> R { return (m >= 1 && m <= ncnst) ? R(trin(m, k)) : R(0); };
+
+    int exit_code = EX_NONE, path = 0, exit_iter = 0;
+    int kinv = 0, klcl = 0, klfc = 0, krel = 0, kbup = 0, kpen = 0;
+    int limit_ufrc = 0, limit_ppen = 0, limit_emf = 0;
+    double th, qvj, qlj, qij;
+
+    // ---- wet bulb for the evaporation limiter (uwshcu.F90:1216; findsp 5164-5454) ----
+    for (int k = 1; k <= mkx; ++k) {
+        double qsp, enerr;
+        const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
+        if (doit == 0) record_error(a.err, UWSHCU_ERR_FINDSP_CONV, i);
+        else if (doit == 2 && enerr > 1.e-4) record_error(a.err, UWSHCU_ERR_FINDSP_ENTH, i);
+        W(W_QW0, k) = R(qsp);
+    }
+
+    // ---- S0 environment, set A (uwshcu.F90:1255-1299) ----
+    for (int k = 1; k <= mkx; ++k) W(A_EXN0, k) = R(pow(in(a.p0, k) / C.p00, C.rovcp));
+    for (int k = 0; k <= mkx; ++k) W(A_EXNS0, k) = R(pow(in(a.ps0, k) / C.p00, C.rovcp));
+    for (int k = 1; k <= mkx; ++k) {
+        const R qv0 = R(in(a.qv0, k)), ql0 = R(in(a.ql0, k)), qi0 = R(in(a.qi0, k)), t0 = R(in(a.t0, k));
+        const R qt0 = (qv0 + ql0 + qi0);
+        W(A_QT0, k) = qt0;
+        W(A_THL0, k) = (t0 - xlv * ql0 / cp - xls * qi0 / cp) / W(A_EXN0, k);
+    }
+    slope([&](int k) { return W(A_THL0, k); }, A_SSTHL0);
+    slope([&](int k) { return W(A_QT0, k); }, A_SSQT0);
+    slope([&](int k) { return R(in(a.u0, k)); }, A_SSU0);
+    slope([&](int k) { return R(in(a.v0, k)); }, A_SSV0);
+    for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
+
+    zero_pass_state();
+    for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
+
+    double cush = a.cush[i];
+    const double pblh = a.pblh[i];
+
+    // iteration-1 values restored for the implicit-CIN pass (uwshcu.F90:1693-1713)
+    int kinv_o = 0, klcl_o = 0, klfc_o = 0;
+    double plcl_o = 0, plfc_o = 0, tkeavg_o = 0, thvlmin_o = 0, qtsrc_o = 0, thvlsrc_o = 0, thlsrc_o = 0;
+    double usrc_o = 0, vsrc_o = 0, thv0lcl_o = 0, cin_i = 0, cinlcl_i = 0, ke = 0;
+    double cush_s = 0;
+
+    // locals that persist across stages
+    double plcl = 0, plfc = 0, tkeavg = 0, thvlmin = 0, qtsrc = 0, thvlsrc = 0, thlsrc = 0, usrc = 0, vsrc = 0;
+    double thv0lcl = 0, cin = 0, cinlcl = 0;
+    R prel = 0, thv0rel = 0, cbmf = 0, wlcl = 0, ufrclcl = 0, ppen = 0, thlu_top = 0, qtu_top = 0;
+    R rho0j = 0, drage = 0, bogbot = 0, bogtop = 0, wtwb = 0;
+    R trsrc_tmp = 0;
+    (void)trsrc_tmp;
+
+    if (!interface_env(A_THL0)) { exit_code = EX_CONDEN; goto L333; }
+
+    for (int iter = 1; iter <= 2; ++iter) {
+        exit_iter = iter;
+        const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
+        const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
+        const int e_thvb = (iter == 1) ? A_THV0BOT : B_THV0BOT, e_thvt = e_thvb + 1;
+        const int e_thvlb = (iter == 1) ? A_THVL0BOT : B_THVL0BOT, e_thvlt = e_thvlb + 1;
+        const double tscaleh = cush;
+        cush = -1.0;
+
+        // ---- S1 kinv (uwshcu.F90:1455-1468) ----
+        kinv = 1;
+        for (int k = mkx - 1; k >= 1; --k) {
+            if ((pblh + 5.0 - in(a.zs0, k)) * (pblh + 5.0 - in(a.zs0, k + 1)) < 0.0) { kinv = k + 1; break; }
+        }
+        if (kinv <= 1) { exit_code = EX_KINV1; goto L333; }
+
+        // ---- S2 PBL averages (uwshcu.F90:1484-1499) ----
+        {
+            double dpsum = 0.0;
+            tkeavg = 0.0;
+            thvlmin = 1000.0;
+            for (int k = 0; k <= kinv - 1; ++k) {
+                double dpi;
+                if (k == 0) dpi = in(a.ps0, 0) - in(a.p0, 1);
+                else if (k == kinv - 1) dpi = in(a.p0, kinv - 1) - in(a.ps0, kinv - 1);
+                else dpi = in(a.p0, k) - in(a.p0, k + 1);
+                dpsum = dpsum + dpi;
+                tkeavg = tkeavg + dpi * in(a.tke, k);
+                if (k != 0) thvlmin = fmin(thvlmin, fmin((double)W(e_thvlb, k), (double)W(e_thvlt, k)));
+            }
+            tkeavg = tkeavg / dpsum;
+        }
+
+        // ---- S3 source air (uwshcu.F90:1509-1516) ----
+        qtsrc = W(e_qt, 1);
+        thvlsrc = thvlmin;
+        thlsrc = thvlsrc / (1.0 + C.zvir * qtsrc);
+        usrc = R(in(a.u0, kinv - 1)) + W(A_SSU0, kinv - 1) * R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
+        vsrc = R(in(a.v0, kinv - 1)) + W(A_SSV0, kinv - 1) * R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
+
+        // ---- S4 LCL (uwshcu.F90:1533-1549) ----
+        plcl = T.qsinvert(qtsrc, thlsrc, in(a.ps0, 0));
+        klcl = mkx;
+        for (int k = 0; k <= mkx; ++k) {
+            if (in(a.ps0, k) < plcl) { klcl = k; break; }
+        }
+        klcl = klcl > 1 ? klcl : 1;
+        if (plcl < 30000.0) { exit_code = EX_KLCLMKX; goto L333; }
+
+        // ---- S5 thv0lcl (uwshcu.F90:1559-1567) ----
+        {
+            const double thl0lcl = (double)W(e_thl, klcl) + (double)W(e_ssthl, klcl) * (plcl - in(a.p0, klcl));
+            const double qt0lcl = (double)W(e_qt, klcl) + (double)W(e_ssqt, klcl) * (plcl - in(a.p0, klcl));
+            if (T.conden(plcl, thl0lcl, qt0lcl, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+            thv0lcl = th * (1.0 + C.zvir * qvj - qlj - qij);
+        }
+
+        // ---- S6 CIN (uwshcu.F90:1592-1685) ----
+        cin = 0.0;
+        cinlcl = 0.0;
+        plfc = 0.0;
+        klfc = mkx;
+        if (klcl >= kinv) {
+            double thvubot = 0.0, thvutop = 0.0;
+            for (int k = kinv; k <= mkx - 1; ++k) {
+                if (k < klcl) {
+                    thvubot = thvlsrc;
+                    thvutop = thvlsrc;
+                    cin = cin + T.single_cin(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop);
+                } else if (k == klcl) {
+                    thvubot = thvlsrc;
+                    thvutop = thvlsrc;
+                    cin = cin + T.single_cin(in(a.ps0, k - 1), W(e_thvb, k), plcl, thv0lcl, thvubot, thvutop);
+                    cinlcl = fmax(cin, 0.0);
+                    cin = cinlcl;
+                    thvubot = thvlsrc;
+                    if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
+                    T.getbuoy(plcl, thv0lcl, in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                    if (plfc > 0.0) { klfc = k; break; }
+                } else {
+                    thvubot = thvutop;
+                    if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
+                    T.getbuoy(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                    if (plfc > 0.0) { klfc = k; break; }
+                }
+            }
+        } else {
+            cinlcl = 0.0;
+            for (int k = kinv; k <= mkx - 1; ++k) {
+                if (T.conden(in(a.ps0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                const double thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
+                if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                const double thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
+                T.getbuoy(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                if (plfc > 0.0) { klfc = k; break; }
+            }
+        }
+        cin = fmax(0.0, cin);
+        if (klfc >= mkx) { klfc = mkx; exit_code = EX_KLFCMKX; goto L333; }
+
+        // ---- S7 implicit CIN (uwshcu.F90:1693-2014) ----
+        if (iter == 1) {
+            cin_i = cin;
+            cinlcl_i = cinlcl;
+            ke = 1.0 / (1.0 * tkeavg + 5.e-4);      // rbuoy / (rkfre*tkeavg + epsvarw)
+            kinv_o = kinv; klcl_o = klcl; klfc_o = klfc; plcl_o = plcl; plfc_o = plfc; tkeavg_o = tkeavg;
+            thvlmin_o = thvlmin; qtsrc_o = qtsrc; thvlsrc_o = thvlsrc; thlsrc_o = thlsrc; usrc_o = usrc;
+            vsrc_o = vsrc; thv0lcl_o = thv0lcl;
+        } else {
+            const double del_CIN = cin - cin_i;
+            if (del_CIN > 0.0) {
+                const double alpha = compute_alpha(del_CIN, ke);
+                cin = cin_i + alpha * del_CIN;
+                cinlcl = cinlcl_i;
+                kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
+                thvlmin = thvlmin_o; qtsrc = qtsrc_o; thvlsrc = thvlsrc_o; thlsrc = thlsrc_o; usrc = usrc_o;
+                vsrc = vsrc_o; thv0lcl = thv0lcl_o;
+                zero_pass_state();
+            } else {
+                // iteration-1 outputs stand (uwshcu.F90:1897-2010): they are already in the
+                // output arrays; flxprc1/flxsnow1 were never written and stay zero.
+                a.cush[i] = cush_s;
+                path = 1;
+                goto Ldone;
+            }
+        }
+
+        // From here on the environment is set A (the input state).
+        {
+        // ---- S8 release level (uwshcu.F90:2035-2043) ----
+        if (klcl < kinv) {
+            krel = kinv;
+            prel = R(in(a.ps0, krel - 1));
+            thv0rel = W(A_THV0BOT, krel);
+        } else {
+            krel = klcl;
+            prel = R(plcl);
+            thv0rel = R(thv0lcl);
+        }
+
+        // ---- S9 cloud-base closure (uwshcu.F90:2092-2184) ----
+        const R wcrit = sqrt(R(2) * R(cin) * rbuoy);
+        const R sigmaw = sqrt(rkfre * R(tkeavg) + epsvarw);
+        R mu = wcrit / sigmaw / R(1.4142);
+        if (mu >= R(3)) { exit_code = EX_MU3; goto L333; }
+        const R rho0inv = R(in(a.ps0, kinv - 1)) / (r * W(A_THV0TOP, kinv - 1) * W(A_EXNS0, kinv - 1));
+        cbmf = (rho0inv * sigmaw / R(2.5066)) * exp(-(mu * mu));
+        const R cbmflimit = R(0.9) * R(in(a.dp0, kinv - 1)) / g / dt;
+        R mumin0 = R(0);
+        if (cbmf > cbmflimit) mumin0 = sqrt(-log(R(2.5066) * cbmflimit / rho0inv / sigmaw));
+        mu = fmax(fmax(mu, mumin0), mumin1);
+        const R mulcl = sqrt(R(2) * R(cinlcl) * rbuoy) / R(1.4142) / sigmaw;
+        R mulclstar;
+        {
+            const R e1 = exp(-(mu * mu)) / R(2.5066), ef = erfc(mu);
+            mulclstar = sqrt(fmax(R(0), R(2) * (e1 * e1) * (R(1) / (ef * ef) - R(0.25) / (rmaxfrac * rmaxfrac))));
+        }
+        if (mulcl > R(1.e-8) && mulcl > mulclstar) {
+            const R mumin2 = compute_mumin2(mulcl, rmaxfrac, mu);
+            if (mu > mumin2) record_error(a.err, UWSHCU_ERR_MUMIN2, i);
+            mu = fmax(mu, mumin2);
+            if (mu == mumin2) limit_ufrc = 1;
+        }
+        if (mu == mumin1) limit_ufrc = 1;
+        cbmf = (rho0inv * sigmaw / R(2.5066)) * exp(-(mu * mu));
+        const R winv = sigmaw * (R(2) / R(2.5066)) * exp(-(mu * mu)) / erfc(mu);
+        R wtw = winv * winv - R(2) * R(cinlcl) * rbuoy;
+        if (wtw <= R(0)) { exit_code = EX_WTW; goto L333; }
+        wlcl = sqrt(wtw);
+        ufrclcl = cbmf / wlcl / rho0inv;
+        const R wrel = wlcl;
+        if (ufrclcl <= R(0.0001)) { exit_code = EX_UFRC; goto L333; }
+        W(W_UFRC, krel - 1) = ufrclcl;
+        for (int k = kinv - 1; k <= krel - 1; ++k) { out(a.umf, k) = (double)cbmf; out(a.wu, k) = (double)winv; }
+
+        // ---- S10 release-level updraft (uwshcu.F90:2205-2255) ----
+        W(W_EMF, krel - 1) = R(0);
+        out(a.umf, krel - 1) = (double)cbmf;
+        out(a.wu, krel - 1) = (double)wrel;
+        W(W_THLU, krel - 1) = R(thlsrc);
+        W(W_QTU, krel - 1) = R(qtsrc);
+        if (T.conden(prel, thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+        W(W_THVU, krel - 1) = thv_of(th, qvj, qlj, qij);
+        {
+            R uplus = R(0), vplus = R(0);
+            if (krel == kinv) {
+                uplus = PGFc * W(A_SSU0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
+                vplus = PGFc * W(A_SSV0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
+            } else {
+                const int kend = (krel - 1 > kinv) ? krel - 1 : kinv;
+                for (int k = kinv; k <= kend; ++k) {
+                    uplus = uplus + PGFc * W(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                    vplus = vplus + PGFc * W(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                }
+                uplus = uplus + PGFc * W(A_SSU0, krel) * (prel - R(in(a.ps0, krel - 1)));
+                vplus = vplus + PGFc * W(A_SSV0, krel) * (prel - R(in(a.ps0, krel - 1)));
+            }
+            W(W_UU, krel - 1) = R(usrc) + uplus;
+            W(W_VU, krel - 1) = R(vsrc) + vplus;
+        }
+        for (int m = 1; m <= ncnst; ++m) WT(1, m, krel - 1) = R(trin(m, 1));   // trsrc = tr0(1,m)
+
+        // ---- S11 buoyancy sorting (uwshcu.F90:2296-2952) ----
+        R scaleh = R(tscaleh);
+        if (R(tscaleh) < R(0)) scaleh = R(1000);
+        for (int iter_scaleh = 1; iter_scaleh <= 3; ++iter_scaleh) {
+            kbup = krel;
+            kpen = krel;
+            wtw = wlcl * wlcl;
+            R pe = R(0.5) * (prel + R(in(a.ps0, krel)));
+            R dpe = prel - R(in(a.ps0, krel));
+            R exne = R(T.exnf(pe));
+            R thvebot = thv0rel;
+            R thle = W(A_THL0, krel) + W(A_SSTHL0, krel) * (pe - R(in(a.p0, krel)));
+            R qte = W(A_QT0, krel) + W(A_SSQT0, krel) * (pe - R(in(a.p0, krel)));
+            R ue = R(in(a.u0, krel)) + W(A_SSU0, krel) * (pe - R(in(a.p0, krel)));
+            R ve = R(in(a.v0, krel)) + W(A_SSV0, krel) * (pe - R(in(a.p0, krel)));
+            // tre(m) = tr0(krel,m) + sstr0(krel,m)*(pe - p0(krel)) is recomputed where used
+            R pe_tr = pe - R(in(a.p0, krel));
+            int ktr = krel;            // layer whose (tr0, sstr0) define tre, and offset pe_tr
+            bool tr_mid = false;       // after the first layer tre = tr0(k) exactly
+
+            for (int k = krel; k <= mkx - 1; ++k) {
+                const int km1 = k - 1;
+                R thlue = W(W_THLU, km1);
+                R qtue = W(W_QTU, km1);
+                R wue = R(out(a.wu, km1));
+                wtwb = wtw;
+                const R umf_km1 = R(out(a.umf, km1));
+                R umfk = 0, ferk = 0, fdrk = 0;
+                for (int iter_xc = 1; iter_xc <= 2; ++iter_xc) {
+                    wtw = R(out(a.wu, km1)) * R(out(a.wu, km1));
+                    if (T.conden(pe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    const R thv0j = thv_of(th, qvj, qlj, qij);
+                    rho0j = pe / (r * thv0j * exne);
+                    double es, qs;
+                    T.qsat_es_qs((double)(thle * exne), pe, es, qs);
+                    const R excess0 = qte - R(qs);
+                    if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if ((R(qlj) + R(qij)) > criqc) {
+                        const R qlr = R(qlj), qir = R(qij);
+                        const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
+                        const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
+                        qtue = qtue - exql - exqi;
+                        thlue = thlue + (xlv / cp / exne) * exql + (xls / cp / exne) * exqi;
+                    }
+                    if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    const R thvj = thv_of(th, qvj, qlj, qij);
+                    T.qsat_es_qs((double)(thlue * exne), pe, es, qs);
+                    const R excessu = qtue - R(qs);
+                    const R cridis = rle * scaleh;
+                    R xc;
+                    if ((excessu <= R(0) && excess0 <= R(0)) || (excessu >= R(0) && excess0 >= R(0))) {
+                        xc = fmin(R(1), fmax(R(0), R(1) - R(2) * rbuoy * g * cridis / (wue * wue) * (R(1) - thvj / thv0j)));
+                    } else {
+                        const R xsat = excessu / (excessu - excess0);
+                        const R thlxsat = thlue + xsat * (thle - thlue);
+                        const R qtxsat = qtue + xsat * (qte - qtue);
+                        if (T.conden(pe, thlxsat, qtxsat, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        const R thvxsat = thv_of(th, qvj, qlj, qij);
+                        R x_cu = 0, x_en = 0;
+                        for (int kk = 1; kk <= 2; ++kk) {
+                            R thv_x0, thv_x1;
+                            if (kk == 1) {
+                                thv_x0 = thvj;
+                                thv_x1 = (R(1) - R(1) / xsat) * thvj + (R(1) / xsat) * thvxsat;
+                            } else {
+                                thv_x1 = thv0j;
+                                thv_x0 = (xsat / (xsat - R(1))) * thv0j + (R(1) / (R(1) - xsat)) * thvxsat;
+                            }
+                            const R aquad = wue * wue;
+                            const R bquad = R(2) * rbuoy * g * cridis * (thv_x1 - thv_x0) / thv0j - R(2) * (wue * wue);
+                            const R cquad = R(2) * rbuoy * g * cridis * (thv_x0 - thv0j) / thv0j + (wue * wue);
+                            R xs1 = 0, xs2 = 0;
+                            int status;
+                            if (kk == 1) {
+                                if ((bquad * bquad - R(4) * aquad * cquad) >= R(0)) {
+                                    roots(aquad, bquad, cquad, xs1, xs2, status);
+                                    x_cu = fmin(R(1), fmax(R(0), fmin(xsat, fmin(xs1, xs2))));
+                                } else {
+                                    x_cu = xsat;
+                                }
+                            } else {
+                                if ((bquad * bquad - R(4) * aquad * cquad) >= R(0)) {
+                                    roots(aquad, bquad, cquad, xs1, xs2, status);
+                                    x_en = fmin(R(1), fmax(R(0), fmax(xsat, fmin(xs1, xs2))));
+                                } else {
+                                    x_en = R(1);
+                                }
+                            }
+                        }
+                        xc = (x_cu == xsat) ? fmax(x_cu, x_en) : x_cu;
+                    }
+                    const R ee2 = xc * xc;
+                    const R ud2 = R(1) - R(2) * xc + xc * xc;
+                    R reik = (R(0.5) * rkm / R(in(a.z0, k)) / g / rho0j);
+                    if (xc > R(0.5))
+                        reik = fmin(reik, R(0.9) * log(R(in(a.dp0, k)) / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
+                    W(W_REI, k) = reik;
+                    ferk = reik * ee2;
+                    fdrk = reik * ud2;
+                    W(W_FER, k) = ferk;
+                    W(W_FDR, k) = fdrk;
+                    umfk = umf_km1 * exp(dpe * (ferk - fdrk));
+                    W(W_EMF, k) = R(0);
+                    R thluk, qtuk, uuk, vuk;
+                    const R ss_thl = W(A_SSTHL0, k), ss_qt = W(A_SSQT0, k), ss_u = W(A_SSU0, k), ss_v = W(A_SSV0, k);
+                    if (ferk * dpe < R(1.e-4)) {
+                        thluk = W(W_THLU, km1) + (thle + ss_thl * dpe / R(2) - W(W_THLU, km1)) * ferk * dpe;
+                        qtuk = W(W_QTU, km1) + (qte + ss_qt * dpe / R(2) - W(W_QTU, km1)) * ferk * dpe;
+                        uuk = W(W_UU, km1) + (ue + ss_u * dpe / R(2) - W(W_UU, km1)) * ferk * dpe - PGFc * ss_u * dpe;
+                        vuk = W(W_VU, km1) + (ve + ss_v * dpe / R(2) - W(W_VU, km1)) * ferk * dpe - PGFc * ss_v * dpe;
+                        for (int m = 1; m <= ncnst; ++m) {
+                            const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
+                            WT(1, m, k) = WT(1, m, km1) + (tre + WT(0, m, k) * dpe / R(2) - WT(1, m, km1)) * ferk * dpe;
+                        }
+                    } else {
+                        const R ef = exp(-ferk * dpe);
+                        thluk = (thle + ss_thl / ferk - ss_thl * dpe / R(2)) -
+                                (thle + ss_thl * dpe / R(2) - W(W_THLU, km1) + ss_thl / ferk) * ef;
+                        qtuk = (qte + ss_qt / ferk - ss_qt * dpe / R(2)) -
+                               (qte + ss_qt * dpe / R(2) - W(W_QTU, km1) + ss_qt / ferk) * ef;
+                        uuk = (ue + (R(1) - PGFc) * ss_u / ferk - ss_u * dpe / R(2)) -
+                              (ue + ss_u * dpe / R(2) - W(W_UU, km1) + (R(1) - PGFc) * ss_u / ferk) * ef;
+                        vuk = (ve + (R(1) - PGFc) * ss_v / ferk - ss_v * dpe / R(2)) -
+                              (ve + ss_v * dpe / R(2) - W(W_VU, km1) + (R(1) - PGFc) * ss_v / ferk) * ef;
+                        for (int m = 1; m <= ncnst; ++m) {
+                            const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
+                            const R sst = WT(0, m, k);
+                            WT(1, m, k) = (tre + sst / ferk - sst * dpe / R(2)) -
+                                          (tre + sst * dpe / R(2) - WT(1, m, km1) + sst / ferk) * ef;
+                        }
+                    }
+                    if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if ((R(qlj) + R(qij)) > criqc) {
+                        const R qlr = R(qlj), qir = R(qij);
+                        const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
+                        const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
+                        qtuk = qtuk - exql - exqi;
+                        thluk = thluk + (xlv / cp / W(A_EXNS0, k)) * exql + (xls / cp / W(A_EXNS0, k)) * exqi;
+                        W(W_DWTEN, k) = exql;
+                        W(W_DITEN, k) = exqi;
+                    } else {
+                        W(W_DWTEN, k) = R(0);
+                        W(W_DITEN, k) = R(0);
+                    }
+                    W(W_THLU, k) = thluk;
+                    W(W_QTU, k) = qtuk;
+                    W(W_UU, k) = uuk;
+                    W(W_VU, k) = vuk;
+                    out(a.umf, k) = (double)umfk;
+                    if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    const R thvuk = thv_of(th, qvj, qlj, qij);
+                    W(W_THVU, k) = thvuk;
+                    bogbot = rbuoy * (W(W_THVU, km1) / thvebot - R(1));
+                    bogtop = rbuoy * (thvuk / W(A_THV0TOP, k) - R(1));
+                    const R delbog = bogtop - bogbot;
+                    drage = ferk * (R(1) + rdrag);
+                    const R expfac = exp(R(-2) * drage * dpe);
+                    wtwb = wtw;
+                    if (drage * dpe > R(1.e-3))
+                        wtw = wtw * expfac + (delbog + (R(1) - expfac) * (bogbot + delbog / (R(-2) * drage * dpe))) / (rho0j * drage);
+                    else
+                        wtw = wtw + dpe * (bogbot + bogtop) / rho0j;
+                    if (wtw > R(0)) {
+                        thlue = R(0.5) * (W(W_THLU, km1) + thluk);
+                        qtue = R(0.5) * (W(W_QTU, km1) + qtuk);
+                        wue = R(0.5) * sqrt(fmax(wtwb + wtw, R(0)));
+                    } else {
+                        break;   // go to 111
+                    }
+                }
+                // 111 continue
+                if (umfk == R(0)) wtw = R(-1);
+                if (bogtop > R(0) && wtw > R(0)) kbup = k;
+                if (wtw <= R(0)) { kpen = k; break; }   // go to 45
+                {
+                    const R wuk = sqrt(wtw);
+                    out(a.wu, k) = (double)wuk;
+                    if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
+                    const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (W(A_THV0BOT, k + 1) + W(A_THV0TOP, k)) * W(A_EXNS0, k));
+                    R ufrck = umfk / (rhos0j * wuk);
+                    if (ufrck > rmaxfrac) {
+                        limit_ufrc = 1;
+                        ufrck = rmaxfrac;
+                        umfk = rmaxfrac * rhos0j * wuk;
+                        out(a.umf, k) = (double)umfk;
+                        W(W_FDR, k) = ferk - log(umfk / umf_km1) / dpe;
+                    }
+                    W(W_UFRC, k) = ufrck;
+                }
+                pe = R(in(a.p0, k + 1));
+                dpe = R(in(a.dp0, k + 1));
+                exne = W(A_EXN0, k + 1);
+                thvebot = W(A_THV0BOT, k + 1);
+                thle = W(A_THL0, k + 1);
+                qte = W(A_QT0, k + 1);
+                ue = R(in(a.u0, k + 1));
+                ve = R(in(a.v0, k + 1));
+                ktr = k + 1;
+                tr_mid = true;
+            }
+            // 45 continue
+            if (drage == R(0)) {
+                const R aquad = (bogtop - bogbot) / R(in(a.ps0, kpen) - in(a.ps0, kpen - 1));
+                const R bquad = R(2) * bogbot;
+                const R wuk = R(out(a.wu, kpen - 1));
+                const R cquad = -(wuk * wuk) * rho0j;
+                R xc1 = 0, xc2 = 0;
+                int status;
+                roots(aquad, bquad, cquad, xc1, xc2, status);
+                const R dpk = R(in(a.dp0, kpen));
+                if (status == 0) {
+                    if (xc1 <= R(0) && xc2 <= R(0)) {
+                        ppen = fmax(xc1, xc2);
+                        ppen = fmin(R(0), fmax(-dpk, ppen));
+                    } else if (xc1 > R(0) && xc2 > R(0)) {
+                        ppen = -dpk;
+                    } else {
+                        ppen = fmin(xc1, xc2);
+                        ppen = fmin(R(0), fmax(-dpk, ppen));
+                    }
+                } else {
+                    ppen = -dpk;
+                }
+            } else {
+                ppen = compute_ppen(wtwb, drage, bogbot, bogtop, rho0j, R(in(a.dp0, kpen)));
+            }
+            if (ppen == -R(in(a.dp0, kpen)) || ppen == R(0)) limit_ppen = 1;
+            {
+                const R ferp = W(W_FER, kpen), thl0p = W(A_THL0, kpen), ssthlp = W(A_SSTHL0, kpen);
+                const R qt0p = W(A_QT0, kpen), ssqtp = W(A_SSQT0, kpen);
+                if (ferp * (-ppen) < R(1.e-4)) {
+                    thlu_top = W(W_THLU, kpen - 1) + (thl0p + ssthlp * (-ppen) / R(2) - W(W_THLU, kpen - 1)) * ferp * (-ppen);
+                    qtu_top = W(W_QTU, kpen - 1) + (qt0p + ssqtp * (-ppen) / R(2) - W(W_QTU, kpen - 1)) * ferp * (-ppen);
+                } else {
+                    const R ef = exp(-ferp * (-ppen));
+                    thlu_top = (thl0p + ssthlp / ferp - ssthlp * (-ppen) / R(2)) -
+                               (thl0p + ssthlp * (-ppen) / R(2) - W(W_THLU, kpen - 1) + ssthlp / ferp) * ef;
+                    qtu_top = (qt0p + ssqtp / ferp - ssqtp * (-ppen) / R(2)) -
+                              (qt0p + ssqtp * (-ppen) / R(2) - W(W_QTU, kpen - 1) + ssqtp / ferp) * ef;
+                }
+            }
+            {
+                const R ptop = R(in(a.ps0, kpen - 1)) + ppen;
+                if (T.conden(ptop, thlu_top, qtu_top, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                const R exntop = R(pow((double)ptop / C.p00, C.rovcp));
+                if ((R(qlj) + R(qij)) > criqc) {
+                    const R qlr = R(qlj), qir = R(qij);
+                    const R dw = ((qlr + qir) - criqc) * qlr / (qlr + qir);
+                    const R di = ((qlr + qir) - criqc) * qir / (qlr + qir);
+                    W(W_DWTEN, kpen) = dw;
+                    W(W_DITEN, kpen) = di;
+                    qtu_top = qtu_top - dw - di;
+                    thlu_top = thlu_top + (xlv / cp / exntop) * dw + (xls / cp / exntop) * di;
+                } else {
+                    W(W_DWTEN, kpen) = R(0);
+                    W(W_DITEN, kpen) = R(0);
+                }
+                const R rhos0j = R(in(a.ps0, kpen - 1)) / (r * R(0.5) * (W(A_THV0BOT, kpen) + W(A_THV0TOP, kpen - 1)) * W(A_EXNS0, kpen - 1));
+                cush = (double)(R(in(a.zs0, kpen - 1)) - ppen / rhos0j / g);
+                scaleh = R(cush);
+            }
+        }
+
+        // ---- S12 forcedCu filter (uwshcu.F90:2964-2992) ----
+        if (kbup == krel) { exit_code = EX_CUFILTER; goto L333; }
+        for (int k = kpen; k <= mkx; ++k) { out(a.umf, k) = 0.0; W(W_EMF, k) = R(0); W(W_UFRC, k) = R(0); }
+        for (int k = kpen + 1; k <= mkx; ++k) { W(W_DWTEN, k) = R(0); W(W_DITEN, k) = R(0); W(W_FER, k) = R(0); W(W_FDR, k) = R(0); }
+
+        // ---- S13 penetrative entrainment (uwshcu.F90:3055-3183) ----
+        for (int k = 0; k <= mkx; ++k) {
+            W(W_THLUE, k) = W(W_THLU, k);
+            W(W_QTUE, k) = W(W_QTU, k);
+            W(W_UUE, k) = W(W_UU, k);
+            W(W_VUE, k) = W(W_VU, k);
+            for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = WT(1, m, k);
+        }
+        for (int k = kpen - 1; k >= kbup; --k) {
+            const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (W(A_THV0BOT, k + 1) + W(A_THV0TOP, k)) * W(A_EXNS0, k));
+            const R umfk = R(out(a.umf, k));
+            if (k == kpen - 1) {
+                const R x = umfk * ppen * W(W_REI, kpen) * rpen;
+                const R lim2 = R(-0.9) * R(in(a.dp0, kpen)) / g / dt;
+                if (x < R(-0.1) * rhos0j) limit_emf = 1;
+                if (x < lim2) limit_emf = 1;
+                W(W_EMF, k) = fmax(fmax(x, R(-0.1) * rhos0j), lim2);
+                const R dpk = R(in(a.ps0, k) - in(a.p0, kpen));
+                W(W_THLUE, k) = W(A_THL0, kpen) + W(A_SSTHL0, kpen) * dpk;
+                W(W_QTUE, k) = W(A_QT0, kpen) + W(A_SSQT0, kpen) * dpk;
+                W(W_UUE, k) = R(in(a.u0, kpen)) + W(A_SSU0, kpen) * dpk;
+                W(W_VUE, k) = R(in(a.v0, kpen)) + W(A_SSV0, kpen) * dpk;
+                for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = R(trin(m, kpen)) + WT(0, m, kpen) * dpk;
+            } else {
+                const R emf1 = W(W_EMF, k + 1);
+                const R x = emf1 - umfk * R(in(a.dp0, k + 1)) * W(W_REI, k + 1) * rpen;
+                const R lim2 = R(-0.9) * R(in(a.dp0, k + 1)) / g / dt;
+                if (x < R(-0.1) * rhos0j) limit_emf = 1;
+                if (x < lim2) limit_emf = 1;
+                const R emfk = fmax(fmax(x, R(-0.1) * rhos0j), lim2);
+                W(W_EMF, k) = emfk;
+                if (fabs(emfk) > fabs(emf1)) {
+                    W(W_THLUE, k) = (W(W_THLUE, k + 1) * emf1 + W(A_THL0, k + 1) * (emfk - emf1)) / emfk;
+                    W(W_QTUE, k) = (W(W_QTUE, k + 1) * emf1 + W(A_QT0, k + 1) * (emfk - emf1)) / emfk;
+                    W(W_UUE, k) = (W(W_UUE, k + 1) * emf1 + R(in(a.u0, k + 1)) * (emfk - emf1)) / emfk;
+                    W(W_VUE, k) = (W(W_VUE, k + 1) * emf1 + R(in(a.v0, k + 1)) * (emfk - emf1)) / emfk;
+                    for (int m = 1; m <= ncnst; ++m)
+                        WT(2, m, k) = (WT(2, m, k + 1) * emf1 + R(trin(m, k + 1)) * (emfk - emf1)) / emfk;
+                } else {
+                    W(W_THLUE, k) = W(A_THL0, k + 1);
+                    W(W_QTUE, k) = W(A_QT0, k + 1);
+                    W(W_UUE, k) = R(in(a.u0, k + 1));
+                    W(W_VUE, k) = R(in(a.v0, k + 1));
+                    for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = R(trin(m, k + 1));
+                }
+            }
+        }
+
+        // ---- S14 fluxes (uwshcu.F90:3255-3401) ----
+        {
+            // fluxbelowinv (uwshcu.F90:5030-5094) for one scalar; writes xflx(0..kinv-1) via put
+            auto fluxbelowinv = [&](R xsrc, R xmean, R xtopin, R xbotin, auto put) {
+                const R dp = R(in(a.ps0, kinv - 1) - in(a.ps0, kinv));
+                R xbot, xtop;
+                if (fabs(xbotin - xtopin) <= R(1.e-13)) { xbot = xbotin - R(1.e-13); xtop = xtopin + R(1.e-13); }
+                else { xbot = xbotin; xtop = xtopin; }
+                const R xtop_ori = xtop, xbot_ori = xbot;
+                const R rcbmf = (cbmf * g * dt) / dp;
+                R rpeff = (xmean - xtop) / (xbot - xtop);
+                rpeff = fmin(fmax(R(0), rpeff), R(1));
+                if (rpeff == R(0) || rpeff == R(1)) { xbot = xmean; xtop = xmean; }
+                const R rr = rpeff / rcbmf;
+                const R pinv = R(in(a.ps0, kinv - 1)) - rpeff * dp;
+                const R ps00 = R(in(a.ps0, 0));
+                for (int k = 0; k <= kinv - 1; ++k) {
+                    R x = cbmf * (xsrc - xbot) * (ps00 - R(in(a.ps0, k))) / (ps00 - pinv);
+                    if (k == kinv - 1 && rr <= R(1)) x = x - (R(1) - rr) * cbmf * (xtop_ori - xbot_ori);
+                    put(k, x);
+                }
+            };
+            const R dtop = R(in(a.ps0, kinv) - in(a.p0, kinv + 1));
+            const R dbot = R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
+            fluxbelowinv(R(qtsrc), W(A_QT0, kinv), W(A_QT0, kinv + 1) + W(A_SSQT0, kinv + 1) * dtop,
+                         W(A_QT0, kinv - 1) + W(A_SSQT0, kinv - 1) * dbot,
+                         [&](int k, R x) { out(a.qtflx, k) = (double)x; });
+            fluxbelowinv(R(thlsrc), W(A_THL0, kinv), W(A_THL0, kinv + 1) + W(A_SSTHL0, kinv + 1) * dtop,
+                         W(A_THL0, kinv - 1) + W(A_SSTHL0, kinv - 1) * dbot,
+                         [&](int k, R x) { out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * x); });
+            fluxbelowinv(R(usrc), R(in(a.u0, kinv)), R(in(a.u0, kinv + 1)) + W(A_SSU0, kinv + 1) * dtop,
+                         R(in(a.u0, kinv - 1)) + W(A_SSU0, kinv - 1) * dbot,
+                         [&](int k, R x) { W(W_UFLX, k) = x; });
+            fluxbelowinv(R(vsrc), R(in(a.v0, kinv)), R(in(a.v0, kinv + 1)) + W(A_SSV0, kinv + 1) * dtop,
+                         R(in(a.v0, kinv - 1)) + W(A_SSV0, kinv - 1) * dbot,
+                         [&](int k, R x) { W(W_VFLX, k) = x; });
+            for (int m = 1; m <= ncnst; ++m) {
+                fluxbelowinv(R(trin(m, 1)), R(trin(m, kinv)), R(trin(m, kinv + 1)) + WT(0, m, kinv + 1) * dtop,
+                             R(trin(m, kinv - 1)) + WT(0, m, kinv - 1) * dbot,
+                             [&](int k, R x) { WT(3, m, k) = x; });
+            }
+        }
+        {
+            R uplus = R(0), vplus = R(0);
+            for (int k = kinv; k <= krel - 1; ++k) {
+                const int kp1 = k + 1;
+                const R d = R(in(a.ps0, k) - in(a.p0, kp1));
+                out(a.qtflx, k) = (double)(cbmf * (R(qtsrc) - (W(A_QT0, kp1) + W(A_SSQT0, kp1) * d)));
+                out(a.slflx, k) = (double)(cbmf * (R(thlsrc) - (W(A_THL0, kp1) + W(A_SSTHL0, kp1) * d)) * cp * W(A_EXNS0, k));
+                uplus = uplus + PGFc * W(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                vplus = vplus + PGFc * W(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                W(W_UFLX, k) = cbmf * (R(usrc) + uplus - (R(in(a.u0, kp1)) + W(A_SSU0, kp1) * d));
+                W(W_VFLX, k) = cbmf * (R(vsrc) + vplus - (R(in(a.v0, kp1)) + W(A_SSV0, kp1) * d));
+                for (int m = 1; m <= ncnst; ++m)
+                    WT(3, m, k) = cbmf * (R(trin(m, 1)) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
+            }
+        }
+        for (int k = krel; k <= kbup - 1; ++k) {
+            const int kp1 = k + 1;
+            const R d = R(in(a.ps0, k) - in(a.p0, kp1));
+            const R umfk = R(out(a.umf, k));
+            out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * umfk * (W(W_THLU, k) - (W(A_THL0, kp1) + W(A_SSTHL0, kp1) * d)));
+            out(a.qtflx, k) = (double)(umfk * (W(W_QTU, k) - (W(A_QT0, kp1) + W(A_SSQT0, kp1) * d)));
+            W(W_UFLX, k) = umfk * (W(W_UU, k) - (R(in(a.u0, kp1)) + W(A_SSU0, kp1) * d));
+            W(W_VFLX, k) = umfk * (W(W_VU, k) - (R(in(a.v0, kp1)) + W(A_SSV0, kp1) * d));
+            for (int m = 1; m <= ncnst; ++m)
+                WT(3, m, k) = umfk * (WT(1, m, k) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
+        }
+        for (int k = kbup; k <= kpen - 1; ++k) {
+            const R d = R(in(a.ps0, k) - in(a.p0, k));
+            const R emfk = W(W_EMF, k);
+            out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (W(A_THL0, k) + W(A_SSTHL0, k) * d)));
+            out(a.qtflx, k) = (double)(emfk * (W(W_QTUE, k) - (W(A_QT0, k) + W(A_SSQT0, k) * d)));
+            W(W_UFLX, k) = emfk * (W(W_UUE, k) - (R(in(a.u0, k)) + W(A_SSU0, k) * d));
+            W(W_VFLX, k) = emfk * (W(W_VUE, k) - (R(in(a.v0, k)) + W(A_SSV0, k) * d));
+            for (int m = 1; m <= ncnst; ++m)
+                WT(3, m, k) = emfk * (WT(2, m, k) - (R(trin(m, k)) + WT(0, m, k) * d));
+        }
+
+        // ---- S15 subsidence condensate sinks (uwshcu.F90:3416-3476) ----
+        {
+            const R ps00 = R(in(a.ps0, 0)), psinv = R(in(a.ps0, kinv - 1));
+            auto uemf = [&](int k) -> R {   // uwshcu.F90:3416-3422
+                if (k <= kinv - 2) return cbmf * (ps00 - R(in(a.ps0, k))) / (ps00 - psinv);
+                if (k <= krel - 1) return cbmf;
+                if (k <= kbup - 1) return R(out(a.umf, k));
+                if (k <= kpen - 1) return W(W_EMF, k);
+                return R(0);
+            };
+            R uemf_km1 = uemf(0);
+            for (int k = 1; k <= kpen; ++k) {
+                const R uemf_k = uemf(k);
+                const R comsub = R(0.5) * (uemf_k + uemf_km1);
+                uemf_km1 = uemf_k;
+                R thlten_sub, qtten_sub, qlten_sub, qiten_sub, nlten_sub, niten_sub;
+                if (comsub >= R(0)) {
+                    if (k == mkx) {
+                        thlten_sub = qtten_sub = qlten_sub = qiten_sub = nlten_sub = niten_sub = R(0);
+                    } else {
+                        const R dp = R(in(a.p0, k) - in(a.p0, k + 1));
+                        thlten_sub = g * comsub * (W(A_THL0, k + 1) - W(A_THL0, k)) / dp;
+                        qtten_sub = g * comsub * (W(A_QT0, k + 1) - W(A_QT0, k)) / dp;
+                        qlten_sub = g * comsub * (R(in(a.ql0, k + 1)) - R(in(a.ql0, k))) / dp;
+                        qiten_sub = g * comsub * (R(in(a.qi0, k + 1)) - R(in(a.qi0, k))) / dp;
+                        nlten_sub = g * comsub * (trz(ixl, k + 1) - trz(ixl, k)) / dp;
+                        niten_sub = g * comsub * (trz(ixi, k + 1) - trz(ixi, k)) / dp;
+                    }
+                } else {
+                    if (k == 1) {
+                        thlten_sub = qtten_sub = qlten_sub = qiten_sub = nlten_sub = niten_sub = R(0);
+                    } else {
+                        const R dp = R(in(a.p0, k - 1) - in(a.p0, k));
+                        thlten_sub = g * comsub * (W(A_THL0, k) - W(A_THL0, k - 1)) / dp;
+                        qtten_sub = g * comsub * (W(A_QT0, k) - W(A_QT0, k - 1)) / dp;
+                        qlten_sub = g * comsub * (R(in(a.ql0, k)) - R(in(a.ql0, k - 1))) / dp;
+                        qiten_sub = g * comsub * (R(in(a.qi0, k)) - R(in(a.qi0, k - 1))) / dp;
+                        nlten_sub = g * comsub * (trz(ixl, k) - trz(ixl, k - 1)) / dp;
+                        niten_sub = g * comsub * (trz(ixi, k) - trz(ixi, k - 1)) / dp;
+                    }
+                }
+                const R thl_prog = W(A_THL0, k) + thlten_sub * dt;
+                const R qt_prog = fmax(W(A_QT0, k) + qtten_sub * dt, R(1.e-12));
+                if (T.conden(in(a.p0, k), thl_prog, qt_prog, th, qvj, qlj, qij)) { exit_code = EX_SUBCONDEN; goto L333; }
+                W(W_QLSINK, k) = fmax(qlten_sub, -R(in(a.ql0, k)) / dt);
+                W(W_QISINK, k) = fmax(qiten_sub, -R(in(a.qi0, k)) / dt);
+                W(W_NLSINK, k) = fmax(nlten_sub, -trz(ixl, k) / dt);
+                W(W_NISINK, k) = fmax(niten_sub, -trz(ixi, k) / dt);
+            }
+        }
+
+        // ---- S16 tendencies (uwshcu.F90:3488-3794) ----
+        for (int k = 1; k <= kpen; ++k) {
+            const R gdp = g / R(in(a.dp0, k));
+            out(a.uten, k) = (double)((W(W_UFLX, k - 1) - W(W_UFLX, k)) * g / R(in(a.dp0, k)));
+            out(a.vten, k) = (double)((W(W_VFLX, k - 1) - W(W_VFLX, k)) * g / R(in(a.dp0, k)));
+            (void)gdp;
+        }
+        R rliq = R(0), rainflx = R(0), snowflx = R(0);
+        {
+            auto uf = [&](int k) -> R { return R(in(a.u0, k)) + R(out(a.uten, k)) * dt; };
+            auto vf = [&](int k) -> R { return R(in(a.v0, k)) + R(out(a.vten, k)) * dt; };
+            R qlubelow = R(0), qiubelow = R(0);
+            for (int k = 1; k <= kpen; ++k) {
+                const int km1 = k - 1;
+                const R dp0k = R(in(a.dp0, k));
+                const R umf_km1 = R(out(a.umf, k - 1)), umf_k = R(out(a.umf, k));
+                const R dwk = W(W_DWTEN, k) * R(0.5) * (umf_km1 + umf_k) * g / dp0k;
+                const R dik = W(W_DITEN, k) * R(0.5) * (umf_km1 + umf_k) * g / dp0k;
+                W(W_DWTEN, k) = dwk;
+                W(W_DITEN, k) = dik;
+                const R qrk = frc_rasn * dwk, qsk = frc_rasn * dik;
+                out(a.qrten, k) = (double)qrk;
+                out(a.qsten, k) = (double)qsk;
+                rainflx = rainflx + qrk * dp0k / g;
+                snowflx = snowflx + qsk * dp0k / g;
+                R slten = (R(out(a.slflx, km1)) - R(out(a.slflx, k))) * g / dp0k;
+                const R u0k = R(in(a.u0, k)), v0k = R(in(a.v0, k));
+                if (k == 1) {
+                    slten = slten - g / R(4) / dp0k *
+                                        (W(W_UFLX, k) * (uf(k + 1) - uf(k) + R(in(a.u0, k + 1)) - u0k) +
+                                         W(W_VFLX, k) * (vf(k + 1) - vf(k) + R(in(a.v0, k + 1)) - v0k));
+                } else if (k >= 2 && k <= kpen - 1) {
+                    slten = slten - g / R(4) / dp0k *
+                                        (W(W_UFLX, k) * (uf(k + 1) - uf(k) + R(in(a.u0, k + 1)) - u0k) +
+                                         W(W_UFLX, k - 1) * (uf(k) - uf(k - 1) + u0k - R(in(a.u0, k - 1))) +
+                                         W(W_VFLX, k) * (vf(k + 1) - vf(k) + R(in(a.v0, k + 1)) - v0k) +
+                                         W(W_VFLX, k - 1) * (vf(k) - vf(k - 1) + v0k - R(in(a.v0, k - 1))));
+                } else if (k == kpen) {
+                    slten = slten - g / R(4) / dp0k *
+                                        (W(W_UFLX, k - 1) * (uf(k) - uf(k - 1) + u0k - R(in(a.u0, k - 1))) +
+                                         W(W_VFLX, k - 1) * (vf(k) - vf(k - 1) + v0k - R(in(a.v0, k - 1))));
+                }
+                const R qtten = (R(out(a.qtflx, km1)) - R(out(a.qtflx, k))) * g / dp0k;
+
+                R qlu_mid, qiu_mid, qlr, qir;
+                if (k < krel || k > kpen) {
+                    qlu_mid = R(0); qiu_mid = R(0); qlr = R(0); qir = R(0);
+                } else if (k == krel) {
+                    if (T.conden(prel, W(W_THLU, krel - 1), W(W_QTU, krel - 1), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    qlubelow = R(qlj);
+                    qiubelow = R(qij);
+                    if (T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    qlr = R(qlj); qir = R(qij);
+                    const R fr = (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    qlu_mid = R(0.5) * (qlubelow + qlr) * (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    qiu_mid = R(0.5) * (qiubelow + qir) * (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    (void)fr;
+                } else if (k == kpen) {
+                    if (T.conden(R(in(a.ps0, k - 1)) + ppen, thlu_top, qtu_top, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    qlr = R(qlj); qir = R(qij);
+                    qlu_mid = R(0.5) * (qlubelow + qlr) * (-ppen) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    qiu_mid = R(0.5) * (qiubelow + qir) * (-ppen) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                } else {
+                    if (T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    qlr = R(qlj); qir = R(qij);
+                    qlu_mid = R(0.5) * (qlubelow + qlr);
+                    qiu_mid = R(0.5) * (qiubelow + qir);
+                }
+                qlubelow = qlr;
+                qiubelow = qir;
+
+                R qc_l = (R(1) - frc_rasn) * dwk;
+                R qc_i = (R(1) - frc_rasn) * dik;
+                R qc_lm, qc_im, nc_lm, nc_im;
+                const R ql0k = R(in(a.ql0, k)), qi0k = R(in(a.qi0, k));
+                if (k <= kbup) {
+                    const R fdrk = W(W_FDR, k);
+                    qc_l = qc_l + g * R(0.5) * (umf_km1 + umf_k) * fdrk * qlu_mid;
+                    qc_i = qc_i + g * R(0.5) * (umf_km1 + umf_k) * fdrk * qiu_mid;
+                    qc_lm = -g * R(0.5) * (umf_km1 + umf_k) * fdrk * ql0k;
+                    qc_im = -g * R(0.5) * (umf_km1 + umf_k) * fdrk * qi0k;
+                    nc_lm = -g * R(0.5) * (umf_km1 + umf_k) * fdrk * trz(ixl, k);
+                    nc_im = -g * R(0.5) * (umf_km1 + umf_k) * fdrk * trz(ixi, k);
+                } else {
+                    qc_lm = R(0); qc_im = R(0); nc_lm = R(0); nc_im = R(0);
+                }
+                if (k == kbup) {
+                    const R dps = R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    qc_l = qc_l + g * umf_k * qlr / dps;
+                    qc_i = qc_i + g * umf_k * qir / dps;
+                    qc_lm = qc_lm - g * umf_k * ql0k / dps;
+                    qc_im = qc_im - g * umf_k * qi0k / dps;
+                    nc_lm = nc_lm - g * umf_k * trz(ixl, k) / dps;
+                    nc_im = nc_im - g * umf_k * trz(ixi, k) / dps;
+                    double ql_e, qi_e;
+                    if (T.conden(in(a.p0, k), W(W_THLUE, k), W(W_QTUE, k), th, qvj, ql_e, qi_e)) { exit_code = EX_KBUPCONDEN; goto L333; }
+                    const R nl_e = (ql_e > 0.0 && has_nl) ? WT(2, ixl, k) : R(0);
+                    const R ni_e = (qi_e > 0.0 && has_ni) ? WT(2, ixi, k) : R(0);
+                    const R emfk = W(W_EMF, k);
+                    qc_lm = qc_lm - g * emfk * (R(ql_e) - ql0k) / dps;
+                    qc_im = qc_im - g * emfk * (R(qi_e) - qi0k) / dps;
+                    nc_lm = nc_lm - g * emfk * (nl_e - trz(ixl, k)) / dps;
+                    nc_im = nc_im - g * emfk * (ni_e - trz(ixi, k)) / dps;
+                }
+                const R qltenk = qc_l + frc_rasn * dwk + (fmax(R(0), ql0k + (qc_lm + W(W_QLSINK, k)) * dt) - ql0k) / dt;
+                const R qitenk = qc_i + frc_rasn * dik + (fmax(R(0), qi0k + (qc_im + W(W_QISINK, k)) * dt) - qi0k) / dt;
+                if (has_nl) trout(ixl, k) = (double)fmax(nc_lm + W(W_NLSINK, k), -trz(ixl, k) / dt);
+                if (has_ni) trout(ixi, k) = (double)fmax(nc_im + W(W_NISINK, k), -trz(ixi, k) / dt);
+                out(a.qlten, k) = (double)qltenk;
+                out(a.qiten, k) = (double)qitenk;
+                out(a.qvten, k) = (double)(qtten - qltenk - qitenk);
+                out(a.sten, k) = (double)(slten + xlv * qltenk + xls * qitenk);
+                W(W_QCL, k) = qc_l;
+                W(W_QCI, k) = qc_i;
+                const R qck = qc_l + qc_i;
+                out(a.qc, k) = (double)qck;
+                rliq = rliq + qck * dp0k / g / R(1000);
+            }
+        }
+
+        // ---- S17 precipitation evaporation (uwshcu.F90:3808-3934) ----
+        R flxrain_k = R(0), flxsnow_k = R(0);   // fluxes at the top interface of layer k
+        {
+            R evpint_rain = R(0), evpint_snow = R(0);
+            const bool final_pass = (iter == 2);
+            if (final_pass) { out(a.flxprc1, mkx) = 0.0; out(a.flxsnow1, mkx) = 0.0; }
+            for (int k = mkx; k >= 1; --k) {
+                const R dp0k = R(in(a.dp0, k));
+                const R t0k = R(in(a.t0, k)), qv0k = R(in(a.qv0, k));
+                R snowmlt = (t0k > R(273.16)) ? fmax(R(0), flxsnow_k * g / dp0k) : R(0);
+                double es, qs;
+                T.qsat_es_qs(in(a.t0, k), in(a.p0, k), es, qs);
+                const R subsat = fmax((R(1) - qv0k / R(qs)), R(0));
+                R evprain = kevp * subsat * sqrt(flxrain_k + snowmlt * dp0k / g);
+                R evpsnow = kevp * subsat * sqrt(fmax(flxsnow_k - snowmlt * dp0k / g, R(0)));
+                const R evplimit = fmax(R(0), (W(W_QW0, k) - qv0k) / dt);
+                R evplimit_rain = fmin(evplimit, (flxrain_k + snowmlt * dp0k / g) * g / dp0k);
+                evplimit_rain = fmin(evplimit_rain, (rainflx - evpint_rain) * g / dp0k);
+                evprain = fmax(R(0), fmin(evplimit_rain, evprain));
+                R evplimit_snow = fmin(evplimit, fmax(flxsnow_k - snowmlt * dp0k / g, R(0)) * g / dp0k);
+                evplimit_snow = fmin(evplimit_snow, (snowflx - evpint_snow) * g / dp0k);
+                evpsnow = fmax(R(0), fmin(evplimit_snow, evpsnow));
+                if ((evprain + evpsnow) > evplimit) {
+                    const R tmp1 = evprain * evplimit / (evprain + evpsnow);
+                    const R tmp2 = evpsnow * evplimit / (evprain + evpsnow);
+                    evprain = tmp1;
+                    evpsnow = tmp2;
+                }
+                out(a.evapc, k) = (double)(evprain + evpsnow);
+                evpint_rain = evpint_rain + evprain * dp0k / g;
+                evpint_snow = evpint_snow + evpsnow * dp0k / g;
+                const R qrk = R(out(a.qrten, k)), qsk = R(out(a.qsten, k));
+                const R ntraprd = qrk - evprain + snowmlt;
+                const R ntsnprd = qsk - evpsnow - snowmlt;
+                R flxrain_km1 = flxrain_k + ntraprd * dp0k / g;
+                R flxsnow_km1 = flxsnow_k + ntsnprd * dp0k / g;
+                flxrain_km1 = fmax(flxrain_km1, R(0));
+                flxsnow_km1 = fmax(flxsnow_km1, R(0));
+                flxrain_k = flxrain_km1;
+                flxsnow_k = flxsnow_km1;
+                if (final_pass) {
+                    out(a.flxprc1, k - 1) = (double)(flxrain_km1 + flxsnow_km1);
+                    out(a.flxsnow1, k - 1) = (double)flxsnow_km1;
+                }
+                const R qlt = R(out(a.qlten, k)) - qrk;
+                const R qit = R(out(a.qiten, k)) - qsk;
+                const R qvt = R(out(a.qvten, k)) + evprain + evpsnow;
+                out(a.qlten, k) = (double)qlt;
+                out(a.qiten, k) = (double)qit;
+                out(a.qvten, k) = (double)qvt;
+                out(a.sten, k) = (double)(R(out(a.sten, k)) - xlv * evprain - xls * evpsnow - (xls - xlv) * snowmlt);
+            }
+        }
+        const R precip = (flxrain_k + flxsnow_k) / R(1000);
+        const R snow = flxsnow_k / R(1000);
+
+        // ---- S18 reserved condensate (uwshcu.F90:3966-3981) ----
+        for (int k = 1; k <= kpen; ++k) {
+            out(a.qlten, k) = (double)(R(out(a.qlten, k)) - W(W_QCL, k));
+            out(a.qiten, k) = (double)(R(out(a.qiten, k)) - W(W_QCI, k));
+            out(a.sten, k) = (double)(R(out(a.sten, k)) - (xls - xlv) * W(W_QCI, k));
+        }
+
+        // ---- S19 positive moisture, FP64 (uwshcu.F90:3995-4001, 5096-5162) ----
+        {
+            const double qvmin = C.qmin[1], qlmin = C.qmin[2], qimin = C.qmin[3];
+            const double xlvd = C.xlv, xlsd = C.xls;
+            double dqv = 0.0;
+            double carry_qv = 0.0, carry_ten = 0.0;   // adjustments of qv(k-1) and qvten(k-1) from level k
+            bool have_carry = false;
+            for (int k = mkx; k >= 1; --k) {
+                double qvten = out(a.qvten, k), qlten = out(a.qlten, k), qiten = out(a.qiten, k), sten = out(a.sten, k);
+                double qv = in(a.qv0, k) + qvten * dtd;
+                double ql = in(a.ql0, k) + qlten * dtd;
+                double qi = in(a.qi0, k) + qiten * dtd;
+                if (have_carry) {
+                    qv = qv - carry_qv;
+                    qvten = qvten - carry_ten;
+                }
+                const double dql = fmax(0.0, 1.0 * qlmin - ql);
+                const double dqi = fmax(0.0, 1.0 * qimin - qi);
+                qlten = qlten + dql / dtd;
+                qiten = qiten + dqi / dtd;
+                qvten = qvten - (dql + dqi) / dtd;
+                sten = sten + xlvd * (dql / dtd) + xlsd * (dqi / dtd);
+                ql = ql + dql;
+                qi = qi + dqi;
+                qv = qv - dql - dqi;
+                dqv = fmax(0.0, 1.0 * qvmin - qv);
+                qvten = qvten + dqv / dtd;
+                qv = qv + dqv;
+                if (k != 1) {
+                    const double dpk = in(a.dp0, k), dpkm1 = in(a.dp0, k - 1);
+                    carry_qv = dqv * dpk / dpkm1;
+                    carry_ten = dqv * dpk / dpkm1 / dtd;
+                    have_carry = true;
+                }
+                qv = fmax(qv, qvmin);
+                out(a.qvten, k) = qvten;
+                out(a.qlten, k) = qlten;
+                out(a.qiten, k) = qiten;
+                out(a.sten, k) = sten;
+                W(W_QVSTAR, k) = R(qv);
+            }
+            if (dqv > 1.e-20) {
+                double sum = 0.0;
+                for (int k = 1; k <= mkx; ++k) {
+                    const double qv = (double)W(W_QVSTAR, k);
+                    if (qv > 2.0 * qvmin) sum = sum + qv * in(a.dp0, k);
+                }
+                const double aa = dqv * in(a.dp0, 1) / fmax(1.e-20, sum);
+                if (aa < 0.5) {
+                    for (int k = 1; k <= mkx; ++k) {
+                        const double qv = (double)W(W_QVSTAR, k);
+                        if (qv > 2.0 * qvmin) {
+                            const double dum = aa * qv;
+                            out(a.qvten, k) = out(a.qvten, k) - dum / dtd;
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- S20 constituent limiter (uwshcu.F90:4007-4076) ----
+        for (int m = 4; m <= ncnst; ++m) {
+            if (m == ixl || m == ixi) continue;
+            const R trmin = R(C.qmin[m]);
+            const bool wet = C.wet[m] != 0;
+            auto pdelx = [&](int k) -> R { return wet ? R(in(a.dp0, k)) : R(in(a.dpdry0, k)); };
+            W(W_TRFD, 0) = R(0);
+            for (int k = 1; k <= mkx - 1; ++k) {
+                const R dum = (R(trin(m, k)) - trmin) * pdelx(k) / g / dt + WT(3, m, k - 1) - WT(3, m, k) + W(W_TRFD, k - 1);
+                W(W_TRFD, k) = fmin(R(0), dum);
+            }
+            W(W_TRFD, mkx) = R(0);
+            W(W_TRFU, mkx) = R(0);
+            W(W_TRFU, 0) = R(0);
+            for (int k = mkx; k >= 2; --k) {
+                const R dum = (R(trin(m, k)) - trmin) * pdelx(k) / g / dt + WT(3, m, k - 1) - WT(3, m, k) +
+                              W(W_TRFD, k - 1) - W(W_TRFD, k) - W(W_TRFU, k);
+                W(W_TRFU, k - 1) = fmax(R(0), -dum);
+            }
+            for (int k = 1; k <= mkx; ++k) {
+                trout(m, k) = (double)((WT(3, m, k - 1) - WT(3, m, k) + W(W_TRFD, k - 1) - W(W_TRFD, k) +
+                                        W(W_TRFU, k - 1) - W(W_TRFU, k)) * g / pdelx(k));
+            }
+        }
+
+        // ---- S21 in-cloud diagnostics (uwshcu.F90:4092-4158) ----
+        {
+            if (T.conden(prel, W(W_THLU, krel - 1), W(W_QTU, krel - 1), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+            R qcubelow = R(qlj) + R(qij), qlubelow = R(qlj), qiubelow = R(qij);
+            for (int k = krel; k <= kpen; ++k) {
+                int idc;
+                if (k == kpen) idc = T.conden(R(in(a.ps0, k - 1)) + ppen, thlu_top, qtu_top, th, qvj, qlj, qij);
+                else idc = T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij);
+                if (idc) { exit_code = EX_CONDEN; goto L333; }
+                const R qlr = R(qlj), qir = R(qij);
+                R qcuk = R(0.5) * (qcubelow + qlr + qir);
+                R qluk = R(0.5) * (qlubelow + qlr);
+                R qiuk = R(0.5) * (qiubelow + qir);
+                R cufrck = (W(W_UFRC, k - 1) + W(W_UFRC, k));
+                if (k == krel) {
+                    cufrck = (ufrclcl + W(W_UFRC, k)) * (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                } else if (k == kpen) {
+                    cufrck = (W(W_UFRC, k - 1) + R(0)) * (-ppen) / R(in(a.ps0, k - 1) - in(a.ps0, k));
+                    if ((qlr + qir) > criqc) {
+                        qcuk = R(0.5) * (qcubelow + criqc);
+                        qluk = R(0.5) * (qlubelow + criqc * qlr / (qlr + qir));
+                        qiuk = R(0.5) * (qiubelow + criqc * qir / (qlr + qir));
+                    }
+                }
+                out(a.qcu, k) = (double)qcuk;
+                out(a.qlu, k) = (double)qluk;
+                out(a.qiu, k) = (double)qiuk;
+                out(a.cufrc, k) = (double)cufrck;
+                qcubelow = qlr + qir;
+                qlubelow = qlr;
+                qiubelow = qir;
+            }
+        }
+
+        // column scalars of this pass
+        a.precip[i] = (double)precip;
+        a.snow[i] = (double)snow;
+        a.cbmf[i] = (double)cbmf;
+        a.rliq[i] = (double)rliq;
+        a.cnt[i] = (double)(mkx + 1 - kpen);
+        a.cnb[i] = (double)(mkx + 1 - (krel - 1));
+
+        // ---- S22 provisional state and environment set B (uwshcu.F90:4169-4329) ----
+        if (iter == 1) {
+            cush_s = cush;
+            for (int k = 1; k <= mkx; ++k) {
+                const R qv0s = R(in(a.qv0, k)) + R(out(a.qvten, k)) * dt;
+                const R ql0s = R(in(a.ql0, k)) + R(out(a.qlten, k)) * dt;
+                const R qi0s = R(in(a.qi0, k)) + R(out(a.qiten, k)) * dt;
+                const R t0s = R(in(a.t0, k)) + R(out(a.sten, k)) * dt / cp;
+                const R qt0 = (qv0s + ql0s + qi0s);
+                W(B_QT0, k) = qt0;
+                W(B_THL0, k) = (t0s - xlv * ql0s / cp - xls * qi0s / cp) / W(A_EXN0, k);
+            }
+            slope([&](int k) { return W(B_THL0, k); }, B_SSTHL0);
+            slope([&](int k) { return W(B_QT0, k); }, B_SSQT0);
+            if (!interface_env(B_THL0)) { exit_code = EX_CONDEN; goto L333; }
+        }
+        }
+    }
+
+    // successful implicit-CIN pass
+    a.cush[i] = cush;
+    path = 2;
+    goto Ldone;
+
+L333:
+    write_exit(mkx);
+    path = 0;
+
+Ldone:
+    if (a.diag) {
+        const size_t L = (size_t)a.ldc;
+        int* d = a.diag + i;
+        d[0 * L] = (path == 0) ? exit_code : 0;
+        d[1 * L] = kinv; d[2 * L] = klcl; d[3 * L] = klfc; d[4 * L] = krel; d[5 * L] = kbup; d[6 * L] = kpen;
+        d[7 * L] = path; d[8 * L] = limit_ufrc; d[9 * L] = limit_ppen; d[10 * L] = limit_emf; d[11 * L] = exit_iter;
+    }
+}
+
+}  // namespace uwshcu
+#endif

@@ -1,0 +1,306 @@
+// uwshcu_thermo.h -- device thermodynamic primitives of the UW shallow-cumulus scheme.
+//
+// Each function follows the reference routine cited beside it.  The saturation
+// vapour pressure table lives in LDS (250 doubles per workgroup); every lookup is a
+// two-point gather from it.
+#ifndef UWSHCU_THERMO_H
+#define UWSHCU_THERMO_H
+
+#include <hip/hip_runtime.h>
+#include "uwshcu_types.h"
+
+// Every primitive is callable from host code too: the test suite builds the column
+// code for the CPU to check its logic against the oracle before it runs on a GPU.
+#define UWSHCU_HD __host__ __device__
+
+namespace uwshcu {
+
+// Record the first fatal condition (code, column) of a launch.
+UWSHCU_HD inline void record_error(int* err, int code, int col) {
+    int expected = 0;
+    if (__atomic_compare_exchange_n(&err[0], &expected, code, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) err[1] = col;
+}
+
+struct Thermo {
+    const uwshcu_consts* c;   // kernel-argument copy
+    const double* estbl;      // LDS copy of the table
+
+    UWSHCU_HD inline double exnf(double p) const { return pow(p / c->p00, c->rovcp); }  // uwshcu.F90:50-54
+
+    // wv_saturation.F90:64-80
+    UWSHCU_HD inline double estblf(double td) const {
+        double e = fmax(fmin(td, c->tmax), c->tmin);
+        double d = e - c->tmin;
+        int i = (int)d;                        // int(e-tmin)+1, 1-based -> i (0-based)
+        double ai = (double)i;                 // aint(e-tmin) for e >= tmin
+        return (c->tmin + ai - e + 1.0) * estbl[i] - (c->tmin + ai - e) * estbl[i + 1];
+    }
+
+    // vqsatd with len=1, es and qs only (wv_saturation.F90:530-546)
+    UWSHCU_HD inline void qsat_es_qs(double t, double p, double& es, double& qs) const {
+        const double omeps = 1.0 - c->epsqs;
+        es = estblf(t);
+        qs = c->epsqs * es / (p - omeps * es);
+        qs = fmin(1.0, qs);
+        if (qs < 0.0) { qs = 1.0; es = p; }
+    }
+
+    // vqsatd with len=1 including gam = (L/cp) dqs/dT (wv_saturation.F90:482-609)
+    UWSHCU_HD inline void qsat_full(double t, double p, double& es, double& qs, double& gam) const {
+        const double omeps = 1.0 - c->epsqs;
+        qsat_es_qs(t, p, es, qs);
+        double hlatsb, hlatvp, hltalt, desdt;
+        if (!c->icephs || c->ttrice == 0.0) {
+            hlatvp = c->hlatv - 2369.0 * (t - c->tmelt);
+            hlatsb = c->icephs ? c->hlatv + c->hlatf : c->hlatv;
+            hltalt = (t < c->tmelt) ? hlatsb : hlatvp;
+            desdt = hltalt * es / (c->rgasv * t * t);
+        } else {
+            const double trinv = 1.0 / c->ttrice;
+            const double tc = t - c->tmelt;
+            const bool lflg = (tc >= -c->ttrice && tc < 0.0);
+            const double weight = fmin(-tc * trinv, 1.0);
+            hlatsb = c->hlatv + weight * c->hlatf;
+            hlatvp = c->hlatv - 2369.0 * tc;
+            hltalt = (t < c->tmelt) ? hlatsb : hlatvp;
+            double tterm = 0.0;
+            if (lflg) tterm = c->pcf[0] + tc * (c->pcf[1] + tc * (c->pcf[2] + tc * (c->pcf[3] + tc * c->pcf[4])));
+            desdt = hltalt * es / (c->rgasv * t * t) + tterm * trinv;
+        }
+        gam = hltalt * qs * p * desdt / (c->cpws * es * (p - omeps * es));
+        if (qs == 1.0) gam = 0.0;
+    }
+
+    // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
+    UWSHCU_HD inline int conden(double p, double thl, double qt, double& th, double& qv, double& ql,
+                                          double& qi) const {
+        const double ex = exnf(p);
+        const double tc = thl * ex;
+        const double nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
+        const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
+        double temps = tc, es, qs;
+        qsat_es_qs(temps, p, es, qs);
+        double rvls = qs;
+        if (qs >= qt) {
+            qv = qt;
+            ql = 0.0;
+            qi = 0.0;
+            th = tc / ex;
+            return 0;
+        }
+        for (int it = 0; it < 10; ++it) {
+            temps = temps + ((tc - temps) * c->cp / leff + qt - rvls) /
+                            (c->cp / leff + c->ep2 * leff * rvls / c->r / temps / temps);
+            qsat_es_qs(temps, p, es, qs);
+            rvls = qs;
+        }
+        const double qc = fmax(qt - qs, 0.0);
+        qv = qt - qc;
+        ql = qc * (1.0 - nu);
+        qi = nu * qc;
+        th = temps / ex;
+        return (fabs((temps - (leff / c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
+    }
+
+    // uwshcu.F90:4863-4929
+    UWSHCU_HD double qsinvert(double qt, double thl, double psfc) const {
+        const double psmin = 100.0 * 100.0, dpsmax = 1.0;
+        double es, qs, gam;
+        const double Ti = thl * pow(psfc / c->p00, c->rovcp);
+        qsat_full(Ti, psfc, es, qs, gam);
+        const double rhi = qt / qs;
+        if (rhi <= 0.01) return psmin;
+        const double TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - log(rhi) / 2840.0);
+        const double PiLCL = TLCL / thl;
+        double ps = c->p00 * pow(PiLCL, 1.0 / c->rovcp);
+        for (int it = 0; it < 10; ++it) {
+            const double Pis = pow(ps / c->p00, c->rovcp);
+            const double Ts = thl * Pis;
+            qsat_full(Ts, ps, es, qs, gam);
+            const double err = qt - qs;
+            const double nu = fmax(fmin((268.0 - Ts) / 20.0, 1.0), 0.0);
+            const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
+            const double dlnqsdT = gam * (c->cp / leff) / qs;
+            const double dTdPis = thl;
+            const double dPisdps = c->rovcp * Pis / ps;
+            const double dlnqsdps = -1.0 / (ps - (1.0 - c->ep2) * es);
+            const double derrdps = -qs * (dlnqsdT * dTdPis * dPisdps + dlnqsdps);
+            const double dps = -err / derrdps;
+            ps = ps + dps;
+            if (ps < 0.0) return psmin;
+            if (fabs(dps) <= dpsmax) return ps;
+        }
+        return psmin;
+    }
+
+    // uwshcu.F90:4704-4715
+    UWSHCU_HD inline double single_cin(double pbot, double thv0bot, double ptop, double thv0top,
+                                                 double thvubot, double thvutop) const {
+        return ((1.0 - thvubot / thv0bot) + (1.0 - thvutop / thv0top)) * (pbot - ptop) /
+               (pbot / (c->r * thv0bot * exnf(pbot)) + ptop / (c->r * thv0top * exnf(ptop)));
+    }
+
+    // uwshcu.F90:4672-4702
+    UWSHCU_HD inline void getbuoy(double pbot, double thv0bot, double ptop, double thv0top,
+                                            double thvubot, double thvutop, double& plfc, double& cin) const {
+        if (thvubot > thv0bot && thvutop > thv0top) {
+            plfc = pbot;
+            return;
+        }
+        const double den = (pbot / (c->r * thv0bot * exnf(pbot)) + ptop / (c->r * thv0top * exnf(ptop)));
+        if (thvubot <= thv0bot && thvutop <= thv0top) {
+            cin = cin - ((thvubot / thv0bot - 1.0) + (thvutop / thv0top - 1.0)) * (pbot - ptop) / den;
+        } else if (thvubot > thv0bot && thvutop <= thv0top) {
+            const double frc = (thvutop / thv0top - 1.0) / ((thvutop / thv0top - 1.0) - (thvubot / thv0bot - 1.0));
+            cin = cin - (thvutop / thv0top - 1.0) * ((ptop + frc * (pbot - ptop)) - ptop) / den;
+        } else {
+            const double frc = (thvubot / thv0bot - 1.0) / ((thvubot / thv0bot - 1.0) - (thvutop / thv0top - 1.0));
+            plfc = pbot - frc * (pbot - ptop);
+            cin = cin - (thvubot / thv0bot - 1.0) * (pbot - plfc) / den;
+        }
+    }
+};
+
+// uwshcu.F90:4787-4829
+template <class R>
+UWSHCU_HD inline void roots(R a, R b, R c, R& r1, R& r2, int& status) {
+    status = 0;
+    if (a == R(0)) {
+        if (b == R(0)) status = 1;
+        else r1 = -c / b;
+        r2 = r1;
+    } else if (b == R(0)) {
+        if (a * c > R(0)) status = 2;
+        else r1 = sqrt(-c / a);
+        r2 = -r1;
+    } else if ((b * b - R(4) * a * c) < R(0)) {
+        status = 3;
+    } else {
+        const R q = R(-0.5) * (b + copysign(R(1), b) * sqrt(b * b - R(4) * a * c));
+        r1 = q / a;
+        r2 = c / q;
+    }
+}
+
+// uwshcu.F90:4931-4950
+UWSHCU_HD inline double compute_alpha(double del_CIN, double ke) {
+    double x0 = 0.0;
+    for (int it = 0; it < 10; ++it) {
+        const double e = exp(-x0 * ke * del_CIN);
+        x0 = x0 - (e - x0) / (-ke * del_CIN * e - 1.0);
+    }
+    return x0;
+}
+
+// uwshcu.F90:4952-4979 (erfc is the device libm's)
+template <class R>
+UWSHCU_HD inline R compute_mumin2(R mulcl, R rmaxfrac, R mulow) {
+    R x0 = mulow;
+    for (int it = 0; it < 10; ++it) {
+        const R ex = exp(-(x0 * x0));
+        const R ef = erfc(x0);
+        const R exf = ex / ef;
+        const R a = ex / R(2) / rmaxfrac;
+        const R b = mulcl * R(2.5066) / R(2);
+        const R f = R(0.5) * (exf * exf) - R(0.5) * (a * a) - (b * b);
+        const R fs = (R(2) * (exf * exf)) * (exf / sqrt(R(3.141592)) - x0) + (R(0.5) * x0 * (ex * ex)) / (rmaxfrac * rmaxfrac);
+        x0 = x0 - f / fs;
+    }
+    return x0;
+}
+
+// uwshcu.F90:4981-5028
+template <class R>
+UWSHCU_HD inline R compute_ppen(R wtwb, R D, R bogbot, R bogtop, R rho0j, R dpen) {
+    const R SB = (bogtop - bogbot) / dpen;
+    const R s00 = bogbot / rho0j - D * wtwb;
+    R x0;
+    if (D * dpen < R(1.e-8)) {
+        if (s00 >= R(0)) x0 = dpen;
+        else x0 = fmax(R(0), fmin(dpen, R(-0.5) * wtwb / s00));
+    } else {
+        x0 = (s00 >= R(0)) ? dpen : R(0);
+        for (int it = 0; it < 5; ++it) {
+            const R e = exp(R(-2) * D * x0);
+            const R f = e * (wtwb - (bogbot - SB / (R(2) * D)) / (D * rho0j)) + (SB * x0 + bogbot - SB / (R(2) * D)) / (D * rho0j);
+            R fs = R(-2) * D * e * (wtwb - (bogbot - SB / (R(2) * D)) / (D * rho0j)) + (SB) / (D * rho0j);
+            if (fs >= R(0)) fs = fmax(fs, R(1.e-10));
+            else fs = fmin(fs, R(-1.e-10));
+            x0 = x0 - f / fs;
+        }
+    }
+    return -fmax(R(0), fmin(dpen, x0));
+}
+
+// Wet-bulb point of findsp (uwshcu.F90:5246-5450).  A point's sequence of updates
+// depends only on its own inputs (converged points are masked, and the chunk-wide
+// early exit fires only when no unconverged point remains), so each (column,level)
+// is solved independently.  Returns the doit code: 0 not converged, 1 skipped,
+// 2 converged, 4 too cold; enerr is the relative enthalpy error of converged points.
+UWSHCU_HD inline int findsp_point(const Thermo& T, double q, double t, double p, double& qsp_out,
+                                            double& enerr) {
+    const uwshcu_consts* c = T.c;
+    const double omeps = 1.0 - c->epsqs, trinv = 1.0 / c->ttrice;
+    const double a1 = 7.5 * log(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
+    const double dttol = 1.e-4, dqtol = 1.e-4, tt0 = 273.15;
+    const double tlim = fmin(fmax(t, 173.0), 373.0);
+    double es = T.estblf(tlim);
+    const double denom = p - omeps * es;
+    double qs = c->epsqs * es / denom;
+    double tsp, qsp, enin, enout = 1.0;
+    int doit = 0;
+    if (p > 5.0 * es && qs > 0.0 && qs < 0.5) {
+        qs = fmin(c->epsqs * es / denom, 1.0);
+        const double tc = tlim - tt0;
+        const double weight = fmin(-tc * trinv, 1.0);
+        const double hlatsb = c->hlatv + weight * c->hlatf;
+        const double hlatvp = c->hlatv - 2369.0 * tc;
+        const double hltalt = (tlim < tt0) ? hlatsb : hlatvp;
+        enin = c->cpws * tlim + hltalt * q;
+        const double tmp = q - qs;
+        const double c1 = hltalt * c3;
+        const double c2 = (tlim + 36.0) * (tlim + 36.0);
+        const double r1b = c2 / (c2 + c1 * qs);
+        const double qvd = r1b * tmp;
+        tsp = tlim + ((hltalt / c->cpws) * qvd);
+        es = T.estblf(tsp);
+        qsp = fmin(c->epsqs * es / (p - omeps * es), 1.0);
+    } else {
+        doit = 1;
+        tsp = tlim;
+        qsp = q;
+        enin = 1.0;
+    }
+    for (int l = 0; l < 8 && doit == 0; ++l) {
+        es = T.estblf(tsp);
+        qs = fmin(c->epsqs * es / (p - omeps * es), 1.0);
+        const double tc = tsp - tt0;
+        const bool lflg = (tc >= -c->ttrice && tc < 0.0);
+        const double weight = fmin(-tc * trinv, 1.0);
+        const double hlatsb = c->hlatv + weight * c->hlatf;
+        const double hlatvp = c->hlatv - 2369.0 * tc;
+        const double hltalt = (tsp < tt0) ? hlatsb : hlatvp;
+        double tterm = 0.0;
+        if (lflg) tterm = c->pcf[0] + tc * (c->pcf[1] + tc * (c->pcf[2] + tc * (c->pcf[3] + tc * c->pcf[4])));
+        const double desdt = hltalt * es / (c->rgasv * tsp * tsp) + tterm * trinv;
+        const double dqsdt = (c->epsqs + omeps * qs) / (p - omeps * es) * desdt;
+        const double g = enin - (c->cpws * tsp + hltalt * qsp);
+        const double dgdt = -(c->cpws + hltalt * dqsdt);
+        const double t1 = tsp - g / dgdt;
+        const double dtr = fabs(t1 - tsp) / t1;
+        tsp = fmax(t1, c->tmin);
+        es = T.estblf(tsp);
+        const double q1 = fmin(c->epsqs * es / (p - omeps * es), 1.0);
+        const double dq = fabs(q1 - qsp) / fmax(q1, 1.e-12);
+        qsp = q1;
+        if (dtr < dttol && dq < dqtol) doit = 2;
+        enout = c->cpws * tsp + hltalt * qsp;
+        if (tsp < 174.16) doit = 4;
+    }
+    qsp_out = qsp;
+    enerr = (doit == 2) ? fabs((enin - enout) / (enin + enout)) : 0.0;
+    return doit;
+}
+
+}  // namespace uwshcu
+#endif

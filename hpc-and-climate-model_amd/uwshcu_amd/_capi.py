@@ -1,0 +1,119 @@
+"""ctypes binding of the C ABI in ``include/uwshcu.h`` (libuwshcu_hip.so).
+
+The library is built in-tree by ``uwshcu_amd.build.build()`` (hipcc for gfx950).
+Loading fails loudly when it is missing: there is no CPU path in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libuwshcu_hip.so")
+
+NCNST_MAX = 32
+ESTBL_LEN = 250
+NDIAG = 12
+
+FP64 = 0
+MIXED = 1
+
+ERRORS = {
+    0: "ok",
+    -1: "findsp not converging",
+    -2: "findsp enthalpy not conserved",
+    -3: "mu > mumin2 in the cloud-base closure",
+    -10: "invalid argument",
+    -11: "workspace allocation failed",
+    -12: "HIP runtime error",
+    -13: "no usable GPU",
+}
+
+
+class Consts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("xlv", "xlf", "xls", "cp", "zvir", "r", "g", "ep2", "p00", "rovcp",
+                                          "tmin", "tmax", "ttrice", "epsqs", "rgasv", "hlatf", "hlatv", "cpws",
+                                          "tmelt")] + [
+        ("icephs", C.c_int), ("ixnumliq", C.c_int), ("ixnumice", C.c_int), ("pad_", C.c_int),
+        ("pcf", C.c_double * 5),
+        ("qmin", C.c_double * (NCNST_MAX + 1)),
+        ("wet", C.c_int * (NCNST_MAX + 1)),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [("rpen", C.c_double), ("rle", C.c_double), ("rkm", C.c_double)]
+
+
+# Argument names of the forward entry points, after (ctx, precision, par, pcols, ncol,
+# pver, ncnst, dt), in the order of compute_uwshcu_inv (uwshcu.F90:251-265).
+INPUT_NAMES = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke",
+               "cldfrc", "concldfrc", "pblh")
+OUTPUT_NAMES = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "qiten", "sten", "uten",
+                "vten", "trten", "qrten", "qsten", "precip", "snow", "evapc", "cufrc", "qcu", "qlu", "qiu",
+                "cbmf", "qc", "rliq", "cnt", "cnb")
+# full ordered list of the array arguments
+ARRAY_ARGS = INPUT_NAMES + ("cush",) + OUTPUT_NAMES + ("dpdry0", "wu")
+
+SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params", "uwshcu_ctx_create",
+           "uwshcu_ctx_destroy", "uwshcu_ctx_set_workspace_limit", "uwshcu_workspace_bytes_per_column",
+           "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
+           "uwshcu_set_dropin_params", "uwshcu_dropin_last_status")
+
+_lib = None
+
+
+class UwshcuError(RuntimeError):
+    def __init__(self, code, col=-1, what="uwshcu"):
+        self.code, self.col = code, col
+        super().__init__(f"{what}: {ERRORS.get(code, 'error')} (code {code}, column {col})")
+
+
+def lib():
+    """Load libuwshcu_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with uwshcu_amd.build.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.c_double
+    L.uwshcu_default_consts.argtypes = [C.POINTER(Consts)]
+    L.uwshcu_build_estbl.argtypes = [C.POINTER(Consts), C.POINTER(C.c_double)]
+    L.uwshcu_default_params.argtypes = [C.POINTER(Params)]
+    L.uwshcu_ctx_create.argtypes = [C.POINTER(Consts), C.POINTER(C.c_double), C.c_int, C.POINTER(vp)]
+    L.uwshcu_ctx_destroy.argtypes = [vp]
+    L.uwshcu_ctx_set_workspace_limit.argtypes = [vp, C.c_size_t]
+    L.uwshcu_workspace_bytes_per_column.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.uwshcu_workspace_bytes_per_column.restype = C.c_size_t
+    head = [vp, C.c_int, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp]
+    L.uwshcu_forward_device.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, vp]
+    L.uwshcu_forward_host.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, ip]
+    L.uwshcu_status.argtypes = [vp, vp, ip]
+    L.uwshcu_set_dropin_params.argtypes = [C.POINTER(Params)]
+    L.uwshcu_dropin_last_status.argtypes = [ip]
+    _lib = L
+    return L
+
+
+def default_consts() -> Consts:
+    c = Consts()
+    lib().uwshcu_default_consts(C.byref(c))
+    return c
+
+
+def build_estbl(c: Consts | None = None):
+    c = c or default_consts()
+    t = (C.c_double * ESTBL_LEN)()
+    rc = lib().uwshcu_build_estbl(C.byref(c), t)
+    if rc:
+        raise UwshcuError(rc, what="uwshcu_build_estbl")
+    return list(t)
+
+
+def default_params(rpen=10.0, rle=0.1, rkm=14.0) -> Params:
+    return Params(rpen, rle, rkm)
+
+
+def workspace_bytes_per_column(pver, ncnst, precision=FP64) -> int:
+    return int(lib().uwshcu_workspace_bytes_per_column(pver, ncnst, precision))

@@ -1,0 +1,144 @@
+"""Host-side mirror of compute_uwshcu_inv over the C ABI, on device-resident torch tensors.
+
+``compute_uwshcu_inv`` takes the same chunk fields as the reference routine
+(uwshcu.F90:251-265) and returns the same outputs.  Field layout: a layer field is a
+C-contiguous float64 tensor ``[pver, pcols]`` (= Fortran ``(pcols, pver)``), an
+interface field ``[pver+1, pcols]``, constituents ``[ncnst, pver, pcols]``, column
+scalars ``[pcols]``; level 0 is the model top.  Tensors live on the GPU; the library
+writes the outputs in place, stream-ordered on torch's current stream.
+
+PyTorch only supplies device memory and streams here; every computation runs in
+the HIP kernels of libuwshcu_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _capi
+from ._capi import FP64, MIXED, UwshcuError  # noqa: F401
+
+LAYER_OUT = ("qvten", "qlten", "qiten", "sten", "uten", "vten", "qrten", "qsten", "evapc", "cufrc", "qcu", "qlu",
+             "qiu", "qc")
+IFACE_OUT = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu")
+COLUMN_OUT = ("precip", "snow", "cbmf", "rliq", "cnt", "cnb")
+
+
+class Context:
+    """A library context on one GPU: device constants, saturation table, workspace."""
+
+    def __init__(self, device: int | torch.device | None = None, consts: _capi.Consts | None = None,
+                 estbl=None, workspace_limit: int | None = None):
+        L = _capi.lib()
+        if device is None:
+            device = torch.cuda.current_device()
+        if isinstance(device, torch.device):
+            device = device.index if device.index is not None else torch.cuda.current_device()
+        self.device = int(device)
+        self.consts = consts or _capi.default_consts()
+        tab = None
+        if estbl is not None:
+            tab = (C.c_double * _capi.ESTBL_LEN)(*estbl)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            rc = L.uwshcu_ctx_create(C.byref(self.consts), tab, self.device, C.byref(h))
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_ctx_create")
+        self._h = h
+        if workspace_limit:
+            L.uwshcu_ctx_set_workspace_limit(self._h, C.c_size_t(int(workspace_limit)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.lib().uwshcu_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    def alloc_outputs(self, pcols: int, pver: int, ncnst: int, diag: bool = False) -> dict:
+        dev = torch.device("cuda", self.device)
+        o = {k: torch.empty((pver + 1, pcols), dtype=torch.float64, device=dev) for k in IFACE_OUT}
+        o.update({k: torch.empty((pver, pcols), dtype=torch.float64, device=dev) for k in LAYER_OUT})
+        o["trten"] = torch.empty((max(ncnst, 1), pver, pcols), dtype=torch.float64, device=dev)
+        o.update({k: torch.empty((pcols,), dtype=torch.float64, device=dev) for k in COLUMN_OUT})
+        if diag:
+            o["diag"] = torch.empty((_capi.NDIAG, pcols), dtype=torch.int32, device=dev)
+        return o
+
+    def forward_into(self, inp: dict, out: dict, cush: torch.Tensor, *, ncol: int | None = None, dt: float = 1800.0,
+                     precision: int = FP64, params: _capi.Params | None = None, stream=None) -> None:
+        """Launch the forward physics; writes ``out`` and ``cush`` in place (asynchronous)."""
+        pver, pcols = inp["p0"].shape
+        ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
+        ncol = pcols if ncol is None else ncol
+        par = params or _capi.default_params()
+        ptrs = {}
+        for k in _capi.INPUT_NAMES + ("dpdry0",):
+            t = inp[k]
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+            ptrs[k] = t.data_ptr()
+        ptrs["cush"] = cush.data_ptr()
+        for k in _capi.OUTPUT_NAMES + ("wu",):
+            ptrs[k] = out[k].data_ptr()
+        diag = out.get("diag")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        args = [self._h, precision, C.byref(par), pcols, ncol, pver, ncnst, C.c_double(dt)]
+        args += [C.c_void_p(ptrs[k]) for k in _capi.ARRAY_ARGS]
+        args += [C.c_void_p(diag.data_ptr() if diag is not None else 0), C.c_void_p(sp)]
+        rc = _capi.lib().uwshcu_forward_device(*args)
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_forward_device")
+
+    def status(self, stream=None) -> tuple[int, int]:
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        bad = C.c_int(-1)
+        rc = _capi.lib().uwshcu_status(self._h, C.c_void_p(sp), C.byref(bad))
+        return rc, bad.value
+
+    def check(self, stream=None):
+        rc, bad = self.status(stream)
+        if rc:
+            raise UwshcuError(rc, bad)
+
+    def compute_uwshcu_inv(self, inp: dict, *, dt: float = 1800.0, precision: int = FP64,
+                           params: _capi.Params | None = None, diag: bool = False, ncol: int | None = None) -> dict:
+        """Run the chunk and return the outputs (synchronous; raises on a fatal condition)."""
+        pver, pcols = inp["p0"].shape
+        ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
+        out = self.alloc_outputs(pcols, pver, ncnst, diag)
+        cush = inp["cush"].clone()
+        self.forward_into(inp, out, cush, ncol=ncol, dt=dt, precision=precision, params=params)
+        self.check()
+        out["cush"] = cush
+        return out
+
+
+def chunk_to_device(chunk, device="cuda") -> dict:
+    """Copy a generator chunk (numpy fields) to device tensors."""
+    return {k: torch.from_numpy(v).to(device) for k, v in chunk.fields.items()}
+
+
+_default_ctx = {}
+
+
+def default_context(device=None) -> Context:
+    d = torch.cuda.current_device() if device is None else int(device)
+    if d not in _default_ctx:
+        _default_ctx[d] = Context(d)
+    return _default_ctx[d]
+
+
+def compute_uwshcu_inv(inp: dict, *, dt: float = 1800.0, precision: int = FP64, params=None, diag=False) -> dict:
+    """Module-level form of :meth:`Context.compute_uwshcu_inv` on the current device."""
+    return default_context().compute_uwshcu_inv(inp, dt=dt, precision=precision, params=params, diag=diag)

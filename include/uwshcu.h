@@ -1,0 +1,133 @@
+/*
+ * uwshcu.h -- C ABI of the MI355X UW shallow-cumulus column physics.
+ *
+ * Drop-in for the call the convect_shallow physics driver makes to
+ * compute_uwshcu_inv (uwshcu.F90:251-265, called at convect_shallow.F90:585-600)
+ * and for the reference's C entry point bxcompute_uwshcu_inv_
+ * (bxcompute_uwshcu_inv.cpp:6172-6184).
+ *
+ * Array conventions (all entry points): Fortran column-major chunk arrays with
+ * leading dimension pcols and the column index fastest, CAM level order (row 0
+ * = model top):
+ *   interface fields  (pcols, pver+1):  ps0 zs0 tke | umf slflx qtflx flxprc1 flxsnow1 wu
+ *   layer fields      (pcols, pver):    p0 z0 dp0 dpdry0 u0 v0 qv0 ql0 qi0 t0 s0 cldfrc concldfrc
+ *                                       | qvten qlten qiten sten uten vten qrten qsten evapc
+ *                                         cufrc qcu qlu qiu qc
+ *   constituents      (pcols, pver, ncnst): tr0 | trten
+ *   column scalars    (pcols):          pblh, cush (in/out) | precip snow cbmf rliq cnt cnb
+ * Only columns 0..ncol-1 are read or written.  cnt/cnb are returned in CAM
+ * level numbering (uwshcu.F90:434-435).  cldfrc/concldfrc are accepted and not
+ * read, as in the reference (uwshcu.F90:1240-1241).
+ *
+ * Constituents: NUMLIQ/NUMICE sit at the 1-based indices consts.ixnumliq /
+ * consts.ixnumice (defaults 4 and 5).  When ncnst is smaller than those indices
+ * the number constituents are taken as absent and zero; only trten(:,:,1:ncnst)
+ * is written.
+ *
+ * Errors: functions return UWSHCU_OK (0), a negative code from uwshcu_types.h,
+ * and never abort.  The conditions the reference stops the model for
+ * (findsp non-convergence or enthalpy error, mu > mumin2) are reported as
+ * negative codes with the offending column.
+ */
+#ifndef UWSHCU_H
+#define UWSHCU_H
+
+#include <stddef.h>
+#include "uwshcu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uwshcu_ctx uwshcu_ctx;
+
+/* CESM constants and table parameters; replaces the module set-up done by
+ * init_uwshcu (uwshcu.F90:102-249) and esinti/gestbl (esinti.F90:39-51,
+ * wv_saturation.F90:82-204).  Does not fill a table. */
+int uwshcu_default_consts(uwshcu_consts *c);
+
+/* Goff-Gratch saturation-vapour-pressure table exactly as gestbl builds it
+ * (wv_saturation.F90:141-169 with gffgch.F90:1-127): estbl[0..249]. */
+int uwshcu_build_estbl(const uwshcu_consts *c, double *estbl);
+
+/* Default tunables: rpen = 10 (namelist uwshcu_rpen), rle = 0.1, rkm = 14. */
+int uwshcu_default_params(uwshcu_params *p);
+
+/* Create a context on a GPU (device < 0: the current device).  c == NULL uses
+ * uwshcu_default_consts; estbl == NULL builds the table from c.  The context owns
+ * the device copies of the constants and the table, a status word and a
+ * workspace that grows on demand up to a limit (default 24 GiB). */
+int uwshcu_ctx_create(const uwshcu_consts *c, const double *estbl, int device, uwshcu_ctx **out);
+int uwshcu_ctx_destroy(uwshcu_ctx *ctx);
+int uwshcu_ctx_set_workspace_limit(uwshcu_ctx *ctx, size_t bytes);
+
+/* Bytes of workspace needed per column. */
+size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision);
+
+/* Forward physics on device-resident arrays (device pointers, same order as
+ * compute_uwshcu_inv).  Asynchronous on `stream` (a hipStream_t; NULL = the
+ * null stream); no host synchronisation.  diag (nullable, int [UWSHCU_NDIAG][pcols])
+ * receives per-column integer diagnostics: exit reason, kinv, klcl, klfc, krel,
+ * kbup, kpen (UW bottom-up indices), path (0 exit, 1 explicit-CIN result kept,
+ * 2 implicit-CIN result), limit_ufrc, limit_ppen, limit_emf, exit iteration.
+ * Call uwshcu_status to synchronise and read the fatal-condition status. */
+int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *par,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
+
+/* Synchronise `stream` and return the status of the forward calls issued on it
+ * since the previous uwshcu_status call (0 or a negative fatal code); *bad_col
+ * (nullable) receives the offending column. */
+int uwshcu_status(uwshcu_ctx *ctx, void *stream, int *bad_col);
+
+/* Forward physics on host arrays: copies the chunk to the GPU, runs, copies the
+ * outputs back and returns the status.  Same argument order. */
+int uwshcu_forward_host(uwshcu_ctx *ctx, int precision, const uwshcu_params *par,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, int *bad_col);
+
+/* Fortran-callable drop-in with the reference's symbol and argument list
+ * (bxcompute_uwshcu_inv.cpp:6172-6184; all arguments by reference).  Uses a
+ * process-wide FP64 context on the current device with the default constants
+ * and the tunables set by uwshcu_set_dropin_params.  Leading dimension is
+ * *pcols and the column count *ncol.  A fatal condition is printed to stderr
+ * and kept for uwshcu_dropin_last_status(). */
+void bxcompute_uwshcu_inv_(int *pcols, int *pver, int *ncol, int *pcnst, double *ztodt,
+    double *ps0_inv, double *zs0_inv, double *p0_inv, double *z0_inv, double *dp0_inv,
+    double *u0_inv, double *v0_inv, double *qv0_inv, double *ql0_inv, double *qi0_inv,
+    double *t0_inv, double *s0_inv, double *tr0_inv,
+    double *tke_inv, double *cldfrct_inv, double *concldfrct_inv, double *pblh, double *cush,
+    double *umf_inv, double *slflx_inv, double *qtflx_inv,
+    double *flxprc1_inv, double *flxsnow1_inv,
+    double *qvten_inv, double *qlten_inv, double *qiten_inv,
+    double *sten_inv, double *uten_inv, double *vten_inv, double *trten_inv,
+    double *qrten_inv, double *qsten_inv, double *precip, double *snow, double *evapc_inv,
+    double *cufrc_inv, double *qcu_inv, double *qlu_inv, double *qiu_inv,
+    double *cbmf, double *qc_inv, double *rliq,
+    double *cnt_inv, double *cnb_inv, int *lchnk, double *dpdry0_inv, double *wu_inv);
+
+/* Tunables used by bxcompute_uwshcu_inv_ (the reference reads rpen from the
+ * uwshcu_nl namelist, uwshcu.F90:58-98). */
+int uwshcu_set_dropin_params(const uwshcu_params *par);
+int uwshcu_dropin_last_status(int *bad_col);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,56 @@
+// uwshcu_types.h -- plain-C types shared by the HIP kernels and the C-ABI host layer.
+//
+// No torch or HIP types appear here: these structs cross the extern "C" boundary.
+#ifndef UWSHCU_TYPES_H
+#define UWSHCU_TYPES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UWSHCU_NCNST_MAX 32
+#define UWSHCU_ESTBL_LEN 250
+#define UWSHCU_NDIAG 12
+
+// Physical constants of the scheme and of the saturation table.  Field meaning
+// follows the module variables of uwshcu.F90:25-37 and wv_saturation.F90:48-60.
+typedef struct uwshcu_consts {
+    double xlv, xlf, xls, cp, zvir, r, g, ep2, p00, rovcp;   // uwshcu.F90:230-239
+    double tmin, tmax, ttrice, epsqs, rgasv, hlatf, hlatv, cpws, tmelt;  // wv_saturation.F90:125-140
+    int    icephs;
+    int    ixnumliq, ixnumice;       // 1-based constituent indices of NUMLIQ / NUMICE
+    int    pad_;
+    double pcf[5];                   // wv_saturation.F90:178-182
+    double qmin[UWSHCU_NCNST_MAX + 1];   // 1-based; qmin[1..3] = Q, CLDLIQ, CLDICE
+    int    wet[UWSHCU_NCNST_MAX + 1];    // 1 = 'wet' constituent, 0 = 'dry' (cnst_get_type_byind)
+} uwshcu_consts;
+
+// Tunable parameters: rpen is the uwshcu_nl namelist value (uwshcu.F90:58-98);
+// rle and rkm are compile-time parameters in the reference (uwshcu.F90:1036-1038)
+// and runtime values here.
+typedef struct uwshcu_params {
+    double rpen, rle, rkm;
+} uwshcu_params;
+
+// Precision modes.
+enum {
+    UWSHCU_FP64  = 0,   // all arithmetic in FP64
+    UWSHCU_MIXED = 1    // FP32 bulk; FP64 saturation adjustment, LCL, CIN and fix-ups
+};
+
+// Return codes (negative values are conditions the reference treats as fatal).
+enum {
+    UWSHCU_OK                 = 0,
+    UWSHCU_ERR_FINDSP_CONV    = -1,   // findsp not converging (uwshcu.F90:5419-5433)
+    UWSHCU_ERR_FINDSP_ENTH    = -2,   // findsp enthalpy error (uwshcu.F90:5434-5449)
+    UWSHCU_ERR_MUMIN2         = -3,   // mu > mumin2 (uwshcu.F90:2119-2122)
+    UWSHCU_ERR_ARG            = -10,  // invalid sizes or null pointers
+    UWSHCU_ERR_NOMEM          = -11,  // workspace allocation failed
+    UWSHCU_ERR_HIP            = -12,  // a HIP runtime call failed
+    UWSHCU_ERR_NO_DEVICE      = -13   // no usable GPU
+};
+
+#ifdef __cplusplus
+}
+#endif
+#endif

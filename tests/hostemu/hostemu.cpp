@@ -1,0 +1,49 @@
+// hostemu.cpp -- TEST HARNESS: runs the product's column code (uwshcu_column.h) on the
+// CPU, one column at a time, so that its logic can be compared with the oracle in a
+// container without a GPU.  It is built only by the test suite and never shipped;
+// the library the product loads contains no CPU path.
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "uwshcu_column.h"
+
+using namespace uwshcu;
+
+extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int precision, const uwshcu_params* par,
+                               int pcols, int ncol, int pver, int ncnst, double dt, const double* ps0,
+                               const double* zs0, const double* p0, const double* z0, const double* dp0,
+                               const double* u0, const double* v0, const double* qv0, const double* ql0,
+                               const double* qi0, const double* t0, const double* s0, const double* tr0,
+                               const double* tke, const double* pblh, double* cush, double* umf, double* slflx,
+                               double* qtflx, double* flxprc1, double* flxsnow1, double* qvten, double* qlten,
+                               double* qiten, double* sten, double* uten, double* vten, double* trten,
+                               double* qrten, double* qsten, double* precip, double* snow, double* evapc,
+                               double* cufrc, double* qcu, double* qlu, double* qiu, double* cbmf, double* qc,
+                               double* rliq, double* cnt, double* cnb, const double* dpdry0, double* wu, int* diag,
+                               int* bad_col) {
+    FwdArgs a{};
+    a.ncol = ncol; a.mkx = pver; a.ncnst = ncnst; a.ldc = pcols; a.col0 = 0; a.dt = dt; a.par = *par;
+    a.ps0 = ps0; a.zs0 = zs0; a.p0 = p0; a.z0 = z0; a.dp0 = dp0; a.dpdry0 = dpdry0; a.u0 = u0; a.v0 = v0;
+    a.qv0 = qv0; a.ql0 = ql0; a.qi0 = qi0; a.t0 = t0; a.s0 = s0; a.tr0 = tr0; a.tke = tke; a.pblh = pblh;
+    a.cush = cush; a.umf = umf; a.slflx = slflx; a.qtflx = qtflx; a.flxprc1 = flxprc1; a.flxsnow1 = flxsnow1;
+    a.qvten = qvten; a.qlten = qlten; a.qiten = qiten; a.sten = sten; a.uten = uten; a.vten = vten;
+    a.trten = trten; a.qrten = qrten; a.qsten = qsten; a.precip = precip; a.snow = snow; a.evapc = evapc;
+    a.cufrc = cufrc; a.qcu = qcu; a.qlu = qlu; a.qiu = qiu; a.cbmf = cbmf; a.qc = qc; a.rliq = rliq;
+    a.cnt = cnt; a.cnb = cnb; a.wu = wu; a.diag = diag;
+    int err[2] = {0, -1};
+    a.err = err;
+    const size_t esz = precision == UWSHCU_FP64 ? sizeof(double) : sizeof(float);
+    a.wld = ncol;
+    std::vector<char> ws((size_t)ws_arrays(ncnst) * (pver + 1) * (size_t)ncol * esz + 64);
+    std::memset(ws.data(), 0xff, ws.size());   // NaN pattern: unwritten workspace reads show up
+    a.ws = ws.data();
+    const Thermo T{c, estbl};
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ncol; ++i) {
+        if (precision == UWSHCU_FP64) Column<double>(a, T, i).run();
+        else Column<float>(a, T, i).run();
+    }
+    if (bad_col) *bad_col = err[0] ? err[1] : -1;
+    return err[0];
+}
