@@ -243,8 +243,8 @@ UWSHCU_HD void Column<R>::run() const {
     }
 
     // ---- S0 environment, set A (uwshcu.F90:1255-1299) ----
-    for (int k = 1; k <= mkx; ++k) W(A_EXN0, k) = R(pow(in(a.p0, k) / C.p00, C.rovcp));
-    for (int k = 0; k <= mkx; ++k) W(A_EXNS0, k) = R(pow(in(a.ps0, k) / C.p00, C.rovcp));
+    for (int k = 1; k <= mkx; ++k) W(A_EXN0, k) = R(upow(in(a.p0, k) / C.p00, C.rovcp));
+    for (int k = 0; k <= mkx; ++k) W(A_EXNS0, k) = R(upow(in(a.ps0, k) / C.p00, C.rovcp));
     for (int k = 1; k <= mkx; ++k) {
         const R qv0 = R(in(a.qv0, k)), ql0 = R(in(a.ql0, k)), qi0 = R(in(a.qi0, k)), t0 = R(in(a.t0, k));
         const R qt0 = (qv0 + ql0 + qi0);
@@ -427,15 +427,15 @@ UWSHCU_HD void Column<R>::run() const {
         R mu = wcrit / sigmaw / R(1.4142);
         if (mu >= R(3)) { exit_code = EX_MU3; goto L333; }
         const R rho0inv = R(in(a.ps0, kinv - 1)) / (r * W(A_THV0TOP, kinv - 1) * W(A_EXNS0, kinv - 1));
-        cbmf = (rho0inv * sigmaw / R(2.5066)) * exp(-(mu * mu));
+        cbmf = (rho0inv * sigmaw / R(2.5066)) * uexp(-(mu * mu));
         const R cbmflimit = R(0.9) * R(in(a.dp0, kinv - 1)) / g / dt;
         R mumin0 = R(0);
-        if (cbmf > cbmflimit) mumin0 = sqrt(-log(R(2.5066) * cbmflimit / rho0inv / sigmaw));
+        if (cbmf > cbmflimit) mumin0 = sqrt(-ulog(R(2.5066) * cbmflimit / rho0inv / sigmaw));
         mu = fmax(fmax(mu, mumin0), mumin1);
         const R mulcl = sqrt(R(2) * R(cinlcl) * rbuoy) / R(1.4142) / sigmaw;
         R mulclstar;
         {
-            const R e1 = exp(-(mu * mu)) / R(2.5066), ef = erfc(mu);
+            const R e1 = uexp(-(mu * mu)) / R(2.5066), ef = uerfc(mu);
             mulclstar = sqrt(fmax(R(0), R(2) * (e1 * e1) * (R(1) / (ef * ef) - R(0.25) / (rmaxfrac * rmaxfrac))));
         }
         if (mulcl > R(1.e-8) && mulcl > mulclstar) {
@@ -445,8 +445,8 @@ UWSHCU_HD void Column<R>::run() const {
             if (mu == mumin2) limit_ufrc = 1;
         }
         if (mu == mumin1) limit_ufrc = 1;
-        cbmf = (rho0inv * sigmaw / R(2.5066)) * exp(-(mu * mu));
-        const R winv = sigmaw * (R(2) / R(2.5066)) * exp(-(mu * mu)) / erfc(mu);
+        cbmf = (rho0inv * sigmaw / R(2.5066)) * uexp(-(mu * mu));
+        const R winv = sigmaw * (R(2) / R(2.5066)) * uexp(-(mu * mu)) / uerfc(mu);
         R wtw = winv * winv - R(2) * R(cinlcl) * rbuoy;
         if (wtw <= R(0)) { exit_code = EX_WTW; goto L333; }
         wlcl = sqrt(wtw);
@@ -578,13 +578,13 @@ UWSHCU_HD void Column<R>::run() const {
                     const R ud2 = R(1) - R(2) * xc + xc * xc;
                     R reik = (R(0.5) * rkm / R(in(a.z0, k)) / g / rho0j);
                     if (xc > R(0.5))
-                        reik = fmin(reik, R(0.9) * log(R(in(a.dp0, k)) / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
+                        reik = fmin(reik, R(0.9) * ulog(R(in(a.dp0, k)) / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
                     W(W_REI, k) = reik;
                     ferk = reik * ee2;
                     fdrk = reik * ud2;
                     W(W_FER, k) = ferk;
                     W(W_FDR, k) = fdrk;
-                    umfk = umf_km1 * exp(dpe * (ferk - fdrk));
+                    umfk = umf_km1 * uexp(dpe * (ferk - fdrk));
                     W(W_EMF, k) = R(0);
                     R thluk, qtuk, uuk, vuk;
                     const R ss_thl = W(A_SSTHL0, k), ss_qt = W(A_SSQT0, k), ss_u = W(A_SSU0, k), ss_v = W(A_SSV0, k);
@@ -598,7 +598,7 @@ UWSHCU_HD void Column<R>::run() const {
                             WT(1, m, k) = WT(1, m, km1) + (tre + WT(0, m, k) * dpe / R(2) - WT(1, m, km1)) * ferk * dpe;
                         }
                     } else {
-                        const R ef = exp(-ferk * dpe);
+                        const R ef = uexp(-ferk * dpe);
                         thluk = (thle + ss_thl / ferk - ss_thl * dpe / R(2)) -
                                 (thle + ss_thl * dpe / R(2) - W(W_THLU, km1) + ss_thl / ferk) * ef;
                         qtuk = (qte + ss_qt / ferk - ss_qt * dpe / R(2)) -
@@ -639,7 +639,7 @@ UWSHCU_HD void Column<R>::run() const {
                     bogtop = rbuoy * (thvuk / W(A_THV0TOP, k) - R(1));
                     const R delbog = bogtop - bogbot;
                     drage = ferk * (R(1) + rdrag);
-                    const R expfac = exp(R(-2) * drage * dpe);
+                    const R expfac = uexp(R(-2) * drage * dpe);
                     wtwb = wtw;
                     if (drage * dpe > R(1.e-3))
                         wtw = wtw * expfac + (delbog + (R(1) - expfac) * (bogbot + delbog / (R(-2) * drage * dpe))) / (rho0j * drage);
@@ -668,7 +668,7 @@ UWSHCU_HD void Column<R>::run() const {
                         ufrck = rmaxfrac;
                         umfk = rmaxfrac * rhos0j * wuk;
                         out(a.umf, k) = (double)umfk;
-                        W(W_FDR, k) = ferk - log(umfk / umf_km1) / dpe;
+                        W(W_FDR, k) = ferk - ulog(umfk / umf_km1) / dpe;
                     }
                     W(W_UFRC, k) = ufrck;
                 }
@@ -717,7 +717,7 @@ UWSHCU_HD void Column<R>::run() const {
                     thlu_top = W(W_THLU, kpen - 1) + (thl0p + ssthlp * (-ppen) / R(2) - W(W_THLU, kpen - 1)) * ferp * (-ppen);
                     qtu_top = W(W_QTU, kpen - 1) + (qt0p + ssqtp * (-ppen) / R(2) - W(W_QTU, kpen - 1)) * ferp * (-ppen);
                 } else {
-                    const R ef = exp(-ferp * (-ppen));
+                    const R ef = uexp(-ferp * (-ppen));
                     thlu_top = (thl0p + ssthlp / ferp - ssthlp * (-ppen) / R(2)) -
                                (thl0p + ssthlp * (-ppen) / R(2) - W(W_THLU, kpen - 1) + ssthlp / ferp) * ef;
                     qtu_top = (qt0p + ssqtp / ferp - ssqtp * (-ppen) / R(2)) -
@@ -727,7 +727,7 @@ UWSHCU_HD void Column<R>::run() const {
             {
                 const R ptop = R(in(a.ps0, kpen - 1)) + ppen;
                 if (T.conden(ptop, thlu_top, qtu_top, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-                const R exntop = R(pow((double)ptop / C.p00, C.rovcp));
+                const R exntop = R(upow((double)ptop / C.p00, C.rovcp));
                 if ((R(qlj) + R(qij)) > criqc) {
                     const R qlr = R(qlj), qir = R(qij);
                     const R dw = ((qlr + qir) - criqc) * qlr / (qlr + qir);

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include "uwshcu_types.h"
+#include "uwshcu_math.h"
 
 // Every primitive is callable from host code too: the test suite builds the column
 // code for the CPU to check its logic against the oracle before it runs on a GPU.
@@ -25,7 +26,7 @@ struct Thermo {
     const uwshcu_consts* c;   // kernel-argument copy
     const double* estbl;      // LDS copy of the table
 
-    UWSHCU_HD inline double exnf(double p) const { return pow(p / c->p00, c->rovcp); }  // uwshcu.F90:50-54
+    UWSHCU_HD inline double exnf(double p) const { return upow(p / c->p00, c->rovcp); }  // uwshcu.F90:50-54
 
     // wv_saturation.F90:64-80
     UWSHCU_HD inline double estblf(double td) const {
@@ -106,15 +107,15 @@ struct Thermo {
     UWSHCU_HD double qsinvert(double qt, double thl, double psfc) const {
         const double psmin = 100.0 * 100.0, dpsmax = 1.0;
         double es, qs, gam;
-        const double Ti = thl * pow(psfc / c->p00, c->rovcp);
+        const double Ti = thl * upow(psfc / c->p00, c->rovcp);
         qsat_full(Ti, psfc, es, qs, gam);
         const double rhi = qt / qs;
         if (rhi <= 0.01) return psmin;
-        const double TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - log(rhi) / 2840.0);
+        const double TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - ulog(rhi) / 2840.0);
         const double PiLCL = TLCL / thl;
-        double ps = c->p00 * pow(PiLCL, 1.0 / c->rovcp);
+        double ps = c->p00 * upow(PiLCL, 1.0 / c->rovcp);
         for (int it = 0; it < 10; ++it) {
-            const double Pis = pow(ps / c->p00, c->rovcp);
+            const double Pis = upow(ps / c->p00, c->rovcp);
             const double Ts = thl * Pis;
             qsat_full(Ts, ps, es, qs, gam);
             const double err = qt - qs;
@@ -186,7 +187,7 @@ UWSHCU_HD inline void roots(R a, R b, R c, R& r1, R& r2, int& status) {
 UWSHCU_HD inline double compute_alpha(double del_CIN, double ke) {
     double x0 = 0.0;
     for (int it = 0; it < 10; ++it) {
-        const double e = exp(-x0 * ke * del_CIN);
+        const double e = uexp(-x0 * ke * del_CIN);
         x0 = x0 - (e - x0) / (-ke * del_CIN * e - 1.0);
     }
     return x0;
@@ -197,8 +198,8 @@ template <class R>
 UWSHCU_HD inline R compute_mumin2(R mulcl, R rmaxfrac, R mulow) {
     R x0 = mulow;
     for (int it = 0; it < 10; ++it) {
-        const R ex = exp(-(x0 * x0));
-        const R ef = erfc(x0);
+        const R ex = uexp(-(x0 * x0));
+        const R ef = uerfc(x0);
         const R exf = ex / ef;
         const R a = ex / R(2) / rmaxfrac;
         const R b = mulcl * R(2.5066) / R(2);
@@ -221,7 +222,7 @@ UWSHCU_HD inline R compute_ppen(R wtwb, R D, R bogbot, R bogtop, R rho0j, R dpen
     } else {
         x0 = (s00 >= R(0)) ? dpen : R(0);
         for (int it = 0; it < 5; ++it) {
-            const R e = exp(R(-2) * D * x0);
+            const R e = uexp(R(-2) * D * x0);
             const R f = e * (wtwb - (bogbot - SB / (R(2) * D)) / (D * rho0j)) + (SB * x0 + bogbot - SB / (R(2) * D)) / (D * rho0j);
             R fs = R(-2) * D * e * (wtwb - (bogbot - SB / (R(2) * D)) / (D * rho0j)) + (SB) / (D * rho0j);
             if (fs >= R(0)) fs = fmax(fs, R(1.e-10));
@@ -241,7 +242,7 @@ UWSHCU_HD inline int findsp_point(const Thermo& T, double q, double t, double p,
                                             double& enerr) {
     const uwshcu_consts* c = T.c;
     const double omeps = 1.0 - c->epsqs, trinv = 1.0 / c->ttrice;
-    const double a1 = 7.5 * log(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
+    const double a1 = 7.5 * ulog(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
     const double dttol = 1.e-4, dqtol = 1.e-4, tt0 = 273.15;
     const double tlim = fmin(fmax(t, 173.0), 373.0);
     double es = T.estblf(tlim);

@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.path.join(HERE, "liboracle.so")          # host libm
+LIB_PATH_PM = os.path.join(HERE, "liboracle_pm.so")    # portable exp/log/pow/erfc
 NCNST_MAX = 32
 NDIAG = 12
 
@@ -42,15 +43,16 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(portable: bool = False):
+    """The oracle library; ``portable`` selects the portable-math flavour."""
+    if portable not in _libs:
+        path = LIB_PATH_PM if portable else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         d = C.c_double
         pc = C.POINTER(Consts)
@@ -78,8 +80,8 @@ def lib():
         L.orc_findsp_point.argtypes = [pc, d, d, d, dp, dp, dp]
         L.orc_findsp_point.restype = C.c_int
         L.orc_compute_uwshcu_inv_omp.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[portable] = L
+    return _libs[portable]
 
 
 def default_consts(rpen=10.0, rle=0.1, rkm=14.0) -> Consts:
@@ -93,8 +95,12 @@ def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def run(chunk, consts: Consts | None = None, nthreads: int = 0):
-    """Run the oracle on a generator chunk; returns (outputs dict, diag [ncol, NDIAG], rc, bad_col)."""
+def run(chunk, consts: Consts | None = None, nthreads: int = 0, portable: bool = False):
+    """Run the oracle on a generator chunk; returns (outputs dict, diag [ncol, NDIAG], rc, bad_col).
+
+    portable=False uses the host libm (the faithful reading of the reference);
+    portable=True the portable FP64 functions of orc_math.h.
+    """
     c = consts or default_consts()
     f = chunk.fields
     n, L, ncnst = chunk.ncol, chunk.pver, chunk.ncnst
@@ -115,6 +121,6 @@ def run(chunk, consts: Consts | None = None, nthreads: int = 0):
                                   "cufrc", "qcu", "qlu", "qiu", "cbmf", "qc", "rliq", "cnt", "cnb")]
     args += [_p(np.ascontiguousarray(f["dpdry0"])), _p(out["wu"]),
              diag.ctypes.data_as(C.POINTER(C.c_int)), C.byref(bad)]
-    rc = lib().orc_compute_uwshcu_inv_omp(*args)
+    rc = lib(portable).orc_compute_uwshcu_inv_omp(*args)
     out["cush"] = cush
     return out, diag, rc, bad.value

@@ -28,6 +28,23 @@
 #include <string.h>
 #include "uwshcu_oracle.h"
 
+/* Transcendental functions of the column physics: the host libm by default, or
+ * the portable FP64 versions of orc_math.h (-DORC_PORTABLE_MATH) that the HIP
+ * kernel reproduces bit for bit.  The saturation table builder always uses the
+ * host libm (the table is data shared by both sides). */
+#ifdef ORC_PORTABLE_MATH
+#include "orc_math.h"
+#define XEXP orc_pm_exp
+#define XLOG orc_pm_log
+#define XPOW orc_pm_pow
+#define XERFC orc_pm_erfc
+#else
+#define XEXP exp
+#define XLOG log
+#define XPOW pow
+#define XERFC erfc
+#endif
+
 #define KMAX 200            /* mkx must be < KMAX */
 #define NT   (ORC_NCNST_MAX + 1)
 
@@ -177,7 +194,7 @@ void orc_qsat(const orc_consts *c, double t, double p, double *es_out, double *q
 }
 
 /* uwshcu.F90:50-54 */
-static inline double exnf(const orc_consts *c, double pressure) { return pow(pressure / c->p00, c->rovcp); }
+static inline double exnf(const orc_consts *c, double pressure) { return XPOW(pressure / c->p00, c->rovcp); }
 
 /* uwshcu.F90:4718-4785 */
 void orc_conden(const orc_consts *c, double p, double thl, double qt,
@@ -257,15 +274,15 @@ double orc_qsinvert(const orc_consts *c, double qt, double thl, double psfc)
 {
     double ps, Pis, Ts, err, dlnqsdT, dTdPis, dPisdps, dlnqsdps, derrdps, dps;
     double Ti, rhi, TLCL, PiLCL, psmin = 100.0 * 100.0, dpsmax = 1.0, es, qs, gam, leff, nu;
-    Ti = thl * pow(psfc / c->p00, c->rovcp);
+    Ti = thl * XPOW(psfc / c->p00, c->rovcp);
     orc_qsat(c, Ti, psfc, &es, &qs, &gam);
     rhi = qt / qs;
     if (rhi <= 0.01) return psmin;
-    TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - log(rhi) / 2840.0);
+    TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - XLOG(rhi) / 2840.0);
     PiLCL = TLCL / thl;
-    ps = c->p00 * pow(PiLCL, 1.0 / c->rovcp);
+    ps = c->p00 * XPOW(PiLCL, 1.0 / c->rovcp);
     for (int i = 1; i <= 10; ++i) {
-        Pis = pow(ps / c->p00, c->rovcp);
+        Pis = XPOW(ps / c->p00, c->rovcp);
         Ts = thl * Pis;
         orc_qsat(c, Ts, ps, &es, &qs, &gam);
         err = qt - qs;
@@ -289,7 +306,7 @@ double orc_compute_alpha(double del_CIN, double ke)
 {
     double x0 = 0.0, x1;
     for (int it = 1; it <= 10; ++it) {
-        x1 = x0 - (exp(-x0 * ke * del_CIN) - x0) / (-ke * del_CIN * exp(-x0 * ke * del_CIN) - 1.0);
+        x1 = x0 - (XEXP(-x0 * ke * del_CIN) - x0) / (-ke * del_CIN * XEXP(-x0 * ke * del_CIN) - 1.0);
         x0 = x1;
     }
     return x0;
@@ -301,8 +318,8 @@ double orc_compute_mumin2(double mulcl, double rmaxfrac, double mulow)
 {
     double x0 = mulow, x1, ex, ef, exf, f, fs;
     for (int it = 1; it <= 10; ++it) {
-        ex = exp(-(x0 * x0));
-        ef = erfc(x0);
+        ex = XEXP(-(x0 * x0));
+        ef = XERFC(x0);
         exf = ex / ef;
         f = 0.5 * (exf * exf) - 0.5 * ((ex / 2.0 / rmaxfrac) * (ex / 2.0 / rmaxfrac)) -
             ((mulcl * 2.5066 / 2.0) * (mulcl * 2.5066 / 2.0));
@@ -326,9 +343,9 @@ double orc_compute_ppen(double wtwb, double D, double bogbot, double bogtop, dou
         if (s00 >= 0.0) x0 = dpen;
         else x0 = 0.0;
         for (int it = 1; it <= 5; ++it) {
-            f = exp(-2.0 * D * x0) * (wtwb - (bogbot - SB / (2.0 * D)) / (D * rho0j)) +
+            f = XEXP(-2.0 * D * x0) * (wtwb - (bogbot - SB / (2.0 * D)) / (D * rho0j)) +
                 (SB * x0 + bogbot - SB / (2.0 * D)) / (D * rho0j);
-            fs = -2.0 * D * exp(-2.0 * D * x0) * (wtwb - (bogbot - SB / (2.0 * D)) / (D * rho0j)) +
+            fs = -2.0 * D * XEXP(-2.0 * D * x0) * (wtwb - (bogbot - SB / (2.0 * D)) / (D * rho0j)) +
                  (SB) / (D * rho0j);
             if (fs >= 0.0) fs = dmax(fs, 1.e-10);
             else fs = dmin(fs, -1.e-10);
@@ -453,7 +470,7 @@ int orc_findsp_point(const orc_consts *c, double q, double t, double p, double *
                      double *enerr)
 {
     const double omeps = 1.0 - c->epsqs, trinv = 1.0 / c->ttrice;
-    const double a1 = 7.5 * log(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
+    const double a1 = 7.5 * XLOG(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
     const double dttol = 1.e-4, dqtol = 1.e-4, tt0 = 273.15;
     double tlim, es, denom, qs, tc, weight, hlatsb, hlatvp, hltalt, enin, enout = 1.0, tmp, c1, c2, r1b, qvd;
     double tsp, qsp, tterm, desdt, dqsdt, gg, dgdt, t1, dt, q1, dq;
@@ -634,8 +651,8 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
     for (m = 0; m < NT; ++m) { trsrc[m] = tre[m] = trsrc_o[m] = 0.0; }
 
     /* ---- S0 environment (uwshcu.F90:1255-1299) ---- */
-    for (k = 1; k <= mkx; ++k) exn0[k] = pow(p0[k] / c->p00, c->rovcp);
-    for (k = 0; k <= mkx; ++k) exns0[k] = pow(ps0[k] / c->p00, c->rovcp);
+    for (k = 1; k <= mkx; ++k) exn0[k] = XPOW(p0[k] / c->p00, c->rovcp);
+    for (k = 0; k <= mkx; ++k) exns0[k] = XPOW(ps0[k] / c->p00, c->rovcp);
     for (k = 1; k <= mkx; ++k) {
         qt0[k] = (qv0[k] + ql0[k] + qi0[k]);
         thl0[k] = (t0[k] - xlv * ql0[k] / cp - xls * qi0[k] / cp) / exn0[k];
@@ -865,16 +882,16 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
         mu = wcrit / sigmaw / 1.4142;
         if (mu >= 3.0) { exit_code = EX_MU3; id_exit = 1; goto L333; }
         rho0inv = ps0[kinv - 1] / (r * thv0top[kinv - 1] * exns0[kinv - 1]);
-        cbmf = (rho0inv * sigmaw / 2.5066) * exp(-(mu * mu));
+        cbmf = (rho0inv * sigmaw / 2.5066) * XEXP(-(mu * mu));
         {
             double cbmflimit = 0.9 * dp0[kinv - 1] / g / dt;
             mumin0 = 0.0;
-            if (cbmf > cbmflimit) mumin0 = sqrt(-log(2.5066 * cbmflimit / rho0inv / sigmaw));
+            if (cbmf > cbmflimit) mumin0 = sqrt(-XLOG(2.5066 * cbmflimit / rho0inv / sigmaw));
         }
         mu = dmax(dmax(mu, mumin0), mumin1);
         mulcl = sqrt(2.0 * cinlcl * rbuoy) / 1.4142 / sigmaw;
         {
-            double e1 = exp(-(mu * mu)) / 2.5066, ef = erfc(mu);
+            double e1 = XEXP(-(mu * mu)) / 2.5066, ef = XERFC(mu);
             mulclstar = sqrt(dmax(0.0, 2.0 * (e1 * e1) * (1.0 / (ef * ef) - 0.25 / (rmaxfrac * rmaxfrac))));
         }
         if (mulcl > 1.e-8 && mulcl > mulclstar) {
@@ -884,8 +901,8 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
             if (mu == mumin2) limit_ufrc = 1;
         }
         if (mu == mumin1) limit_ufrc = 1;
-        cbmf = (rho0inv * sigmaw / 2.5066) * exp(-(mu * mu));
-        winv = sigmaw * (2.0 / 2.5066) * exp(-(mu * mu)) / erfc(mu);
+        cbmf = (rho0inv * sigmaw / 2.5066) * XEXP(-(mu * mu));
+        winv = sigmaw * (2.0 / 2.5066) * XEXP(-(mu * mu)) / XERFC(mu);
         wtw = winv * winv - 2.0 * cinlcl * rbuoy;
         if (wtw <= 0.0) { exit_code = EX_WTW; id_exit = 1; goto L333; }
         wlcl = sqrt(wtw);
@@ -1013,10 +1030,10 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                     ud2 = 1.0 - 2.0 * xc + xc * xc;
                     rei[k] = (0.5 * rkm / z0[k] / g / rho0j);
                     if (xc > 0.5)
-                        rei[k] = dmin(rei[k], 0.9 * log(dp0[k] / g / dt / umf[km1] + 1.0) / dpe / (2.0 * xc - 1.0));
+                        rei[k] = dmin(rei[k], 0.9 * XLOG(dp0[k] / g / dt / umf[km1] + 1.0) / dpe / (2.0 * xc - 1.0));
                     fer[k] = rei[k] * ee2;
                     fdr[k] = rei[k] * ud2;
-                    umf[k] = umf[km1] * exp(dpe * (fer[k] - fdr[k]));
+                    umf[k] = umf[km1] * XEXP(dpe * (fer[k] - fdr[k]));
                     emf[k] = 0.0;
                     if (fer[k] * dpe < 1.e-4) {
                         thlu[k] = thlu[km1] + (thle + ssthl0[k] * dpe / 2.0 - thlu[km1]) * fer[k] * dpe;
@@ -1027,7 +1044,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                             W->tru[m][k] = W->tru[m][km1] +
                                            (tre[m] + W->sstr0[m][k] * dpe / 2.0 - W->tru[m][km1]) * fer[k] * dpe;
                     } else {
-                        double ef = exp(-fer[k] * dpe);
+                        double ef = XEXP(-fer[k] * dpe);
                         thlu[k] = (thle + ssthl0[k] / fer[k] - ssthl0[k] * dpe / 2.0) -
                                   (thle + ssthl0[k] * dpe / 2.0 - thlu[km1] + ssthl0[k] / fer[k]) * ef;
                         qtu[k] = (qte + ssqt0[k] / fer[k] - ssqt0[k] * dpe / 2.0) -
@@ -1061,7 +1078,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                     bogtop = rbuoy * (thvu[k] / thv0top[k] - 1.0);
                     delbog = bogtop - bogbot;
                     drage = fer[k] * (1.0 + rdrag);
-                    expfac = exp(-2.0 * drage * dpe);
+                    expfac = XEXP(-2.0 * drage * dpe);
                     wtwb = wtw;
                     if (drage * dpe > 1.e-3)
                         wtw = wtw * expfac + (delbog + (1.0 - expfac) * (bogbot + delbog / (-2.0 * drage * dpe))) /
@@ -1088,7 +1105,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                     limit_ufrc = 1;
                     ufrc[k] = rmaxfrac;
                     umf[k] = rmaxfrac * rhos0j * wu[k];
-                    fdr[k] = fer[k] - log(umf[k] / umf[km1]) / dpe;
+                    fdr[k] = fer[k] - XLOG(umf[k] / umf[km1]) / dpe;
                 }
                 pe = p0[k + 1];
                 dpe = dp0[k + 1];
@@ -1128,7 +1145,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                            (thl0[kpen] + ssthl0[kpen] * (-ppen) / 2.0 - thlu[kpen - 1]) * fer[kpen] * (-ppen);
                 qtu_top = qtu[kpen - 1] + (qt0[kpen] + ssqt0[kpen] * (-ppen) / 2.0 - qtu[kpen - 1]) * fer[kpen] * (-ppen);
             } else {
-                double ef = exp(-fer[kpen] * (-ppen));
+                double ef = XEXP(-fer[kpen] * (-ppen));
                 thlu_top = (thl0[kpen] + ssthl0[kpen] / fer[kpen] - ssthl0[kpen] * (-ppen) / 2.0) -
                            (thl0[kpen] + ssthl0[kpen] * (-ppen) / 2.0 - thlu[kpen - 1] + ssthl0[kpen] / fer[kpen]) * ef;
                 qtu_top = (qt0[kpen] + ssqt0[kpen] / fer[kpen] - ssqt0[kpen] * (-ppen) / 2.0) -
@@ -1136,7 +1153,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
             }
             orc_conden(c, ps0[kpen - 1] + ppen, thlu_top, qtu_top, &thj, &qvj, &qlj, &qij, &qse, &id_check);
             if (id_check == 1) { exit_code = EX_CONDEN; id_exit = 1; goto L333; }
-            exntop = pow((ps0[kpen - 1] + ppen) / c->p00, c->rovcp);
+            exntop = XPOW((ps0[kpen - 1] + ppen) / c->p00, c->rovcp);
             if ((qlj + qij) > criqc) {
                 dwten[kpen] = ((qlj + qij) - criqc) * qlj / (qlj + qij);
                 diten[kpen] = ((qlj + qij) - criqc) * qij / (qlj + qij);

@@ -1,0 +1,78 @@
+"""The C ABI library (CPU only: loads and host-side functions, no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from uwshcu_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names += re.findall(r"^\s*(?:int|void|size_t)\s+\**(\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.lib()
+    declared = _declared_functions()
+    assert "uwshcu_forward_device" in declared and "bxcompute_uwshcu_inv_" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_capi.SYMBOLS) <= set(declared)
+
+
+def test_struct_layout_matches_header():
+    # offsets the HIP side relies on: the ctypes mirror must match the C struct
+    assert C.sizeof(_capi.Consts) == 8 * 19 + 4 * 4 + 8 * 5 + 8 * 33 + 4 * 33 + 4
+    assert C.sizeof(_capi.Params) == 24
+
+
+def test_default_constants_equal_the_oracle():
+    c = _capi.default_consts()
+    o = oracle.default_consts()
+    for f, _ in _capi.Consts._fields_:
+        if f in ("pad_", "pcf", "qmin", "wet", "ixnumliq", "ixnumice"):
+            continue
+        if hasattr(o, f):
+            assert getattr(c, f) == getattr(o, f), f
+    assert list(c.pcf) == list(o.pcf)
+    assert list(c.qmin) == list(o.qmin) and list(c.wet) == list(o.wet)
+    assert (c.ixnumliq, c.ixnumice) == (o.ixnumliq, o.ixnumice) == (4, 5)
+
+
+def test_table_builder_equals_the_oracle_table():
+    t = np.array(_capi.build_estbl())
+    np.testing.assert_array_equal(t, np.array(oracle.default_consts().estbl))
+
+
+def test_workspace_size():
+    fp64 = _capi.workspace_bytes_per_column(35, 3, _capi.FP64)
+    mixed = _capi.workspace_bytes_per_column(35, 3, _capi.MIXED)
+    assert fp64 == 2 * mixed and fp64 % 36 == 0
+
+
+def test_device_entry_points_reject_bad_arguments_without_a_gpu():
+    lib = _capi.lib()
+    par = _capi.default_params()
+    nul = [C.c_void_p(0)] * len(_capi.ARRAY_ARGS)
+    rc = lib.uwshcu_forward_device(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
+    assert rc == -10
+    rc = lib.uwshcu_forward_host(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
+    assert rc == -10
+
+
+def test_product_fails_loudly_without_the_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(_capi, "_lib", None)
+    monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="missing"):
+        _capi.lib()

@@ -1,9 +1,21 @@
 """GPU parity: the FP64 HIP path against the CPU oracle on identical seeded chunks.
 
-Bar: every output within 1e-10 per-column relative error of the oracle (see
-parity.column_rel_err), and the integer diagnostics (exit reason, kinv, klcl,
-klfc, krel, kbup, kpen, path, limiters, exit iteration) and cnt/cnb identical.
+Two oracles, both FP64 C restatements of the reference (oracle/uwshcu_oracle.c):
+
+* the portable-math oracle evaluates exp/log/pow/erfc with the same IEEE operation
+  sequence the kernel uses (uwshcu_math.h / orc_math.h).  The GPU must match it
+  bit for bit: every output and every integer diagnostic (exit reason, kinv,
+  klcl, klfc, krel, kbup, kpen, path, limiters, exit iteration).
+* the host-libm oracle is the faithful reading of the reference.  Against it the
+  integer diagnostics must be identical and the outputs within 1e-10 per-column
+  relative error, except in the few columns where a last-bit difference in a
+  transcendental function crosses one of the scheme's discontinuous branches;
+  the test checks that exactly those columns also separate the two CPU oracles,
+  i.e. that the difference comes from the libm and not from the GPU.
 """
+import ctypes as C
+import os
+
 import numpy as np
 import pytest
 
@@ -14,6 +26,7 @@ from uwshcu_amd.generator import make_chunk
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def _gpu_run(chunk, precision=0, params=None):
@@ -28,22 +41,48 @@ def _gpu_run(chunk, precision=0, params=None):
     return res
 
 
+def _worst_per_column(a, b):
+    w = np.zeros(np.asarray(b["precip"]).shape[0])
+    for k in b:
+        w = np.maximum(w, parity.column_rel_err(a[k], b[k]))
+    return w
+
+
 def _check(chunk, label):
-    ref, rdiag, rrc, _ = oracle.run(chunk, nthreads=8)
-    assert rrc == 0
     got = _gpu_run(chunk)
-    errs = parity.compare(got, ref)
-    worst = max(errs.items(), key=lambda kv: kv[1])
-    mism = np.where((got["diag"] != rdiag).any(axis=1))[0]
-    frac = (rdiag[:, 7] > 0).mean()
-    print(f"\n[{label}] convecting {frac:.3f} worst {worst[0]} {worst[1]:.3e} diag-mismatch cols {len(mism)}",
-          flush=True)
-    print(f"[{label}] reference-harness RMS {parity.reference_rms_errors(got, ref)}", flush=True)
-    assert len(mism) == 0, f"integer diagnostics differ in columns {mism[:10]}"
-    np.testing.assert_array_equal(got["cnt"], ref["cnt"])
-    np.testing.assert_array_equal(got["cnb"], ref["cnb"])
-    bad = {k: v for k, v in errs.items() if v > TOL}
-    assert not bad, f"fields above {TOL}: {bad}"
+    pm, pmdiag, rc, _ = oracle.run(chunk, nthreads=8, portable=True)
+    assert rc == 0
+    np.testing.assert_array_equal(got["diag"], pmdiag)
+    differing = [k for k in pm if not np.array_equal(got[k], pm[k])]
+    assert not differing, f"[{label}] not bit-identical to the portable-math oracle: {differing}"
+
+    libm, ldiag, rc, _ = oracle.run(chunk, nthreads=8)
+    w_gpu = _worst_per_column(got, libm)
+    w_cpu = _worst_per_column(pm, libm)
+    n_bad = int((w_gpu > TOL).sum())
+    print(f"\n[{label}] convecting {(ldiag[:, 7] > 0).mean():.3f}; bit-identical to the portable-math oracle; "
+          f"vs host-libm oracle: {n_bad}/{chunk.ncol} columns above {TOL:g} (worst {w_gpu.max():.2e}), "
+          f"integer diagnostics differ in {int((got['diag'] != ldiag).any(axis=1).sum())} columns", flush=True)
+    print(f"[{label}] reference-harness RMS vs host-libm oracle {parity.reference_rms_errors(got, libm)}", flush=True)
+    np.testing.assert_array_equal(np.flatnonzero(w_gpu > TOL), np.flatnonzero(w_cpu > TOL))
+    assert n_bad <= max(2, chunk.ncol // 50)
+    assert (got["diag"] != ldiag).any(axis=1).mean() <= 0.001
+
+
+def test_portable_math_is_bit_identical_on_gpu_and_host():
+    L = C.CDLL(os.path.join(HERE, "hostemu", "libmathprobe.so"))
+    rng = np.random.default_rng(0)
+    n = 1 << 18
+    x = np.concatenate([rng.uniform(-40, 40, n // 2), rng.uniform(-3, 6, n // 2)])
+    y = np.exp(rng.uniform(-30, 30, n))
+    go, ho = np.zeros((n, 8)), np.zeros((n, 8))
+    pd = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert L.probe_gpu(n, pd(x), pd(y), pd(go)) == 0
+    assert L.probe_host(n, pd(x), pd(y), pd(ho)) == 0
+    names = ("exp", "log", "pow", "erfc", "sqrt", "div", "fmax-fmin", "floor")
+    for j, nm in enumerate(names):
+        diff = np.flatnonzero(go[:, j].view(np.int64) != ho[:, j].view(np.int64))
+        assert diff.size == 0, f"{nm}: {diff.size} inputs differ, e.g. x={x[diff[0]]!r} y={y[diff[0]]!r}"
 
 
 def test_c1_chunk_16x30():
@@ -58,12 +97,17 @@ def test_72_levels_10_constituents():
     _check(make_chunk(4096, 72, 10, seed=2, ptop=100.0), "4096x72x10")
 
 
+def test_convecting_fraction_sweep():
+    for inv in (0.9, 0.5, 0.05):
+        _check(make_chunk(8192, 35, 3, seed=7, inversion_frac=inv), f"8192x35x3 capped {inv}")
+
+
 def test_padded_pcols_and_partial_ncol():
-    """pcols > ncol: only the first ncol columns are touched (reference indexes by pcols)."""
+    """pcols > ncol: only the first ncol columns are touched (the reference indexes by pcols)."""
     import torch
     from uwshcu_amd import device
     ch = make_chunk(200, 30, 3, seed=3)
-    ref, rdiag, _, _ = oracle.run(ch, nthreads=8)
+    pm, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
     ctx = device.default_context()
     inp = device.chunk_to_device(ch)
     sentinel = 12345.0
@@ -74,7 +118,51 @@ def test_padded_pcols_and_partial_ncol():
     ctx.forward_into(inp, out, cush, ncol=150, dt=ch.dt)
     ctx.check()
     got = {k: v.cpu().numpy() for k, v in out.items()}
-    for k in ("umf", "qvten", "sten", "precip"):
-        g = got[k]
-        assert np.all(g[..., 150:] == sentinel), k
-        assert parity.column_rel_err(g[..., :150], ref[k][..., :150]).max() <= TOL, k
+    for k in ("umf", "qvten", "sten", "precip", "trten", "wu"):
+        assert np.all(got[k][..., 150:] == sentinel), k
+        np.testing.assert_array_equal(got[k][..., :150], pm[k][..., :150])
+
+
+def test_workspace_batches_give_the_same_result():
+    """A workspace limit smaller than the chunk splits it into launches over column batches."""
+    import torch
+    from uwshcu_amd import _capi, device
+    ch = make_chunk(3000, 30, 3, seed=4)
+    pm, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    ctx = device.Context(torch.cuda.current_device(),
+                         workspace_limit=700 * _capi.workspace_bytes_per_column(30, 3, _capi.FP64))
+    out = ctx.compute_uwshcu_inv(device.chunk_to_device(ch), dt=ch.dt)
+    for k in pm:
+        np.testing.assert_array_equal(out[k].cpu().numpy(), pm[k], err_msg=k)
+    ctx.close()
+
+
+def test_host_pointer_entry_and_fortran_dropin_symbol():
+    """uwshcu_forward_host and bxcompute_uwshcu_inv_ (the reference's C symbol) on host arrays."""
+    from uwshcu_amd import _capi
+    ch = make_chunk(257, 30, 3, seed=5)
+    pm, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    f = ch.fields
+    n, Lv, nc = ch.ncol, ch.pver, ch.ncnst
+    outs = {k: np.zeros((Lv + 1, n)) for k in ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu")}
+    outs.update({k: np.zeros((Lv, n)) for k in ("qvten", "qlten", "qiten", "sten", "uten", "vten", "qrten", "qsten",
+                                                "evapc", "cufrc", "qcu", "qlu", "qiu", "qc")})
+    outs["trten"] = np.zeros((nc, Lv, n))
+    outs.update({k: np.zeros(n) for k in ("precip", "snow", "cbmf", "rliq", "cnt", "cnb")})
+    cush = f["cush"].copy()
+    pd = lambda a: a.ctypes.data_as(C.c_void_p)
+    L = _capi.lib()
+    pi = lambda v: C.byref(C.c_int(v))
+    args = [pi(n), pi(Lv), pi(n), pi(nc), C.byref(C.c_double(ch.dt))]
+    args += [pd(f[k]) for k in ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0",
+                                "tke", "cldfrc", "concldfrc", "pblh")] + [pd(cush)]
+    args += [pd(outs[k]) for k in ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "qiten", "sten",
+                                   "uten", "vten", "trten", "qrten", "qsten", "precip", "snow", "evapc", "cufrc",
+                                   "qcu", "qlu", "qiu", "cbmf", "qc", "rliq", "cnt", "cnb")]
+    args += [pi(7), pd(f["dpdry0"]), pd(outs["wu"])]
+    L.bxcompute_uwshcu_inv_(*args)
+    bad = C.c_int(0)
+    assert L.uwshcu_dropin_last_status(C.byref(bad)) == 0
+    outs["cush"] = cush
+    for k in pm:
+        np.testing.assert_array_equal(outs[k], pm[k], err_msg=k)

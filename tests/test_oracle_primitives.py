@@ -1,0 +1,361 @@
+"""Pins for the oracle's thermodynamic primitives (CPU only).
+
+Each function of oracle/uwshcu_oracle.c is checked against something the
+mathematics fixes independently of the restatement: closed forms, published
+values, textbook solvers (numpy.roots, scipy's Lambert W and erfc), brute-force
+Newton iterations, or invariants.  A dropped term, a wrong sign or a transposed
+operand in the restatement fails at least one of these.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+import oracle
+
+L = oracle.lib()
+CST = oracle.default_consts()
+D = C.c_double
+
+
+def _qsat(t, p):
+    es, qs, gam = D(), D(), D()
+    L.orc_qsat(C.byref(CST), t, p, C.byref(es), C.byref(qs), C.byref(gam))
+    return es.value, qs.value, gam.value
+
+
+def _conden(p, thl, qt):
+    th, qv, ql, qi, rv = D(), D(), D(), D(), D()
+    idc = C.c_int()
+    L.orc_conden(C.byref(CST), p, thl, qt, C.byref(th), C.byref(qv), C.byref(ql), C.byref(qi), C.byref(rv),
+                 C.byref(idc))
+    return th.value, qv.value, ql.value, qi.value, idc.value
+
+
+# --------------------------------------------------------------------- constants
+def test_constants_match_the_values_recorded_in_the_reference():
+    # decimal values printed in mpiuwshcu.cpp:36-46 and :87-97 (rovcp there is
+    # 1.2857..., i.e. r/cp + 1; the model's init_uwshcu sets rovcp = r/cp)
+    rec = dict(xlv=2501000.0, xlf=333700.0, xls=2834700.0, cp=1004.64, zvir=0.607793072824156,
+               r=287.042311365049, g=9.80616, ep2=0.621970586204516, epsqs=0.6219705862045155076,
+               rgasv=461.5046398201599231470, tmin=173.16, tmax=375.16, tmelt=273.15)
+    for k, v in rec.items():
+        assert getattr(CST, k) == pytest.approx(v, rel=1e-14), k
+    assert CST.rovcp == pytest.approx(CST.r / CST.cp, rel=0)
+    assert CST.rovcp == pytest.approx(0.2857165864041, rel=1e-12)
+
+
+# ------------------------------------------------------------------ es table
+def test_goff_gratch_boiling_point_is_one_atmosphere():
+    # at T = 373.16 K every correction term of the water formula vanishes:
+    # es = 10**log10(1013.246) * 100 = 101324.6 Pa
+    es = D()
+    L.orc_gffgch(C.byref(CST), 373.16, C.byref(es), 0)
+    assert es.value == pytest.approx(101324.6, rel=1e-14)
+
+
+def test_goff_gratch_ice_point_is_the_published_6_1071_hpa():
+    # Goff-Gratch over ice at the ice point: the log term vanishes and the formula
+    # reduces to its reference pressure 6.1071 hPa (just below tmelt: ice branch)
+    es = D()
+    L.orc_gffgch(C.byref(CST), CST.tmelt - 1e-9, C.byref(es), 1)
+    assert es.value == pytest.approx(610.71, rel=2e-5)
+
+
+def test_goff_gratch_water_at_tmelt_uses_water_branch():
+    # at T >= tmelt the water formula applies even when ice is requested (gffgch.F90:96,118)
+    es_w, es_i = D(), D()
+    L.orc_gffgch(C.byref(CST), CST.tmelt, C.byref(es_w), 0)
+    L.orc_gffgch(C.byref(CST), CST.tmelt, C.byref(es_i), 1)
+    assert es_w.value == es_i.value
+
+
+@pytest.mark.parametrize("t,es_pub", [(273.16, 611.657), (283.15, 1228.1), (293.15, 2338.8), (303.15, 4246.0)])
+def test_goff_gratch_water_within_half_percent_of_modern_values(t, es_pub):
+    # Goff-Gratch (1946) differs from modern formulations by well under 0.5 % here
+    es = D()
+    L.orc_gffgch(C.byref(CST), t, C.byref(es), 0)
+    assert es.value == pytest.approx(es_pub, rel=5e-3)
+
+
+def test_goff_gratch_against_independent_vectorised_formula():
+    from uwshcu_amd.generator import _goff_gratch_table
+    _, _, tab = _goff_gratch_table()
+    np.testing.assert_allclose(np.array(CST.estbl)[:len(tab)], tab, rtol=1e-13)
+
+
+def test_table_layout_and_transition():
+    tab = np.array(CST.estbl)
+    lentbl = int(CST.tmax - CST.tmin + 2.000001)
+    assert lentbl == 204
+    assert np.all(tab[lentbl:] == -99999.0)
+    t = CST.tmin + np.arange(lentbl)
+    assert np.all(np.diff(tab[:lentbl]) > 0)          # monotone in T
+    es_w, es_i = D(), D()
+    for j in range(lentbl):
+        L.orc_gffgch(C.byref(CST), t[j], C.byref(es_w), 0)
+        L.orc_gffgch(C.byref(CST), t[j], C.byref(es_i), 1)
+        w = min((CST.tmelt - t[j]) / 20.0, 1.0)
+        if t[j] >= CST.tmelt:
+            exp = es_w.value
+        elif t[j] < CST.tmelt - 20.0:
+            exp = es_i.value
+        else:
+            exp = w * es_i.value + (1 - w) * es_w.value     # 20 K blend, gffgch.F90:112-113
+        assert tab[j] == pytest.approx(exp, rel=1e-15)
+
+
+def test_estblf_interpolates_linearly_between_knots_and_clamps():
+    tab = np.array(CST.estbl)
+    for j in (0, 50, 100, 150, 201):
+        t = CST.tmin + j
+        assert L.orc_estblf(C.byref(CST), t) == pytest.approx(tab[j], rel=1e-15)
+        mid = L.orc_estblf(C.byref(CST), t + 0.25)
+        assert mid == pytest.approx(0.75 * tab[j] + 0.25 * tab[j + 1], rel=1e-13)
+    assert L.orc_estblf(C.byref(CST), 100.0) == pytest.approx(tab[0], rel=1e-15)
+    assert L.orc_estblf(C.byref(CST), 500.0) == pytest.approx(L.orc_estblf(C.byref(CST), CST.tmax), rel=0)
+
+
+# ----------------------------------------------------------------------- qsat
+@pytest.mark.parametrize("t,p", [(300.0, 1e5), (280.0, 8e4), (262.0, 6e4), (230.0, 3e4), (250.0, 5e4)])
+def test_qsat_definition_and_gamma(t, p):
+    es, qs, gam = _qsat(t, p)
+    assert es == pytest.approx(L.orc_estblf(C.byref(CST), t), rel=0)
+    eps = CST.epsqs
+    assert qs == pytest.approx(eps * es / (p - (1 - eps) * es), rel=1e-15)
+    # gam = (L/cp) dqs/dT with an analytic des/dT (Clausius-Clapeyron plus the
+    # water-ice transition polynomial), "accurate to within 1 percent" of the
+    # Goff-Gratch slope (wv_saturation.F90:561-566); compare with a centred
+    # difference of the Goff-Gratch function itself.
+    def gg(tt):
+        e = D()
+        L.orc_gffgch(C.byref(CST), tt, C.byref(e), -20)
+        return e.value
+    h = 1e-4
+    desdt = (gg(t + h) - gg(t - h)) / (2 * h)
+    omeps = 1 - eps
+    dqsdt = eps * p / (p - omeps * es) ** 2 * desdt
+    tc = t - CST.tmelt
+    hl = CST.hlatv - 2369.0 * tc if t >= CST.tmelt else CST.hlatv + min(-tc / 20.0, 1.0) * CST.hlatf
+    assert gam == pytest.approx(hl / CST.cpws * dqsdt, rel=0.015)
+
+
+def test_qsat_caps_at_one_when_es_exceeds_pressure():
+    es, qs, gam = _qsat(360.0, 3e4)
+    assert qs == 1.0 and gam == 0.0
+
+
+# --------------------------------------------------------------------- conden
+def _brute_conden(p, thl, qt, iters=200):
+    """Saturation adjustment by plain fixed-point on T with the oracle's qsat."""
+    ex = (p / CST.p00) ** CST.rovcp
+    tc = thl * ex
+    nu = max(min((268.0 - tc) / 20.0, 1.0), 0.0)
+    leff = (1 - nu) * CST.xlv + nu * CST.xls
+    if _qsat(tc, p)[1] >= qt:
+        return tc, qt, 0.0
+    t = tc
+    for _ in range(iters):      # T = tc + (L/cp)(qt - qs(T)), damped
+        t = 0.5 * t + 0.5 * (tc + leff / CST.cp * (qt - _qsat(t, p)[1]))
+    qc = qt - _qsat(t, p)[1]
+    return t, qt - qc, qc
+
+
+@pytest.mark.parametrize("p,thl,qt", [(9.5e4, 298.0, 0.018), (8.0e4, 300.0, 0.016), (6.0e4, 312.0, 0.006),
+                                      (7.0e4, 300.0, 0.004), (4.0e4, 320.0, 0.0015), (9.0e4, 290.0, 0.005)])
+def test_conden_invariants_and_brute_force(p, thl, qt):
+    th, qv, ql, qi, idc = _conden(p, thl, qt)
+    assert idc == 0
+    ex = (p / CST.p00) ** CST.rovcp
+    t = th * ex
+    assert qv + ql + qi == pytest.approx(qt, rel=1e-15)              # water conserved
+    tb, qvb, qcb = _brute_conden(p, thl, qt)
+    assert t == pytest.approx(tb, abs=1e-8)
+    if ql + qi > 0:
+        tc = thl * ex
+        nu = max(min((268.0 - tc) / 20.0, 1.0), 0.0)
+        assert qi / (ql + qi) == pytest.approx(nu, abs=1e-14)            # ice fraction ramp 248-268 K
+        assert qv == pytest.approx(_qsat(t, p)[1], rel=1e-9)             # saturated
+        leff = (1 - nu) * CST.xlv + nu * CST.xls
+        assert t - leff / CST.cp * (ql + qi) == pytest.approx(tc, abs=1e-6)   # thl conserved
+    else:
+        assert qv == qt and th == pytest.approx(thl, rel=1e-15)
+
+
+def test_conden_flags_unconverged_adjustment():
+    # absurd supersaturation: the 10-step Newton cannot close the 1 K residual
+    assert _conden(9.0e4, 200.0, 0.5)[4] == 1
+
+
+# ----------------------------------------------------------------------- roots
+@pytest.mark.parametrize("a,b,c", [(1.0, -3.0, 2.0), (2.0, 5.0, -3.0), (1e-3, 4.0, 1.0), (3.0, 1e-9, -2.0)])
+def test_roots_against_numpy(a, b, c):
+    r1, r2, st = D(), D(), C.c_int()
+    L.orc_roots(a, b, c, C.byref(r1), C.byref(r2), C.byref(st))
+    assert st.value == 0
+    np.testing.assert_allclose(sorted([r1.value, r2.value]), sorted(np.roots([a, b, c]).real), rtol=1e-12)
+
+
+def test_roots_status_codes():
+    r1, r2, st = D(), D(), C.c_int()
+    L.orc_roots(0.0, 0.0, 1.0, C.byref(r1), C.byref(r2), C.byref(st))
+    assert st.value == 1
+    L.orc_roots(1.0, 0.0, 1.0, C.byref(r1), C.byref(r2), C.byref(st))
+    assert st.value == 2
+    L.orc_roots(1.0, 1.0, 1.0, C.byref(r1), C.byref(r2), C.byref(st))
+    assert st.value == 3
+    L.orc_roots(0.0, 2.0, -4.0, C.byref(r1), C.byref(r2), C.byref(st))
+    assert st.value == 0 and r1.value == 2.0 and r2.value == 2.0
+
+
+# ----------------------------------------------------------------------- slope
+def test_slope_is_minmod_of_adjacent_gradients():
+    mkx = 8
+    p = np.r_[0.0, 1e5 - 1e4 * np.arange(mkx)]          # index 0 unused
+    f = np.r_[0.0, 1.0, 2.0, 4.0, 3.0, 3.5, 4.0, 6.0, 6.5]
+    out = np.zeros(mkx + 1)
+    dp = lambda x: x.ctypes.data_as(C.POINTER(D))
+    L.orc_slope(mkx, dp(f), dp(p), dp(out))
+    g = np.diff(f[1:]) / np.diff(p[1:])                 # gradient between k and k+1
+    exp = np.zeros(mkx + 1)
+    below = g[0]
+    for k in range(2, mkx + 1):
+        above = g[k - 2]
+        exp[k - 1] = max(0, min(above, below)) if above > 0 else min(0, max(above, below))
+        below = above
+    exp[mkx] = exp[mkx - 1]
+    np.testing.assert_array_equal(out, exp)
+    # at a local maximum of f the slope vanishes (minmod)
+    assert out[3] == 0.0
+
+
+# -------------------------------------------------------------------- qsinvert
+@pytest.mark.parametrize("qt,thl,ps", [(0.016, 300.0, 1e5), (0.010, 298.0, 1e5), (0.004, 310.0, 9e4)])
+def test_qsinvert_returns_saturation_pressure(qt, thl, ps):
+    plcl = L.orc_qsinvert(C.byref(CST), qt, thl, ps)
+    t = thl * (plcl / CST.p00) ** CST.rovcp
+    assert _qsat(t, plcl)[1] == pytest.approx(qt, rel=2e-4)      # converged to |dps| <= 1 Pa
+    assert 3e4 < plcl < ps
+
+
+def test_qsinvert_dry_source_falls_back_to_psmin():
+    assert L.orc_qsinvert(C.byref(CST), 1e-6, 300.0, 1e5) == 1e4
+
+
+# ----------------------------------------------------- implicit CIN, closure
+@pytest.mark.parametrize("k", [0.05, 0.5, 1.0, 3.0])
+def test_compute_alpha_is_lambert_w(k):
+    # alpha = exp(-alpha*k)  <=>  alpha = W(k)/k
+    a = L.orc_compute_alpha(k, 1.0)
+    assert a == pytest.approx((sps.lambertw(k).real / k), rel=1e-12)
+
+
+def test_erfc_matches_scipy():
+    for x in (0.0, 0.3, 0.906, 1.7, 2.9):
+        assert math.erfc(x) == pytest.approx(sps.erfc(x), rel=1e-15)
+
+
+@pytest.mark.parametrize("mulcl,mulow", [(1.2, 0.906), (2.0, 1.0), (0.8, 0.906)])
+def test_compute_mumin2_solves_ufrclcl_equation(mulcl, mulow):
+    rmax = 0.10
+    mu = L.orc_compute_mumin2(mulcl, rmax, mulow)
+    ex, ef = math.exp(-mu * mu), sps.erfc(mu)
+    f = 0.5 * (ex / ef) ** 2 - 0.5 * (ex / 2 / rmax) ** 2 - (mulcl * 2.5066 / 2) ** 2
+    assert abs(f) < 1e-9 * (mulcl * 2.5066 / 2) ** 2
+    # at the root the LCL updraft fraction equals rmaxfrac (uwshcu.F90:2106-2167):
+    sig, rho = 1.0, 1.0
+    winv = sig * (2 / 2.5066) * ex / ef
+    cbmf = rho * sig / 2.5066 * ex
+    wlcl = math.sqrt(winv ** 2 - 2 * (mulcl * 1.4142 * sig) ** 2 / 2)
+    # (the scheme uses 2.5066 ~ sqrt(2 pi) and 1.4142 ~ sqrt 2, hence 0.5 % slack)
+    assert cbmf / wlcl / rho == pytest.approx(rmax, rel=5e-3)
+
+
+def test_compute_ppen_zero_vertical_velocity():
+    wtwb, D_, bb, bt, rho, dpen = 0.5, 2e-4, -0.002, -0.01, 1.0, 5000.0
+    x = -L.orc_compute_ppen(wtwb, D_, bb, bt, rho, dpen)
+    SB = (bt - bb) / dpen
+    f = math.exp(-2 * D_ * x) * (wtwb - (bb - SB / (2 * D_)) / (D_ * rho)) + (SB * x + bb - SB / (2 * D_)) / (D_ * rho)
+    assert 0 < x < dpen and abs(f) < 1e-8
+    # small drag: closed-form linear solution x = -0.5 wtwb / s00
+    x2 = -L.orc_compute_ppen(wtwb, 1e-13, bb, bt, rho, dpen)
+    assert x2 == pytest.approx(-0.5 * wtwb / (bb / rho - 1e-13 * wtwb), rel=1e-12)
+
+
+# ------------------------------------------------------------ CIN integrals
+def test_single_cin_and_getbuoy_trapezoids():
+    pb, pt, tb, tt = 9.0e4, 8.5e4, 300.0, 301.0
+    ex = lambda p: (p / CST.p00) ** CST.rovcp
+    den = pb / (CST.r * tb * ex(pb)) + pt / (CST.r * tt * ex(pt))
+    ub, ut = 299.0, 300.2
+    sc = L.orc_single_cin(C.byref(CST), pb, tb, pt, tt, ub, ut)
+    assert sc == pytest.approx(((1 - ub / tb) + (1 - ut / tt)) * (pb - pt) / den, rel=1e-14)
+    plfc, cin = D(0.0), D(1.0)
+    L.orc_getbuoy(C.byref(CST), pb, tb, pt, tt, ub, ut, C.byref(plfc), C.byref(cin))
+    assert cin.value == pytest.approx(1.0 + sc, rel=1e-14) and plfc.value == 0.0   # both negative: full layer
+    plfc, cin = D(0.0), D(1.0)
+    L.orc_getbuoy(C.byref(CST), pb, tb, pt, tt, 301.0, 302.0, C.byref(plfc), C.byref(cin))
+    assert plfc.value == pb and cin.value == 1.0                                    # both positive: LFC at base
+    plfc, cin = D(0.0), D(0.0)
+    L.orc_getbuoy(C.byref(CST), pb, tb, pt, tt, 299.0, 302.0, C.byref(plfc), C.byref(cin))
+    bb, bt = 299.0 / tb - 1, 302.0 / tt - 1
+    frc = bb / (bb - bt)                              # linear buoyancy crosses zero
+    assert plfc.value == pytest.approx(pb - frc * (pb - pt), rel=1e-14)
+    assert cin.value == pytest.approx(-bb * (pb - plfc.value) / den, rel=1e-13)
+
+
+# --------------------------------------------------------------- fluxbelowinv
+def test_fluxbelowinv_linear_profile_and_surface_zero():
+    mkx, kinv = 10, 4
+    ps0 = np.linspace(1.0e5, 5.0e4, mkx + 1)
+    xflx = np.zeros(mkx + 1)
+    dp = lambda x: x.ctypes.data_as(C.POINTER(D))
+    cbmf, dt = 0.01, 1800.0
+    xsrc, xmean, xtop, xbot = 1.0, 0.6, 0.4, 0.9
+    L.orc_fluxbelowinv(C.byref(CST), cbmf, dp(ps0), mkx, kinv, dt, xsrc, xmean, xtop, xbot, dp(xflx))
+    dpi = ps0[kinv - 1] - ps0[kinv]
+    rpeff = (xmean - xtop) / (xbot - xtop)
+    pinv = ps0[kinv - 1] - rpeff * dpi
+    rr = rpeff / (cbmf * CST.g * dt / dpi)
+    exp = cbmf * (xsrc - xbot) * (ps0[0] - ps0[:kinv]) / (ps0[0] - pinv)
+    if rr <= 1:
+        exp[-1] -= (1 - rr) * cbmf * (xtop - xbot)
+    assert xflx[0] == 0.0
+    np.testing.assert_allclose(xflx[:kinv], exp, rtol=1e-13)
+    assert np.all(xflx[kinv:] == 0.0)
+
+
+# ------------------------------------------------------------ positive moisture
+def test_positive_moisture_conserves_column_water_and_enforces_minima():
+    mkx, dt = 6, 1800.0
+    dp = np.r_[0.0, 2000.0, 1800.0, 1500.0, 1500.0, 1200.0, 1000.0]
+    qv = np.r_[0.0, 0.01, 0.008, 5e-13, 0.004, -1e-6, 1e-3]
+    ql = np.r_[0.0, 1e-5, -2e-6, 0.0, 1e-6, 3e-6, -1e-7]
+    qi = np.r_[0.0, 0.0, 1e-6, -3e-7, 0.0, 0.0, 2e-7]
+    s = np.zeros(mkx + 1)
+    tens = [np.zeros(mkx + 1) for _ in range(4)]
+    w0 = ((qv + ql + qi) * dp)[1:].sum()
+    p = lambda x: x.ctypes.data_as(C.POINTER(D))
+    qvmin = 1e-12
+    L.orc_positive_moisture_single(CST.xlv, CST.xls, mkx, dt, qvmin, 0.0, 0.0, p(dp), p(qv), p(ql), p(qi), p(s),
+                                   *[p(t) for t in tens])
+    qvt, qlt, qit, st = tens
+    # corrective tendencies move water but create none
+    assert (((qvt + qlt + qit) * dp)[1:].sum()) == pytest.approx(0.0, abs=1e-20)
+    assert np.all(ql[1:] >= 0.0) and np.all(qi[1:] >= 0.0) and np.all(qv[1:] >= qvmin)
+    # latent heating of the condensed vapour
+    assert np.allclose(st[1:], CST.xlv * qlt[1:] + CST.xls * qit[1:])
+
+
+# -------------------------------------------------------------------- findsp
+@pytest.mark.parametrize("q,t,p", [(0.010, 295.0, 9.5e4), (0.002, 270.0, 7e4), (0.020, 300.0, 1e5)])
+def test_findsp_wet_bulb_conserves_enthalpy_and_saturates(q, t, p):
+    tsp, qsp, en = D(), D(), D()
+    doit = L.orc_findsp_point(C.byref(CST), q, t, p, C.byref(tsp), C.byref(qsp), C.byref(en))
+    assert doit == 2 and en.value < 1e-4
+    es = L.orc_estblf(C.byref(CST), tsp.value)
+    assert qsp.value == pytest.approx(CST.epsqs * es / (p - (1 - CST.epsqs) * es), rel=1e-12)
+    qs_t = _qsat(t, p)[1]
+    assert (tsp.value < t) == (q < qs_t)            # evaporative cooling when subsaturated
