@@ -70,10 +70,17 @@ class UwshcuError(RuntimeError):
 
 
 def lib():
-    """Load libuwshcu_hip.so; raise if it has not been built."""
+    """Load libuwshcu_hip.so; raise if it has not been built.
+
+    torch is imported first: torch ships its own libamdhip64.so.7 with the same
+    soname as the ROCm installation's, and the first one loaded serves the whole
+    process.  Loading torch's first keeps torch and this library on one HIP
+    runtime (the library is ABI-compatible with either).
+    """
     global _lib
     if _lib is not None:
         return _lib
+    import torch  # noqa: F401  (see above)
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with uwshcu_amd.build.build() (hipcc, gfx950)")
     L = C.CDLL(LIB_PATH)

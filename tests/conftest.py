@@ -2,6 +2,7 @@ import os
 import sys
 
 import pytest
+import torch  # noqa: F401  -- load torch's HIP runtime before any HIP library the tests open
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "hpc-and-climate-model_amd"), os.path.join(ROOT, "oracle"),
