@@ -220,7 +220,7 @@ int uwshcu_default_params(uwshcu_params* p) {
 
 size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision) {
     const size_t elem = (precision == UWSHCU_FP64) ? sizeof(double) : sizeof(float);
-    return (size_t)uwshcu::ws_arrays(ncnst) * (size_t)(pver + 1) * elem;
+    return uwshcu::ws_bytes_per_column(pver, ncnst, elem);
 }
 
 int uwshcu_ctx_create(const uwshcu_consts* c, const double* estbl, int device, uwshcu_ctx** out) {

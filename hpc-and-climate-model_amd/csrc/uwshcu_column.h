@@ -32,20 +32,31 @@
 
 namespace uwshcu {
 
-// Workspace arrays (each mkx+1 rows).  A_* hold the environment of the input
-// state, B_* the environment re-derived from the iteration-1 provisional state
-// (uwshcu.F90:4285-4327), used only for the second CIN evaluation.
-enum WsArray : int {
+// Workspace arrays (each mkx+1 rows).  The environment arrays are FP64 in both
+// precisions: A_* hold the environment of the input state, B_* the environment
+// re-derived from the iteration-1 provisional state (uwshcu.F90:4285-4327), used
+// only for the second CIN evaluation; E_QVSTAR is the positive-moisture fix-up's
+// provisional vapour.  They feed the CIN and cloud-base stages, which run in FP64.
+enum EnvArray : int {
     A_THL0, A_QT0, A_SSTHL0, A_SSQT0, A_SSU0, A_SSV0, A_THV0BOT, A_THV0TOP, A_THVL0BOT, A_THVL0TOP,
     A_EXN0, A_EXNS0,
     B_THL0, B_QT0, B_SSTHL0, B_SSQT0, B_THV0BOT, B_THV0TOP, B_THVL0BOT, B_THVL0TOP,
+    E_QVSTAR,
+    ENV_ARRAYS
+};
+// Plume and flux working arrays, in the bulk type R.
+enum WsArray : int {
     W_EMF, W_THLU, W_QTU, W_UU, W_VU, W_THVU, W_UFRC, W_REI, W_FER, W_FDR, W_DWTEN, W_DITEN,
     W_THLUE, W_QTUE, W_UUE, W_VUE, W_UFLX, W_VFLX, W_QLSINK, W_QISINK, W_NLSINK, W_NISINK,
-    W_QCL, W_QCI, W_QW0, W_QVSTAR, W_TRFD, W_TRFU,
+    W_QCL, W_QCI, W_QW0, W_TRFD, W_TRFU,
     WS_FIXED
 };
-// Per-constituent blocks follow: SSTR0, TRU, TRUE, TRFLX (ncnst arrays each).
+// Per-constituent blocks follow the fixed plume arrays: SSTR0, TRU, TRUE, TRFLX (ncnst arrays each).
 UWSHCU_HD inline int ws_arrays(int ncnst) { return WS_FIXED + 4 * ncnst; }
+// Workspace bytes per column: FP64 environment block, then the R-typed plume block.
+UWSHCU_HD inline size_t ws_bytes_per_column(int mkx, int ncnst, size_t r_size) {
+    return (size_t)(mkx + 1) * ((size_t)ENV_ARRAYS * sizeof(double) + (size_t)ws_arrays(ncnst) * r_size);
+}
 
 struct FwdArgs {
     int ncol, mkx, ncnst, ldc;
@@ -73,11 +84,14 @@ struct Column {
     const uwshcu_consts& C;
     const int i, iw, mkx, KS, ncnst;
     const size_t ldc, wld;
-    R* __restrict__ ws;
+    double* __restrict__ env;   // ENV_ARRAYS * KS rows of wld doubles
+    R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
 
     UWSHCU_HD Column(const FwdArgs& args, const Thermo& th, int col)
         : a(args), T(th), C(*th.c), i(col), iw(col - args.col0), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
-          ldc((size_t)args.ldc), wld((size_t)args.wld), ws(reinterpret_cast<R*>(args.ws)) {}
+          ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<double*>(args.ws)),
+          ws(reinterpret_cast<R*>(reinterpret_cast<double*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
+                                                                         (size_t)args.wld)) {}
 
     // caller arrays: UW level k <-> CAM row mkx-k (layers k=1..mkx and interfaces k=0..mkx)
     UWSHCU_HD inline size_t row(int k) const { return (size_t)(mkx - k) * ldc + i; }
@@ -103,6 +117,11 @@ struct Column {
         UW_CHECK(m >= 1 && m <= ncnst && k >= 1 && k <= mkx);
         return a.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
     }
+    UWSHCU_HD inline double& E(int arr, int k) const {
+        UW_CHECK(arr >= 0 && arr < ENV_ARRAYS && k >= 0 && k <= mkx && iw >= 0 && (long long)iw < (long long)wld);
+        return env[((size_t)arr * KS + k) * wld + iw];
+    }
+    UWSHCU_HD inline R Er(int arr, int k) const { return R(E(arr, k)); }
     UWSHCU_HD inline R& W(int arr, int k) const {
         UW_CHECK(arr >= 0 && arr < WS_FIXED && k >= 0 && k <= mkx && iw >= 0 && (long long)iw < (long long)wld);
         return ws[((size_t)arr * KS + k) * wld + iw]; }
@@ -115,16 +134,16 @@ struct Column {
         return R(thj * (1.0 + C.zvir * qvj - qlj - qij));
     }
 
-    // minmod slopes (uwshcu.F90:4831-4861) of field(k) -> W(dst, k), k = 1..mkx
+    // minmod slopes (uwshcu.F90:4831-4861) of field(k) -> E(dst, k), k = 1..mkx, in FP64
     template <class F>
     UWSHCU_HD inline void slope(F field, int dst) const {
-        R below = (R(field(2)) - R(field(1))) / (R(in(a.p0, 2)) - R(in(a.p0, 1)));
+        double below = (field(2) - field(1)) / (in(a.p0, 2) - in(a.p0, 1));
         for (int k = 2; k <= mkx; ++k) {
-            const R above = (R(field(k)) - R(field(k - 1))) / (R(in(a.p0, k)) - R(in(a.p0, k - 1)));
-            W(dst, k - 1) = (above > R(0)) ? fmax(R(0), fmin(above, below)) : fmin(R(0), fmax(above, below));
+            const double above = (field(k) - field(k - 1)) / (in(a.p0, k) - in(a.p0, k - 1));
+            E(dst, k - 1) = (above > 0.0) ? fmax(0.0, fmin(above, below)) : fmin(0.0, fmax(above, below));
             below = above;
         }
-        W(dst, mkx) = W(dst, mkx - 1);
+        E(dst, mkx) = E(dst, mkx - 1);
     }
     template <class F>
     UWSHCU_HD inline void slope_tr(F field, int m) const {
@@ -147,16 +166,16 @@ struct Column {
         for (int k = 1; k <= mkx; ++k) {
             double th, qv, ql, qi;
             const double pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
-            const R thl0bot = W(thl, k) + W(ssthl, k) * R(pb - pm);
-            const R qt0bot = W(qt, k) + W(ssqt, k) * R(pb - pm);
+            const double thl0bot = E(thl, k) + E(ssthl, k) * (pb - pm);
+            const double qt0bot = E(qt, k) + E(ssqt, k) * (pb - pm);
             if (T.conden(pb, thl0bot, qt0bot, th, qv, ql, qi)) return false;
-            W(thvb, k) = thv_of(th, qv, ql, qi);
-            W(thvlb, k) = thl0bot * (R(1) + R(C.zvir) * qt0bot);
-            const R thl0top = W(thl, k) + W(ssthl, k) * R(pt - pm);
-            const R qt0top = W(qt, k) + W(ssqt, k) * R(pt - pm);
+            E(thvb, k) = th * (1.0 + C.zvir * qv - ql - qi);
+            E(thvlb, k) = thl0bot * (1.0 + C.zvir * qt0bot);
+            const double thl0top = E(thl, k) + E(ssthl, k) * (pt - pm);
+            const double qt0top = E(qt, k) + E(ssqt, k) * (pt - pm);
             if (T.conden(pt, thl0top, qt0top, th, qv, ql, qi)) return false;
-            W(thvt, k) = thv_of(th, qv, ql, qi);
-            W(thvlt, k) = thl0top * (R(1) + R(C.zvir) * qt0top);
+            E(thvt, k) = th * (1.0 + C.zvir * qv - ql - qi);
+            E(thvlt, k) = thl0top * (1.0 + C.zvir * qt0top);
         }
         return true;
     }
@@ -218,8 +237,8 @@ struct Column {
 template <class R>
 UWSHCU_HD void Column<R>::run() const {
     // uwshcu.F90:1040-1068 parameters
-    const R rkfre = R(1), rmaxfrac = R(0.10), mumin1 = R(0.906), rbuoy = R(1), rdrag = R(1);
-    const R epsvarw = R(5.e-4), PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
+    const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
+    const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
     const R xlv = R(C.xlv), xls = R(C.xls), cp = R(C.cp), zvir = R(C.zvir), r = R(C.r), g = R(C.g);
     const R rle = R(a.par.rle), rkm = R(a.par.rkm), rpen = R(a.par.rpen);
     const R dt = R(a.dt);
@@ -243,18 +262,18 @@ UWSHCU_HD void Column<R>::run() const {
     }
 
     // ---- S0 environment, set A (uwshcu.F90:1255-1299) ----
-    for (int k = 1; k <= mkx; ++k) W(A_EXN0, k) = R(upow(in(a.p0, k) / C.p00, C.rovcp));
-    for (int k = 0; k <= mkx; ++k) W(A_EXNS0, k) = R(upow(in(a.ps0, k) / C.p00, C.rovcp));
+    for (int k = 1; k <= mkx; ++k) E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
+    for (int k = 0; k <= mkx; ++k) E(A_EXNS0, k) = upow(in(a.ps0, k) / C.p00, C.rovcp);
     for (int k = 1; k <= mkx; ++k) {
-        const R qv0 = R(in(a.qv0, k)), ql0 = R(in(a.ql0, k)), qi0 = R(in(a.qi0, k)), t0 = R(in(a.t0, k));
-        const R qt0 = (qv0 + ql0 + qi0);
-        W(A_QT0, k) = qt0;
-        W(A_THL0, k) = (t0 - xlv * ql0 / cp - xls * qi0 / cp) / W(A_EXN0, k);
+        const double qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
+        const double qt0 = (qv0 + ql0 + qi0);
+        E(A_QT0, k) = qt0;
+        E(A_THL0, k) = (t0 - C.xlv * ql0 / C.cp - C.xls * qi0 / C.cp) / E(A_EXN0, k);
     }
-    slope([&](int k) { return W(A_THL0, k); }, A_SSTHL0);
-    slope([&](int k) { return W(A_QT0, k); }, A_SSQT0);
-    slope([&](int k) { return R(in(a.u0, k)); }, A_SSU0);
-    slope([&](int k) { return R(in(a.v0, k)); }, A_SSV0);
+    slope([&](int k) { return E(A_THL0, k); }, A_SSTHL0);
+    slope([&](int k) { return E(A_QT0, k); }, A_SSQT0);
+    slope([&](int k) { return in(a.u0, k); }, A_SSU0);
+    slope([&](int k) { return in(a.v0, k); }, A_SSV0);
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
 
     zero_pass_state();
@@ -307,17 +326,17 @@ UWSHCU_HD void Column<R>::run() const {
                 else dpi = in(a.p0, k) - in(a.p0, k + 1);
                 dpsum = dpsum + dpi;
                 tkeavg = tkeavg + dpi * in(a.tke, k);
-                if (k != 0) thvlmin = fmin(thvlmin, fmin((double)W(e_thvlb, k), (double)W(e_thvlt, k)));
+                if (k != 0) thvlmin = fmin(thvlmin, fmin(E(e_thvlb, k), E(e_thvlt, k)));
             }
             tkeavg = tkeavg / dpsum;
         }
 
         // ---- S3 source air (uwshcu.F90:1509-1516) ----
-        qtsrc = W(e_qt, 1);
+        qtsrc = E(e_qt, 1);
         thvlsrc = thvlmin;
         thlsrc = thvlsrc / (1.0 + C.zvir * qtsrc);
-        usrc = R(in(a.u0, kinv - 1)) + W(A_SSU0, kinv - 1) * R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
-        vsrc = R(in(a.v0, kinv - 1)) + W(A_SSV0, kinv - 1) * R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
+        usrc = in(a.u0, kinv - 1) + E(A_SSU0, kinv - 1) * (in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
+        vsrc = in(a.v0, kinv - 1) + E(A_SSV0, kinv - 1) * (in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
 
         // ---- S4 LCL (uwshcu.F90:1533-1549) ----
         plcl = T.qsinvert(qtsrc, thlsrc, in(a.ps0, 0));
@@ -330,8 +349,8 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S5 thv0lcl (uwshcu.F90:1559-1567) ----
         {
-            const double thl0lcl = (double)W(e_thl, klcl) + (double)W(e_ssthl, klcl) * (plcl - in(a.p0, klcl));
-            const double qt0lcl = (double)W(e_qt, klcl) + (double)W(e_ssqt, klcl) * (plcl - in(a.p0, klcl));
+            const double thl0lcl = E(e_thl, klcl) + E(e_ssthl, klcl) * (plcl - in(a.p0, klcl));
+            const double qt0lcl = E(e_qt, klcl) + E(e_ssqt, klcl) * (plcl - in(a.p0, klcl));
             if (T.conden(plcl, thl0lcl, qt0lcl, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
             thv0lcl = th * (1.0 + C.zvir * qvj - qlj - qij);
         }
@@ -347,23 +366,23 @@ UWSHCU_HD void Column<R>::run() const {
                 if (k < klcl) {
                     thvubot = thvlsrc;
                     thvutop = thvlsrc;
-                    cin = cin + T.single_cin(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop);
+                    cin = cin + T.single_cin(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop);
                 } else if (k == klcl) {
                     thvubot = thvlsrc;
                     thvutop = thvlsrc;
-                    cin = cin + T.single_cin(in(a.ps0, k - 1), W(e_thvb, k), plcl, thv0lcl, thvubot, thvutop);
+                    cin = cin + T.single_cin(in(a.ps0, k - 1), E(e_thvb, k), plcl, thv0lcl, thvubot, thvutop);
                     cinlcl = fmax(cin, 0.0);
                     cin = cinlcl;
                     thvubot = thvlsrc;
                     if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
-                    T.getbuoy(plcl, thv0lcl, in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                    T.getbuoy(plcl, thv0lcl, in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                     if (plfc > 0.0) { klfc = k; break; }
                 } else {
                     thvubot = thvutop;
                     if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
-                    T.getbuoy(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                    T.getbuoy(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                     if (plfc > 0.0) { klfc = k; break; }
                 }
             }
@@ -374,7 +393,7 @@ UWSHCU_HD void Column<R>::run() const {
                 const double thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
                 if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 const double thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
-                T.getbuoy(in(a.ps0, k - 1), W(e_thvb, k), in(a.ps0, k), W(e_thvt, k), thvubot, thvutop, plfc, cin);
+                T.getbuoy(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                 if (plfc > 0.0) { klfc = k; break; }
             }
         }
@@ -414,47 +433,54 @@ UWSHCU_HD void Column<R>::run() const {
         if (klcl < kinv) {
             krel = kinv;
             prel = R(in(a.ps0, krel - 1));
-            thv0rel = W(A_THV0BOT, krel);
+            thv0rel = Er(A_THV0BOT, krel);
         } else {
             krel = klcl;
             prel = R(plcl);
             thv0rel = R(thv0lcl);
         }
 
-        // ---- S9 cloud-base closure (uwshcu.F90:2092-2184) ----
-        const R wcrit = sqrt(R(2) * R(cin) * rbuoy);
-        const R sigmaw = sqrt(rkfre * R(tkeavg) + epsvarw);
-        R mu = wcrit / sigmaw / R(1.4142);
-        if (mu >= R(3)) { exit_code = EX_MU3; goto L333; }
-        const R rho0inv = R(in(a.ps0, kinv - 1)) / (r * W(A_THV0TOP, kinv - 1) * W(A_EXNS0, kinv - 1));
-        cbmf = (rho0inv * sigmaw / R(2.5066)) * uexp(-(mu * mu));
-        const R cbmflimit = R(0.9) * R(in(a.dp0, kinv - 1)) / g / dt;
-        R mumin0 = R(0);
-        if (cbmf > cbmflimit) mumin0 = sqrt(-ulog(R(2.5066) * cbmflimit / rho0inv / sigmaw));
-        mu = fmax(fmax(mu, mumin0), mumin1);
-        const R mulcl = sqrt(R(2) * R(cinlcl) * rbuoy) / R(1.4142) / sigmaw;
-        R mulclstar;
+        // ---- S9 cloud-base closure (uwshcu.F90:2092-2184), FP64 ----
+        R wtw;
         {
-            const R e1 = uexp(-(mu * mu)) / R(2.5066), ef = uerfc(mu);
-            mulclstar = sqrt(fmax(R(0), R(2) * (e1 * e1) * (R(1) / (ef * ef) - R(0.25) / (rmaxfrac * rmaxfrac))));
+            const double wcrit = sqrt(2.0 * cin * 1.0);
+            const double sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
+            double mu = wcrit / sigmaw / 1.4142;
+            if (mu >= 3.0) { exit_code = EX_MU3; goto L333; }
+            const double rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
+            double cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
+            const double cbmflimit = 0.9 * in(a.dp0, kinv - 1) / C.g / dtd;
+            double mumin0 = 0.0;
+            if (cbmf_d > cbmflimit) mumin0 = sqrt(-ulog(2.5066 * cbmflimit / rho0inv / sigmaw));
+            mu = fmax(fmax(mu, mumin0), 0.906);
+            const double mulcl = sqrt(2.0 * cinlcl * 1.0) / 1.4142 / sigmaw;
+            double mulclstar;
+            {
+                const double e1 = uexp(-(mu * mu)) / 2.5066, ef = uerfc(mu);
+                mulclstar = sqrt(fmax(0.0, 2.0 * (e1 * e1) * (1.0 / (ef * ef) - 0.25 / (0.10 * 0.10))));
+            }
+            if (mulcl > 1.e-8 && mulcl > mulclstar) {
+                const double mumin2 = compute_mumin2(mulcl, 0.10, mu);
+                if (mu > mumin2) record_error(a.err, UWSHCU_ERR_MUMIN2, i);
+                mu = fmax(mu, mumin2);
+                if (mu == mumin2) limit_ufrc = 1;
+            }
+            if (mu == 0.906) limit_ufrc = 1;
+            cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
+            const double winv = sigmaw * (2.0 / 2.5066) * uexp(-(mu * mu)) / uerfc(mu);
+            const double wtw_d = winv * winv - 2.0 * cinlcl * 1.0;
+            if (wtw_d <= 0.0) { exit_code = EX_WTW; goto L333; }
+            const double wlcl_d = sqrt(wtw_d);
+            const double ufrclcl_d = cbmf_d / wlcl_d / rho0inv;
+            if (ufrclcl_d <= 0.0001) { exit_code = EX_UFRC; goto L333; }
+            cbmf = R(cbmf_d);
+            wlcl = R(wlcl_d);
+            ufrclcl = R(ufrclcl_d);
+            wtw = R(wtw_d);
+            for (int k = kinv - 1; k <= krel - 1; ++k) { out(a.umf, k) = cbmf_d; out(a.wu, k) = winv; }
         }
-        if (mulcl > R(1.e-8) && mulcl > mulclstar) {
-            const R mumin2 = compute_mumin2(mulcl, rmaxfrac, mu);
-            if (mu > mumin2) record_error(a.err, UWSHCU_ERR_MUMIN2, i);
-            mu = fmax(mu, mumin2);
-            if (mu == mumin2) limit_ufrc = 1;
-        }
-        if (mu == mumin1) limit_ufrc = 1;
-        cbmf = (rho0inv * sigmaw / R(2.5066)) * uexp(-(mu * mu));
-        const R winv = sigmaw * (R(2) / R(2.5066)) * uexp(-(mu * mu)) / uerfc(mu);
-        R wtw = winv * winv - R(2) * R(cinlcl) * rbuoy;
-        if (wtw <= R(0)) { exit_code = EX_WTW; goto L333; }
-        wlcl = sqrt(wtw);
-        ufrclcl = cbmf / wlcl / rho0inv;
         const R wrel = wlcl;
-        if (ufrclcl <= R(0.0001)) { exit_code = EX_UFRC; goto L333; }
         W(W_UFRC, krel - 1) = ufrclcl;
-        for (int k = kinv - 1; k <= krel - 1; ++k) { out(a.umf, k) = (double)cbmf; out(a.wu, k) = (double)winv; }
 
         // ---- S10 release-level updraft (uwshcu.F90:2205-2255) ----
         W(W_EMF, krel - 1) = R(0);
@@ -467,16 +493,16 @@ UWSHCU_HD void Column<R>::run() const {
         {
             R uplus = R(0), vplus = R(0);
             if (krel == kinv) {
-                uplus = PGFc * W(A_SSU0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
-                vplus = PGFc * W(A_SSV0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
+                uplus = PGFc * Er(A_SSU0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
+                vplus = PGFc * Er(A_SSV0, kinv) * (prel - R(in(a.ps0, kinv - 1)));
             } else {
                 const int kend = (krel - 1 > kinv) ? krel - 1 : kinv;
                 for (int k = kinv; k <= kend; ++k) {
-                    uplus = uplus + PGFc * W(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
-                    vplus = vplus + PGFc * W(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                    uplus = uplus + PGFc * Er(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                    vplus = vplus + PGFc * Er(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
                 }
-                uplus = uplus + PGFc * W(A_SSU0, krel) * (prel - R(in(a.ps0, krel - 1)));
-                vplus = vplus + PGFc * W(A_SSV0, krel) * (prel - R(in(a.ps0, krel - 1)));
+                uplus = uplus + PGFc * Er(A_SSU0, krel) * (prel - R(in(a.ps0, krel - 1)));
+                vplus = vplus + PGFc * Er(A_SSV0, krel) * (prel - R(in(a.ps0, krel - 1)));
             }
             W(W_UU, krel - 1) = R(usrc) + uplus;
             W(W_VU, krel - 1) = R(vsrc) + vplus;
@@ -494,10 +520,10 @@ UWSHCU_HD void Column<R>::run() const {
             R dpe = prel - R(in(a.ps0, krel));
             R exne = R(T.exnf(pe));
             R thvebot = thv0rel;
-            R thle = W(A_THL0, krel) + W(A_SSTHL0, krel) * (pe - R(in(a.p0, krel)));
-            R qte = W(A_QT0, krel) + W(A_SSQT0, krel) * (pe - R(in(a.p0, krel)));
-            R ue = R(in(a.u0, krel)) + W(A_SSU0, krel) * (pe - R(in(a.p0, krel)));
-            R ve = R(in(a.v0, krel)) + W(A_SSV0, krel) * (pe - R(in(a.p0, krel)));
+            R thle = Er(A_THL0, krel) + Er(A_SSTHL0, krel) * (pe - R(in(a.p0, krel)));
+            R qte = Er(A_QT0, krel) + Er(A_SSQT0, krel) * (pe - R(in(a.p0, krel)));
+            R ue = R(in(a.u0, krel)) + Er(A_SSU0, krel) * (pe - R(in(a.p0, krel)));
+            R ve = R(in(a.v0, krel)) + Er(A_SSV0, krel) * (pe - R(in(a.p0, krel)));
             // tre(m) = tr0(krel,m) + sstr0(krel,m)*(pe - p0(krel)) is recomputed where used
             R pe_tr = pe - R(in(a.p0, krel));
             int ktr = krel;            // layer whose (tr0, sstr0) define tre, and offset pe_tr
@@ -587,7 +613,7 @@ UWSHCU_HD void Column<R>::run() const {
                     umfk = umf_km1 * uexp(dpe * (ferk - fdrk));
                     W(W_EMF, k) = R(0);
                     R thluk, qtuk, uuk, vuk;
-                    const R ss_thl = W(A_SSTHL0, k), ss_qt = W(A_SSQT0, k), ss_u = W(A_SSU0, k), ss_v = W(A_SSV0, k);
+                    const R ss_thl = Er(A_SSTHL0, k), ss_qt = Er(A_SSQT0, k), ss_u = Er(A_SSU0, k), ss_v = Er(A_SSV0, k);
                     if (ferk * dpe < R(1.e-4)) {
                         thluk = W(W_THLU, km1) + (thle + ss_thl * dpe / R(2) - W(W_THLU, km1)) * ferk * dpe;
                         qtuk = W(W_QTU, km1) + (qte + ss_qt * dpe / R(2) - W(W_QTU, km1)) * ferk * dpe;
@@ -620,7 +646,7 @@ UWSHCU_HD void Column<R>::run() const {
                         const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
                         const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
                         qtuk = qtuk - exql - exqi;
-                        thluk = thluk + (xlv / cp / W(A_EXNS0, k)) * exql + (xls / cp / W(A_EXNS0, k)) * exqi;
+                        thluk = thluk + (xlv / cp / Er(A_EXNS0, k)) * exql + (xls / cp / Er(A_EXNS0, k)) * exqi;
                         W(W_DWTEN, k) = exql;
                         W(W_DITEN, k) = exqi;
                     } else {
@@ -636,7 +662,7 @@ UWSHCU_HD void Column<R>::run() const {
                     const R thvuk = thv_of(th, qvj, qlj, qij);
                     W(W_THVU, k) = thvuk;
                     bogbot = rbuoy * (W(W_THVU, km1) / thvebot - R(1));
-                    bogtop = rbuoy * (thvuk / W(A_THV0TOP, k) - R(1));
+                    bogtop = rbuoy * (thvuk / Er(A_THV0TOP, k) - R(1));
                     const R delbog = bogtop - bogbot;
                     drage = ferk * (R(1) + rdrag);
                     const R expfac = uexp(R(-2) * drage * dpe);
@@ -661,7 +687,7 @@ UWSHCU_HD void Column<R>::run() const {
                     const R wuk = sqrt(wtw);
                     out(a.wu, k) = (double)wuk;
                     if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
-                    const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (W(A_THV0BOT, k + 1) + W(A_THV0TOP, k)) * W(A_EXNS0, k));
+                    const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
                     R ufrck = umfk / (rhos0j * wuk);
                     if (ufrck > rmaxfrac) {
                         limit_ufrc = 1;
@@ -674,10 +700,10 @@ UWSHCU_HD void Column<R>::run() const {
                 }
                 pe = R(in(a.p0, k + 1));
                 dpe = R(in(a.dp0, k + 1));
-                exne = W(A_EXN0, k + 1);
-                thvebot = W(A_THV0BOT, k + 1);
-                thle = W(A_THL0, k + 1);
-                qte = W(A_QT0, k + 1);
+                exne = Er(A_EXN0, k + 1);
+                thvebot = Er(A_THV0BOT, k + 1);
+                thle = Er(A_THL0, k + 1);
+                qte = Er(A_QT0, k + 1);
                 ue = R(in(a.u0, k + 1));
                 ve = R(in(a.v0, k + 1));
                 ktr = k + 1;
@@ -711,8 +737,8 @@ UWSHCU_HD void Column<R>::run() const {
             }
             if (ppen == -R(in(a.dp0, kpen)) || ppen == R(0)) limit_ppen = 1;
             {
-                const R ferp = W(W_FER, kpen), thl0p = W(A_THL0, kpen), ssthlp = W(A_SSTHL0, kpen);
-                const R qt0p = W(A_QT0, kpen), ssqtp = W(A_SSQT0, kpen);
+                const R ferp = W(W_FER, kpen), thl0p = Er(A_THL0, kpen), ssthlp = Er(A_SSTHL0, kpen);
+                const R qt0p = Er(A_QT0, kpen), ssqtp = Er(A_SSQT0, kpen);
                 if (ferp * (-ppen) < R(1.e-4)) {
                     thlu_top = W(W_THLU, kpen - 1) + (thl0p + ssthlp * (-ppen) / R(2) - W(W_THLU, kpen - 1)) * ferp * (-ppen);
                     qtu_top = W(W_QTU, kpen - 1) + (qt0p + ssqtp * (-ppen) / R(2) - W(W_QTU, kpen - 1)) * ferp * (-ppen);
@@ -740,7 +766,7 @@ UWSHCU_HD void Column<R>::run() const {
                     W(W_DWTEN, kpen) = R(0);
                     W(W_DITEN, kpen) = R(0);
                 }
-                const R rhos0j = R(in(a.ps0, kpen - 1)) / (r * R(0.5) * (W(A_THV0BOT, kpen) + W(A_THV0TOP, kpen - 1)) * W(A_EXNS0, kpen - 1));
+                const R rhos0j = R(in(a.ps0, kpen - 1)) / (r * R(0.5) * (Er(A_THV0BOT, kpen) + Er(A_THV0TOP, kpen - 1)) * Er(A_EXNS0, kpen - 1));
                 cush = (double)(R(in(a.zs0, kpen - 1)) - ppen / rhos0j / g);
                 scaleh = R(cush);
             }
@@ -760,7 +786,7 @@ UWSHCU_HD void Column<R>::run() const {
             for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = WT(1, m, k);
         }
         for (int k = kpen - 1; k >= kbup; --k) {
-            const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (W(A_THV0BOT, k + 1) + W(A_THV0TOP, k)) * W(A_EXNS0, k));
+            const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
             const R umfk = R(out(a.umf, k));
             if (k == kpen - 1) {
                 const R x = umfk * ppen * W(W_REI, kpen) * rpen;
@@ -769,10 +795,10 @@ UWSHCU_HD void Column<R>::run() const {
                 if (x < lim2) limit_emf = 1;
                 W(W_EMF, k) = fmax(fmax(x, R(-0.1) * rhos0j), lim2);
                 const R dpk = R(in(a.ps0, k) - in(a.p0, kpen));
-                W(W_THLUE, k) = W(A_THL0, kpen) + W(A_SSTHL0, kpen) * dpk;
-                W(W_QTUE, k) = W(A_QT0, kpen) + W(A_SSQT0, kpen) * dpk;
-                W(W_UUE, k) = R(in(a.u0, kpen)) + W(A_SSU0, kpen) * dpk;
-                W(W_VUE, k) = R(in(a.v0, kpen)) + W(A_SSV0, kpen) * dpk;
+                W(W_THLUE, k) = Er(A_THL0, kpen) + Er(A_SSTHL0, kpen) * dpk;
+                W(W_QTUE, k) = Er(A_QT0, kpen) + Er(A_SSQT0, kpen) * dpk;
+                W(W_UUE, k) = R(in(a.u0, kpen)) + Er(A_SSU0, kpen) * dpk;
+                W(W_VUE, k) = R(in(a.v0, kpen)) + Er(A_SSV0, kpen) * dpk;
                 for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = R(trin(m, kpen)) + WT(0, m, kpen) * dpk;
             } else {
                 const R emf1 = W(W_EMF, k + 1);
@@ -783,15 +809,15 @@ UWSHCU_HD void Column<R>::run() const {
                 const R emfk = fmax(fmax(x, R(-0.1) * rhos0j), lim2);
                 W(W_EMF, k) = emfk;
                 if (fabs(emfk) > fabs(emf1)) {
-                    W(W_THLUE, k) = (W(W_THLUE, k + 1) * emf1 + W(A_THL0, k + 1) * (emfk - emf1)) / emfk;
-                    W(W_QTUE, k) = (W(W_QTUE, k + 1) * emf1 + W(A_QT0, k + 1) * (emfk - emf1)) / emfk;
+                    W(W_THLUE, k) = (W(W_THLUE, k + 1) * emf1 + Er(A_THL0, k + 1) * (emfk - emf1)) / emfk;
+                    W(W_QTUE, k) = (W(W_QTUE, k + 1) * emf1 + Er(A_QT0, k + 1) * (emfk - emf1)) / emfk;
                     W(W_UUE, k) = (W(W_UUE, k + 1) * emf1 + R(in(a.u0, k + 1)) * (emfk - emf1)) / emfk;
                     W(W_VUE, k) = (W(W_VUE, k + 1) * emf1 + R(in(a.v0, k + 1)) * (emfk - emf1)) / emfk;
                     for (int m = 1; m <= ncnst; ++m)
                         WT(2, m, k) = (WT(2, m, k + 1) * emf1 + R(trin(m, k + 1)) * (emfk - emf1)) / emfk;
                 } else {
-                    W(W_THLUE, k) = W(A_THL0, k + 1);
-                    W(W_QTUE, k) = W(A_QT0, k + 1);
+                    W(W_THLUE, k) = Er(A_THL0, k + 1);
+                    W(W_QTUE, k) = Er(A_QT0, k + 1);
                     W(W_UUE, k) = R(in(a.u0, k + 1));
                     W(W_VUE, k) = R(in(a.v0, k + 1));
                     for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = R(trin(m, k + 1));
@@ -823,17 +849,17 @@ UWSHCU_HD void Column<R>::run() const {
             };
             const R dtop = R(in(a.ps0, kinv) - in(a.p0, kinv + 1));
             const R dbot = R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
-            fluxbelowinv(R(qtsrc), W(A_QT0, kinv), W(A_QT0, kinv + 1) + W(A_SSQT0, kinv + 1) * dtop,
-                         W(A_QT0, kinv - 1) + W(A_SSQT0, kinv - 1) * dbot,
+            fluxbelowinv(R(qtsrc), Er(A_QT0, kinv), Er(A_QT0, kinv + 1) + Er(A_SSQT0, kinv + 1) * dtop,
+                         Er(A_QT0, kinv - 1) + Er(A_SSQT0, kinv - 1) * dbot,
                          [&](int k, R x) { out(a.qtflx, k) = (double)x; });
-            fluxbelowinv(R(thlsrc), W(A_THL0, kinv), W(A_THL0, kinv + 1) + W(A_SSTHL0, kinv + 1) * dtop,
-                         W(A_THL0, kinv - 1) + W(A_SSTHL0, kinv - 1) * dbot,
-                         [&](int k, R x) { out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * x); });
-            fluxbelowinv(R(usrc), R(in(a.u0, kinv)), R(in(a.u0, kinv + 1)) + W(A_SSU0, kinv + 1) * dtop,
-                         R(in(a.u0, kinv - 1)) + W(A_SSU0, kinv - 1) * dbot,
+            fluxbelowinv(R(thlsrc), Er(A_THL0, kinv), Er(A_THL0, kinv + 1) + Er(A_SSTHL0, kinv + 1) * dtop,
+                         Er(A_THL0, kinv - 1) + Er(A_SSTHL0, kinv - 1) * dbot,
+                         [&](int k, R x) { out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * x); });
+            fluxbelowinv(R(usrc), R(in(a.u0, kinv)), R(in(a.u0, kinv + 1)) + Er(A_SSU0, kinv + 1) * dtop,
+                         R(in(a.u0, kinv - 1)) + Er(A_SSU0, kinv - 1) * dbot,
                          [&](int k, R x) { W(W_UFLX, k) = x; });
-            fluxbelowinv(R(vsrc), R(in(a.v0, kinv)), R(in(a.v0, kinv + 1)) + W(A_SSV0, kinv + 1) * dtop,
-                         R(in(a.v0, kinv - 1)) + W(A_SSV0, kinv - 1) * dbot,
+            fluxbelowinv(R(vsrc), R(in(a.v0, kinv)), R(in(a.v0, kinv + 1)) + Er(A_SSV0, kinv + 1) * dtop,
+                         R(in(a.v0, kinv - 1)) + Er(A_SSV0, kinv - 1) * dbot,
                          [&](int k, R x) { W(W_VFLX, k) = x; });
             for (int m = 1; m <= ncnst; ++m) {
                 fluxbelowinv(R(trin(m, 1)), R(trin(m, kinv)), R(trin(m, kinv + 1)) + WT(0, m, kinv + 1) * dtop,
@@ -846,12 +872,12 @@ UWSHCU_HD void Column<R>::run() const {
             for (int k = kinv; k <= krel - 1; ++k) {
                 const int kp1 = k + 1;
                 const R d = R(in(a.ps0, k) - in(a.p0, kp1));
-                out(a.qtflx, k) = (double)(cbmf * (R(qtsrc) - (W(A_QT0, kp1) + W(A_SSQT0, kp1) * d)));
-                out(a.slflx, k) = (double)(cbmf * (R(thlsrc) - (W(A_THL0, kp1) + W(A_SSTHL0, kp1) * d)) * cp * W(A_EXNS0, k));
-                uplus = uplus + PGFc * W(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
-                vplus = vplus + PGFc * W(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
-                W(W_UFLX, k) = cbmf * (R(usrc) + uplus - (R(in(a.u0, kp1)) + W(A_SSU0, kp1) * d));
-                W(W_VFLX, k) = cbmf * (R(vsrc) + vplus - (R(in(a.v0, kp1)) + W(A_SSV0, kp1) * d));
+                out(a.qtflx, k) = (double)(cbmf * (R(qtsrc) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
+                out(a.slflx, k) = (double)(cbmf * (R(thlsrc) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)) * cp * Er(A_EXNS0, k));
+                uplus = uplus + PGFc * Er(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                vplus = vplus + PGFc * Er(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
+                W(W_UFLX, k) = cbmf * (R(usrc) + uplus - (R(in(a.u0, kp1)) + Er(A_SSU0, kp1) * d));
+                W(W_VFLX, k) = cbmf * (R(vsrc) + vplus - (R(in(a.v0, kp1)) + Er(A_SSV0, kp1) * d));
                 for (int m = 1; m <= ncnst; ++m)
                     WT(3, m, k) = cbmf * (R(trin(m, 1)) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
             }
@@ -860,20 +886,20 @@ UWSHCU_HD void Column<R>::run() const {
             const int kp1 = k + 1;
             const R d = R(in(a.ps0, k) - in(a.p0, kp1));
             const R umfk = R(out(a.umf, k));
-            out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * umfk * (W(W_THLU, k) - (W(A_THL0, kp1) + W(A_SSTHL0, kp1) * d)));
-            out(a.qtflx, k) = (double)(umfk * (W(W_QTU, k) - (W(A_QT0, kp1) + W(A_SSQT0, kp1) * d)));
-            W(W_UFLX, k) = umfk * (W(W_UU, k) - (R(in(a.u0, kp1)) + W(A_SSU0, kp1) * d));
-            W(W_VFLX, k) = umfk * (W(W_VU, k) - (R(in(a.v0, kp1)) + W(A_SSV0, kp1) * d));
+            out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * umfk * (W(W_THLU, k) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)));
+            out(a.qtflx, k) = (double)(umfk * (W(W_QTU, k) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
+            W(W_UFLX, k) = umfk * (W(W_UU, k) - (R(in(a.u0, kp1)) + Er(A_SSU0, kp1) * d));
+            W(W_VFLX, k) = umfk * (W(W_VU, k) - (R(in(a.v0, kp1)) + Er(A_SSV0, kp1) * d));
             for (int m = 1; m <= ncnst; ++m)
                 WT(3, m, k) = umfk * (WT(1, m, k) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
         }
         for (int k = kbup; k <= kpen - 1; ++k) {
             const R d = R(in(a.ps0, k) - in(a.p0, k));
             const R emfk = W(W_EMF, k);
-            out(a.slflx, k) = (double)(cp * W(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (W(A_THL0, k) + W(A_SSTHL0, k) * d)));
-            out(a.qtflx, k) = (double)(emfk * (W(W_QTUE, k) - (W(A_QT0, k) + W(A_SSQT0, k) * d)));
-            W(W_UFLX, k) = emfk * (W(W_UUE, k) - (R(in(a.u0, k)) + W(A_SSU0, k) * d));
-            W(W_VFLX, k) = emfk * (W(W_VUE, k) - (R(in(a.v0, k)) + W(A_SSV0, k) * d));
+            out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (Er(A_THL0, k) + Er(A_SSTHL0, k) * d)));
+            out(a.qtflx, k) = (double)(emfk * (W(W_QTUE, k) - (Er(A_QT0, k) + Er(A_SSQT0, k) * d)));
+            W(W_UFLX, k) = emfk * (W(W_UUE, k) - (R(in(a.u0, k)) + Er(A_SSU0, k) * d));
+            W(W_VFLX, k) = emfk * (W(W_VUE, k) - (R(in(a.v0, k)) + Er(A_SSV0, k) * d));
             for (int m = 1; m <= ncnst; ++m)
                 WT(3, m, k) = emfk * (WT(2, m, k) - (R(trin(m, k)) + WT(0, m, k) * d));
         }
@@ -899,8 +925,8 @@ UWSHCU_HD void Column<R>::run() const {
                         thlten_sub = qtten_sub = qlten_sub = qiten_sub = nlten_sub = niten_sub = R(0);
                     } else {
                         const R dp = R(in(a.p0, k) - in(a.p0, k + 1));
-                        thlten_sub = g * comsub * (W(A_THL0, k + 1) - W(A_THL0, k)) / dp;
-                        qtten_sub = g * comsub * (W(A_QT0, k + 1) - W(A_QT0, k)) / dp;
+                        thlten_sub = g * comsub * (Er(A_THL0, k + 1) - Er(A_THL0, k)) / dp;
+                        qtten_sub = g * comsub * (Er(A_QT0, k + 1) - Er(A_QT0, k)) / dp;
                         qlten_sub = g * comsub * (R(in(a.ql0, k + 1)) - R(in(a.ql0, k))) / dp;
                         qiten_sub = g * comsub * (R(in(a.qi0, k + 1)) - R(in(a.qi0, k))) / dp;
                         nlten_sub = g * comsub * (trz(ixl, k + 1) - trz(ixl, k)) / dp;
@@ -911,16 +937,16 @@ UWSHCU_HD void Column<R>::run() const {
                         thlten_sub = qtten_sub = qlten_sub = qiten_sub = nlten_sub = niten_sub = R(0);
                     } else {
                         const R dp = R(in(a.p0, k - 1) - in(a.p0, k));
-                        thlten_sub = g * comsub * (W(A_THL0, k) - W(A_THL0, k - 1)) / dp;
-                        qtten_sub = g * comsub * (W(A_QT0, k) - W(A_QT0, k - 1)) / dp;
+                        thlten_sub = g * comsub * (Er(A_THL0, k) - Er(A_THL0, k - 1)) / dp;
+                        qtten_sub = g * comsub * (Er(A_QT0, k) - Er(A_QT0, k - 1)) / dp;
                         qlten_sub = g * comsub * (R(in(a.ql0, k)) - R(in(a.ql0, k - 1))) / dp;
                         qiten_sub = g * comsub * (R(in(a.qi0, k)) - R(in(a.qi0, k - 1))) / dp;
                         nlten_sub = g * comsub * (trz(ixl, k) - trz(ixl, k - 1)) / dp;
                         niten_sub = g * comsub * (trz(ixi, k) - trz(ixi, k - 1)) / dp;
                     }
                 }
-                const R thl_prog = W(A_THL0, k) + thlten_sub * dt;
-                const R qt_prog = fmax(W(A_QT0, k) + qtten_sub * dt, R(1.e-12));
+                const R thl_prog = Er(A_THL0, k) + thlten_sub * dt;
+                const R qt_prog = fmax(Er(A_QT0, k) + qtten_sub * dt, R(1.e-12));
                 if (T.conden(in(a.p0, k), thl_prog, qt_prog, th, qvj, qlj, qij)) { exit_code = EX_SUBCONDEN; goto L333; }
                 W(W_QLSINK, k) = fmax(qlten_sub, -R(in(a.ql0, k)) / dt);
                 W(W_QISINK, k) = fmax(qiten_sub, -R(in(a.qi0, k)) / dt);
@@ -1151,18 +1177,18 @@ UWSHCU_HD void Column<R>::run() const {
                 out(a.qlten, k) = qlten;
                 out(a.qiten, k) = qiten;
                 out(a.sten, k) = sten;
-                W(W_QVSTAR, k) = R(qv);
+                E(E_QVSTAR, k) = qv;
             }
             if (dqv > 1.e-20) {
                 double sum = 0.0;
                 for (int k = 1; k <= mkx; ++k) {
-                    const double qv = (double)W(W_QVSTAR, k);
+                    const double qv = E(E_QVSTAR, k);
                     if (qv > 2.0 * qvmin) sum = sum + qv * in(a.dp0, k);
                 }
                 const double aa = dqv * in(a.dp0, 1) / fmax(1.e-20, sum);
                 if (aa < 0.5) {
                     for (int k = 1; k <= mkx; ++k) {
-                        const double qv = (double)W(W_QVSTAR, k);
+                        const double qv = E(E_QVSTAR, k);
                         if (qv > 2.0 * qvmin) {
                             const double dum = aa * qv;
                             out(a.qvten, k) = out(a.qvten, k) - dum / dtd;
@@ -1243,16 +1269,16 @@ UWSHCU_HD void Column<R>::run() const {
         if (iter == 1) {
             cush_s = cush;
             for (int k = 1; k <= mkx; ++k) {
-                const R qv0s = R(in(a.qv0, k)) + R(out(a.qvten, k)) * dt;
-                const R ql0s = R(in(a.ql0, k)) + R(out(a.qlten, k)) * dt;
-                const R qi0s = R(in(a.qi0, k)) + R(out(a.qiten, k)) * dt;
-                const R t0s = R(in(a.t0, k)) + R(out(a.sten, k)) * dt / cp;
-                const R qt0 = (qv0s + ql0s + qi0s);
-                W(B_QT0, k) = qt0;
-                W(B_THL0, k) = (t0s - xlv * ql0s / cp - xls * qi0s / cp) / W(A_EXN0, k);
+                const double qv0s = in(a.qv0, k) + out(a.qvten, k) * dtd;
+                const double ql0s = in(a.ql0, k) + out(a.qlten, k) * dtd;
+                const double qi0s = in(a.qi0, k) + out(a.qiten, k) * dtd;
+                const double t0s = in(a.t0, k) + out(a.sten, k) * dtd / C.cp;
+                const double qt0 = (qv0s + ql0s + qi0s);
+                E(B_QT0, k) = qt0;
+                E(B_THL0, k) = (t0s - C.xlv * ql0s / C.cp - C.xls * qi0s / C.cp) / E(A_EXN0, k);
             }
-            slope([&](int k) { return W(B_THL0, k); }, B_SSTHL0);
-            slope([&](int k) { return W(B_QT0, k); }, B_SSQT0);
+            slope([&](int k) { return E(B_THL0, k); }, B_SSTHL0);
+            slope([&](int k) { return E(B_QT0, k); }, B_SSQT0);
             if (!interface_env(B_THL0)) { exit_code = EX_CONDEN; goto L333; }
         }
         }

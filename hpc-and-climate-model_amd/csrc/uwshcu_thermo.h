@@ -13,6 +13,9 @@
 // Every primitive is callable from host code too: the test suite builds the column
 // code for the CPU to check its logic against the oracle before it runs on a GPU.
 #define UWSHCU_HD __host__ __device__
+#ifndef UWSHCU_OUTLINE
+#define UWSHCU_OUTLINE __attribute__((noinline))
+#endif
 
 namespace uwshcu {
 
@@ -73,7 +76,7 @@ struct Thermo {
     }
 
     // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
-    UWSHCU_HD inline int conden(double p, double thl, double qt, double& th, double& qv, double& ql,
+    UWSHCU_HD UWSHCU_OUTLINE int conden(double p, double thl, double qt, double& th, double& qv, double& ql,
                                           double& qi) const {
         const double ex = exnf(p);
         const double tc = thl * ex;

@@ -35,7 +35,7 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     a.err = err;
     const size_t esz = precision == UWSHCU_FP64 ? sizeof(double) : sizeof(float);
     a.wld = ncol;
-    std::vector<char> ws((size_t)ws_arrays(ncnst) * (pver + 1) * (size_t)ncol * esz + 64);
+    std::vector<char> ws(ws_bytes_per_column(pver, ncnst, esz) * (size_t)ncol + 64);
     std::memset(ws.data(), 0xff, ws.size());   // NaN pattern: unwritten workspace reads show up
     a.ws = ws.data();
     const Thermo T{c, estbl};

@@ -98,3 +98,16 @@ def test_libm_sensitivity_at_72_levels_and_10_constituents():
         w = np.maximum(w, parity.column_rel_err(b[k], a[k]))
     assert (w > 1e-10).mean() < 0.125
     assert np.median(w) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("ncol,pver,ncnst,inv", [(192, 30, 3, 0.25), (128, 72, 10, 0.2), (128, 35, 0, 0.1)])
+def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision):
+    """Build with every array and workspace index asserted (an assertion aborts the
+    process), and check the results equal the unchecked build's."""
+    ch = make_chunk(ncol, pver, ncnst, seed=pver + ncnst, inversion_frac=inv, ptop=100.0 if pver > 60 else 200.0)
+    a, ad, _, _ = hostemu.run(ch, precision=precision, checked=True)
+    b, bd, _, _ = hostemu.run(ch, precision=precision)
+    np.testing.assert_array_equal(ad, bd)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
