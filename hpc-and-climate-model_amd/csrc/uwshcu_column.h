@@ -37,18 +37,23 @@ namespace uwshcu {
 // re-derived from the iteration-1 provisional state (uwshcu.F90:4285-4327), used
 // only for the second CIN evaluation; E_QVSTAR is the positive-moisture fix-up's
 // provisional vapour.  They feed the CIN and cloud-base stages, which run in FP64.
+// E_THV0MID/E_QS0MID hold thv and qs of the environment at layer midpoints
+// (conden(p0, thl0, qt0)), which the buoyancy-sorting loop evaluates for every
+// layer above the release level in every iteration; 0 marks "not yet evaluated",
+// -1 a failed saturation adjustment.
 enum EnvArray : int {
     A_THL0, A_QT0, A_SSTHL0, A_SSQT0, A_SSU0, A_SSV0, A_THV0BOT, A_THV0TOP, A_THVL0BOT, A_THVL0TOP,
     A_EXN0, A_EXNS0,
     B_THL0, B_QT0, B_SSTHL0, B_SSQT0, B_THV0BOT, B_THV0TOP, B_THVL0BOT, B_THVL0TOP,
     E_QVSTAR,
+    E_THV0MID, E_QS0MID,   // saturation state of the layer-mean environment, filled when first used
     ENV_ARRAYS
 };
 // Plume and flux working arrays, in the bulk type R.
 enum WsArray : int {
     W_EMF, W_THLU, W_QTU, W_UU, W_VU, W_THVU, W_UFRC, W_REI, W_FER, W_FDR, W_DWTEN, W_DITEN,
     W_THLUE, W_QTUE, W_UUE, W_VUE, W_UFLX, W_VFLX, W_QLSINK, W_QISINK, W_NLSINK, W_NISINK,
-    W_QCL, W_QCI, W_QW0, W_TRFD, W_TRFU,
+    W_QCL, W_QCI, W_QW0, W_TRFD, W_TRFU, W_QLR, W_QIR,
     WS_FIXED
 };
 // Per-constituent blocks follow the fixed plume arrays: SSTR0, TRU, TRUE, TRFLX (ncnst arrays each).
@@ -180,6 +185,26 @@ struct Column {
         return true;
     }
 
+    // Saturation state of the layer-k mean environment, evaluated once per column
+    // (uwshcu.F90:2390-2397 evaluate it in every buoyancy-sorting iteration with the
+    // same arguments).  Returns false when the saturation adjustment fails.
+    UWSHCU_HD bool env_mid(int k, double& thv0j, double& qs0) const {
+        double& tv = E(E_THV0MID, k);
+        if (tv == 0.0) {
+            double th, qv, ql, qi, es, qs;
+            if (T.conden(in(a.p0, k), E(A_THL0, k), E(A_QT0, k), th, qv, ql, qi)) {
+                tv = -1.0;
+            } else {
+                tv = th * (1.0 + C.zvir * qv - ql - qi);
+                T.qsat_es_qs(E(A_THL0, k) * E(A_EXN0, k), in(a.p0, k), es, qs);
+                E(E_QS0MID, k) = qs;
+            }
+        }
+        thv0j = tv;
+        qs0 = E(E_QS0MID, k);
+        return tv > 0.0;
+    }
+
     // Zero the column's outputs and the working arrays the reference zeroes before a
     // pass (uwshcu.F90:1333-1408, 1817-1869).
     UWSHCU_HD void zero_pass_state() const {
@@ -274,6 +299,7 @@ UWSHCU_HD void Column<R>::run() const {
     slope([&](int k) { return E(A_QT0, k); }, A_SSQT0);
     slope([&](int k) { return in(a.u0, k); }, A_SSU0);
     slope([&](int k) { return in(a.v0, k); }, A_SSV0);
+    for (int k = 0; k <= mkx; ++k) E(E_THV0MID, k) = 0.0;
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
 
     zero_pass_state();
@@ -539,11 +565,18 @@ UWSHCU_HD void Column<R>::run() const {
                 R umfk = 0, ferk = 0, fdrk = 0;
                 for (int iter_xc = 1; iter_xc <= 2; ++iter_xc) {
                     wtw = R(out(a.wu, km1)) * R(out(a.wu, km1));
-                    if (T.conden(pe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-                    const R thv0j = thv_of(th, qvj, qlj, qij);
-                    rho0j = pe / (r * thv0j * exne);
+                    R thv0j;
                     double es, qs;
-                    T.qsat_es_qs((double)(thle * exne), pe, es, qs);
+                    if (k == krel) {
+                        if (T.conden(pe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        thv0j = thv_of(th, qvj, qlj, qij);
+                        T.qsat_es_qs((double)(thle * exne), pe, es, qs);
+                    } else {
+                        double tv;
+                        if (!env_mid(k, tv, qs)) { exit_code = EX_CONDEN; goto L333; }
+                        thv0j = R(tv);
+                    }
+                    rho0j = pe / (r * thv0j * exne);
                     const R excess0 = qte - R(qs);
                     if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     if ((R(qlj) + R(qij)) > criqc) {
@@ -552,8 +585,9 @@ UWSHCU_HD void Column<R>::run() const {
                         const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
                         qtue = qtue - exql - exqi;
                         thlue = thlue + (xlv / cp / exne) * exql + (xls / cp / exne) * exqi;
+                        // unchanged arguments give unchanged results, so only re-adjust here
+                        if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     }
-                    if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     const R thvj = thv_of(th, qvj, qlj, qij);
                     T.qsat_es_qs((double)(thlue * exne), pe, es, qs);
                     const R excessu = qtue - R(qs);
@@ -649,6 +683,7 @@ UWSHCU_HD void Column<R>::run() const {
                         thluk = thluk + (xlv / cp / Er(A_EXNS0, k)) * exql + (xls / cp / Er(A_EXNS0, k)) * exqi;
                         W(W_DWTEN, k) = exql;
                         W(W_DITEN, k) = exqi;
+                        if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     } else {
                         W(W_DWTEN, k) = R(0);
                         W(W_DITEN, k) = R(0);
@@ -658,7 +693,6 @@ UWSHCU_HD void Column<R>::run() const {
                     W(W_UU, k) = uuk;
                     W(W_VU, k) = vuk;
                     out(a.umf, k) = (double)umfk;
-                    if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     const R thvuk = thv_of(th, qvj, qlj, qij);
                     W(W_THVU, k) = thvuk;
                     bogbot = rbuoy * (W(W_THVU, km1) / thvebot - R(1));
@@ -963,6 +997,7 @@ UWSHCU_HD void Column<R>::run() const {
             (void)gdp;
         }
         R rliq = R(0), rainflx = R(0), snowflx = R(0);
+        R qlr_base = R(0), qir_base = R(0);   // updraft condensate at the release level
         {
             auto uf = [&](int k) -> R { return R(in(a.u0, k)) + R(out(a.uten, k)) * dt; };
             auto vf = [&](int k) -> R { return R(in(a.v0, k)) + R(out(a.vten, k)) * dt; };
@@ -1006,6 +1041,8 @@ UWSHCU_HD void Column<R>::run() const {
                     if (T.conden(prel, W(W_THLU, krel - 1), W(W_QTU, krel - 1), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     qlubelow = R(qlj);
                     qiubelow = R(qij);
+                    qlr_base = qlubelow;
+                    qir_base = qiubelow;
                     if (T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     qlr = R(qlj); qir = R(qij);
                     const R fr = (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
@@ -1025,6 +1062,7 @@ UWSHCU_HD void Column<R>::run() const {
                 }
                 qlubelow = qlr;
                 qiubelow = qir;
+                if (k >= krel) { W(W_QLR, k) = qlr; W(W_QIR, k) = qir; }
 
                 R qc_l = (R(1) - frc_rasn) * dwk;
                 R qc_i = (R(1) - frc_rasn) * dik;
@@ -1225,14 +1263,10 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S21 in-cloud diagnostics (uwshcu.F90:4092-4158) ----
         {
-            if (T.conden(prel, W(W_THLU, krel - 1), W(W_QTU, krel - 1), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-            R qcubelow = R(qlj) + R(qij), qlubelow = R(qlj), qiubelow = R(qij);
+            // the tendency stage adjusted the same updraft states: reuse its condensate
+            R qcubelow = qlr_base + qir_base, qlubelow = qlr_base, qiubelow = qir_base;
             for (int k = krel; k <= kpen; ++k) {
-                int idc;
-                if (k == kpen) idc = T.conden(R(in(a.ps0, k - 1)) + ppen, thlu_top, qtu_top, th, qvj, qlj, qij);
-                else idc = T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij);
-                if (idc) { exit_code = EX_CONDEN; goto L333; }
-                const R qlr = R(qlj), qir = R(qij);
+                const R qlr = W(W_QLR, k), qir = W(W_QIR, k);
                 R qcuk = R(0.5) * (qcubelow + qlr + qir);
                 R qluk = R(0.5) * (qlubelow + qlr);
                 R qiuk = R(0.5) * (qiubelow + qir);

@@ -56,12 +56,12 @@ def test_table_builder_equals_the_oracle_table():
 
 
 def test_workspace_size():
-    # an FP64 environment block (21 arrays) and a plume block of 27 + 4*ncnst arrays,
+    # an FP64 environment block (23 arrays) and a plume block of 29 + 4*ncnst arrays,
     # each of pver+1 levels; the plume block is FP32 in the mixed precision
     fp64 = _capi.workspace_bytes_per_column(35, 3, _capi.FP64)
     mixed = _capi.workspace_bytes_per_column(35, 3, _capi.MIXED)
-    assert fp64 == 36 * 8 * (21 + 27 + 12)
-    assert mixed == 36 * (8 * 21 + 4 * (27 + 12))
+    assert fp64 == 36 * 8 * (23 + 29 + 12)
+    assert mixed == 36 * (8 * 23 + 4 * (29 + 12))
 
 
 def test_device_entry_points_reject_bad_arguments_without_a_gpu():
