@@ -31,6 +31,13 @@ struct uwshcu_ctx {
     size_t stage_bytes = 0;
     int* d_diag = nullptr;
     size_t diag_bytes = 0;
+    // mixed precision: input cush copy, branch flags, recompute list and its length
+    double* d_cush_in = nullptr;
+    int* d_amb = nullptr;
+    int* d_list = nullptr;
+    int* d_count = nullptr;
+    size_t mixed_cols = 0;
+    double mixed_tol = 1.e-4;
     hipStream_t stream = nullptr;
     std::mutex mu;
 };
@@ -85,6 +92,36 @@ int ensure_ws(uwshcu_ctx* ctx, size_t bytes) {
     return UWSHCU_OK;
 }
 
+int ensure_mixed(uwshcu_ctx* ctx, size_t ncol) {
+    if (ncol <= ctx->mixed_cols && ctx->d_count) return UWSHCU_OK;
+    if (ctx->d_cush_in || ctx->d_amb || ctx->d_list) {
+        if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+        (void)hipFree(ctx->d_cush_in);
+        (void)hipFree(ctx->d_amb);
+        (void)hipFree(ctx->d_list);
+        ctx->d_cush_in = nullptr;
+        ctx->d_amb = ctx->d_list = nullptr;
+        ctx->mixed_cols = 0;
+    }
+    if (!ctx->d_count && !hip_ok(hipMalloc((void**)&ctx->d_count, sizeof(int)))) return UWSHCU_ERR_NOMEM;
+    if (!hip_ok(hipMalloc((void**)&ctx->d_cush_in, ncol * sizeof(double))) ||
+        !hip_ok(hipMalloc((void**)&ctx->d_amb, ncol * sizeof(int))) ||
+        !hip_ok(hipMalloc((void**)&ctx->d_list, ncol * sizeof(int))))
+        return UWSHCU_ERR_NOMEM;
+    ctx->mixed_cols = ncol;
+    return UWSHCU_OK;
+}
+
+// Columns per launch: as many as the workspace limit allows, a multiple of 64, at
+// least 64 and at most ncol.
+long long batch_columns(const uwshcu_ctx* ctx, int ncol, size_t per_col) {
+    long long nb = (long long)(ctx->ws_limit / per_col);
+    if (nb > ncol) nb = ncol;
+    nb = (nb / 64) * 64;
+    if (nb < 64) nb = 64;
+    return nb;
+}
+
 struct Chunk {
     const double *ps0, *zs0, *p0, *z0, *dp0, *u0, *v0, *qv0, *ql0, *qi0, *t0, *s0, *tr0, *tke, *pblh, *dpdry0;
     double* cush;
@@ -96,25 +133,28 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
                double dt, const Chunk& ch, int* diag, hipStream_t stream) {
     if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
         return UWSHCU_ERR_ARG;
-    if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED) return UWSHCU_ERR_ARG;
+    if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED && precision != UWSHCU_MIXED_UNCHECKED)
+        return UWSHCU_ERR_ARG;
     if (ncol == 0) return UWSHCU_OK;
     if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
-    const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, precision);
-    // batch of columns that shares one workspace
-    long long nb = (long long)(ctx->ws_limit / per_col);
-    if (nb > ncol) nb = ncol;
-    nb = (nb / 64) * 64;
-    if (nb < 64) nb = 64;
-    const long long wld = nb;
-    int rc = ensure_ws(ctx, per_col * (size_t)wld);
+    const bool fp32 = precision != UWSHCU_FP64;
+    const bool recompute = precision == UWSHCU_MIXED;
+    const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, fp32 ? UWSHCU_MIXED : UWSHCU_FP64);
+    const size_t per_col64 = uwshcu_workspace_bytes_per_column(pver, ncnst, UWSHCU_FP64);
+    // batches of columns that share one workspace
+    const long long nb = batch_columns(ctx, ncol, per_col);
+    const long long nb64 = recompute ? batch_columns(ctx, ncol, per_col64) : 0;
+    size_t need = per_col * (size_t)nb;
+    if (recompute && per_col64 * (size_t)nb64 > need) need = per_col64 * (size_t)nb64;
+    int rc = ensure_ws(ctx, need);
     if (rc) return rc;
+    if (recompute && (rc = ensure_mixed(ctx, (size_t)ncol))) return rc;
 
     FwdArgs a{};
     a.ncol = ncol;
     a.mkx = pver;
     a.ncnst = ncnst;
     a.ldc = pcols;
-    a.wld = wld;
     a.dt = dt;
     uwshcu_default_params(&a.par);
     if (par) a.par = *par;
@@ -129,13 +169,36 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     a.diag = diag;
     a.err = ctx->d_err;
     a.ws = ctx->d_ws;
+    if (recompute) {
+        // the FP32 pass overwrites cush; the FP64 recompute reads the input value
+        if (!hip_ok(hipMemcpyAsync(ctx->d_cush_in, ch.cush, (size_t)ncol * sizeof(double), hipMemcpyDeviceToDevice,
+                                   stream)))
+            return UWSHCU_ERR_HIP;
+        a.amb_flag = ctx->d_amb;
+    }
+    a.wld = nb;
     for (long long c0 = 0; c0 < ncol; c0 += nb) {
         const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
         a.col0 = (int)c0;
-        hipError_t e = (precision == UWSHCU_FP64)
-                           ? uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream)
-                           : uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream);
+        hipError_t e = fp32 ? uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream)
+                            : uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream);
         if (!hip_ok(e)) return UWSHCU_ERR_HIP;
+    }
+    if (!recompute) return UWSHCU_OK;
+    // conservation/branch check, then FP64 over the listed columns; the list length
+    // stays on the device (launches past it exit at once)
+    a.col0 = 0;
+    if (!hip_ok(uwshcu::launch_check_columns(a, ctx->d_consts, ctx->mixed_tol, ctx->d_list, ctx->d_count, stream)))
+        return UWSHCU_ERR_HIP;
+    a.amb_flag = nullptr;
+    a.cush_in = ctx->d_cush_in;
+    a.colmap = ctx->d_list;
+    a.colcount = ctx->d_count;
+    a.wld = nb64;
+    for (long long c0 = 0; c0 < ncol; c0 += nb64) {
+        const int n = (int)((ncol - c0) < nb64 ? (ncol - c0) : nb64);
+        a.col0 = (int)c0;
+        if (!hip_ok(uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream))) return UWSHCU_ERR_HIP;
     }
     return UWSHCU_OK;
 }
@@ -261,8 +324,31 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_ws) (void)hipFree(ctx->d_ws);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->d_cush_in) (void)hipFree(ctx->d_cush_in);
+    if (ctx->d_amb) (void)hipFree(ctx->d_amb);
+    if (ctx->d_list) (void)hipFree(ctx->d_list);
+    if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return UWSHCU_OK;
+}
+
+int uwshcu_ctx_set_mixed_tolerance(uwshcu_ctx* ctx, double tol) {
+    if (!ctx || !(tol > 0.0)) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->mixed_tol = tol;
+    return UWSHCU_OK;
+}
+
+int uwshcu_mixed_recomputed(uwshcu_ctx* ctx, void* stream, int* count) {
+    if (!ctx || !count) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *count = 0;
+    if (!ctx->d_count) return UWSHCU_OK;
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    if (!hip_ok(hipMemcpyAsync(count, ctx->d_count, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream)) ||
+        !hip_ok(hipStreamSynchronize((hipStream_t)stream)))
+        return UWSHCU_ERR_HIP;
     return UWSHCU_OK;
 }
 

@@ -1,0 +1,54 @@
+// uwshcu_check.h -- conservation check of one column's outputs, in FP64.
+//
+// The scheme conserves column water and, with the kinetic energy of the momentum
+// tendencies, column energy exactly up to round-off:
+//   water   sum_k (qvten+qlten+qiten) dp/g + (precip + rliq)*1000 = 0
+//   energy  sum_k (sten + dKE/dt + (xlv+xlf) qvten + xlf qlten) dp/g
+//             = -(precip + rliq - snow)*1000*xlf
+// (the budget CAM checks after this scheme: check_energy.F90:336-367 with
+// flx_cnd = precip + rliq and flx_ice = snow, tphysbc.F90:444-449).  In FP64 both
+// close to ~1e-14 of the largest flux; with FP32 plume arithmetic they close to
+// ~1e-6.  A column whose residual exceeds tol times its largest flux, or whose
+// outputs are not finite, fails the check.
+#ifndef UWSHCU_CHECK_H
+#define UWSHCU_CHECK_H
+
+#include "uwshcu_column.h"
+
+namespace uwshcu {
+
+UWSHCU_HD inline bool finite_value(double x) { return x - x == 0.0; }   // false for NaN and +-Inf
+
+UWSHCU_HD inline bool column_fails_check(const FwdArgs& a, const uwshcu_consts& C, int i, double tol) {
+    const int mkx = a.mkx;
+    const size_t ldc = (size_t)a.ldc;
+    auto at = [&](const double* p, int row) { return p[(size_t)row * ldc + i]; };
+    const double dt = a.dt;
+    double w = 0.0, te = 0.0, acc = 0.0, fq = 0.0, fs = 0.0;
+    for (int r = 0; r < mkx; ++r) {
+        const double dpg = at(a.dp0, r) / C.g;
+        const double qv = at(a.qvten, r), ql = at(a.qlten, r), qi = at(a.qiten, r);
+        const double u0 = at(a.u0, r), v0 = at(a.v0, r);
+        const double uf = u0 + at(a.uten, r) * dt, vf = v0 + at(a.vten, r) * dt;
+        const double dke = 0.5 * ((uf * uf - u0 * u0) + (vf * vf - v0 * v0)) / dt;
+        w += (qv + ql + qi) * dpg;
+        te += (at(a.sten, r) + dke + (C.xlv + C.xlf) * qv + C.xlf * ql) * dpg;
+        acc += fabs(at(a.qrten, r)) + fabs(at(a.qsten, r)) + fabs(at(a.evapc, r)) + fabs(at(a.cufrc, r)) +
+               fabs(at(a.qcu, r)) + fabs(at(a.qlu, r)) + fabs(at(a.qiu, r)) + fabs(at(a.qc, r));
+        for (int m = 0; m < a.ncnst; ++m) acc += fabs(a.trten[((size_t)m * mkx + r) * ldc + i]);
+    }
+    for (int r = 0; r <= mkx; ++r) {
+        fq = fmax(fq, fabs(at(a.qtflx, r)));
+        fs = fmax(fs, fabs(at(a.slflx, r)));
+        acc += fabs(at(a.umf, r)) + fabs(at(a.wu, r)) + fabs(at(a.flxprc1, r)) + fabs(at(a.flxsnow1, r));
+    }
+    const double precip = a.precip[i], rliq = a.rliq[i], snow = a.snow[i];
+    w += (precip + rliq) * 1000.0;
+    const double eres = te + (precip + rliq - snow) * 1000.0 * C.xlf;
+    acc += fabs(a.cbmf[i]) + fabs(a.cush[i]) + fs + fq;
+    if (!(finite_value(w) && finite_value(eres) && finite_value(acc))) return true;
+    return fabs(w) > tol * fq || fabs(eres) > tol * fs;
+}
+
+}  // namespace uwshcu
+#endif

@@ -75,7 +75,13 @@ struct FwdArgs {
     double *qrten, *qsten, *precip, *snow, *evapc, *cufrc, *qcu, *qlu, *qiu, *cbmf, *qc, *rliq, *cnt, *cnb, *wu;
     int* diag;          // optional [UWSHCU_NDIAG][ldc]
     int* err;           // [2]: first fatal code, column
-    void* ws;           // workspace, R elements
+    void* ws;           // workspace (see ws_bytes_per_column)
+    // Column selection.  With colmap == nullptr, workspace slot j serves column col0 + j.
+    // Otherwise slot j serves column colmap[col0 + j] for col0 + j < *colcount.
+    const int* colmap;
+    const int* colcount;
+    const double* cush_in;   // optional: read the input cush here instead of from cush
+    int* amb_flag;           // optional [ldc]: 1 where an FP32 branch decision was within rounding
 };
 
 // Exit reasons reported in diag row 0 (same numbering as the oracle).
@@ -92,8 +98,8 @@ struct Column {
     double* __restrict__ env;   // ENV_ARRAYS * KS rows of wld doubles
     R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
 
-    UWSHCU_HD Column(const FwdArgs& args, const Thermo& th, int col)
-        : a(args), T(th), C(*th.c), i(col), iw(col - args.col0), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
+    UWSHCU_HD Column(const FwdArgs& args, const Thermo& th, int col, int slot)
+        : a(args), T(th), C(*th.c), i(col), iw(slot), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
           ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<double*>(args.ws)),
           ws(reinterpret_cast<R*>(reinterpret_cast<double*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
                                                                          (size_t)args.wld)) {}
@@ -272,6 +278,16 @@ UWSHCU_HD void Column<R>::run() const {
     const bool has_nl = ixl >= 1 && ixl <= ncnst, has_ni = ixi >= 1 && ixi <= ncnst;
     auto trz = [&](int m, int k) -> R { return (m >= 1 && m <= ncnst) ? R(trin(m, k)) : R(0); };
 
+    // FP32 branch guard: a decision the reference takes on a quantity computed here
+    // in FP32 is flagged when the quantity lies within a relative 1e-5 of the branch
+    // point, well above the FP32 rounding it carries.  Flagged columns are recomputed
+    // in FP64 by the mixed-precision driver.  No effect in FP64.
+    constexpr bool kFp32 = sizeof(R) == sizeof(float);
+    int amb = 0;
+    auto near = [&](double dist, double scale) {
+        if (kFp32 && fabs(dist) <= 1.e-5 * fabs(scale)) amb = 1;
+    };
+
     int exit_code = EX_NONE, path = 0, exit_iter = 0;
     int kinv = 0, klcl = 0, klfc = 0, krel = 0, kbup = 0, kpen = 0;
     int limit_ufrc = 0, limit_ppen = 0, limit_emf = 0;
@@ -305,7 +321,7 @@ UWSHCU_HD void Column<R>::run() const {
     zero_pass_state();
     for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
 
-    double cush = a.cush[i];
+    double cush = a.cush_in ? a.cush_in[i] : a.cush[i];
     const double pblh = a.pblh[i];
 
     // iteration-1 values restored for the implicit-CIN pass (uwshcu.F90:1693-1713)
@@ -436,6 +452,7 @@ UWSHCU_HD void Column<R>::run() const {
             vsrc_o = vsrc; thv0lcl_o = thv0lcl;
         } else {
             const double del_CIN = cin - cin_i;
+            if (cin_i + cin > 0.0) near(del_CIN, cin_i + cin);
             if (del_CIN > 0.0) {
                 const double alpha = compute_alpha(del_CIN, ke);
                 cin = cin_i + alpha * del_CIN;
@@ -579,6 +596,7 @@ UWSHCU_HD void Column<R>::run() const {
                     rho0j = pe / (r * thv0j * exne);
                     const R excess0 = qte - R(qs);
                     if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
                         const R qlr = R(qlj), qir = R(qij);
                         const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
@@ -591,6 +609,8 @@ UWSHCU_HD void Column<R>::run() const {
                     const R thvj = thv_of(th, qvj, qlj, qij);
                     T.qsat_es_qs((double)(thlue * exne), pe, es, qs);
                     const R excessu = qtue - R(qs);
+                    near(excessu, qtue);
+                    near(excess0, qte);
                     const R cridis = rle * scaleh;
                     R xc;
                     if ((excessu <= R(0) && excess0 <= R(0)) || (excessu >= R(0) && excess0 >= R(0))) {
@@ -616,9 +636,11 @@ UWSHCU_HD void Column<R>::run() const {
                             const R cquad = R(2) * rbuoy * g * cridis * (thv_x0 - thv0j) / thv0j + (wue * wue);
                             R xs1 = 0, xs2 = 0;
                             int status;
+                            near(bquad * bquad - R(4) * aquad * cquad, bquad * bquad + fabs(R(4) * aquad * cquad));
                             if (kk == 1) {
                                 if ((bquad * bquad - R(4) * aquad * cquad) >= R(0)) {
                                     roots(aquad, bquad, cquad, xs1, xs2, status);
+                                    near(xsat - fmin(xs1, xs2), R(1));
                                     x_cu = fmin(R(1), fmax(R(0), fmin(xsat, fmin(xs1, xs2))));
                                 } else {
                                     x_cu = xsat;
@@ -634,6 +656,7 @@ UWSHCU_HD void Column<R>::run() const {
                         }
                         xc = (x_cu == xsat) ? fmax(x_cu, x_en) : x_cu;
                     }
+                    near(xc - R(0.5), R(1));
                     const R ee2 = xc * xc;
                     const R ud2 = R(1) - R(2) * xc + xc * xc;
                     R reik = (R(0.5) * rkm / R(in(a.z0, k)) / g / rho0j);
@@ -675,6 +698,7 @@ UWSHCU_HD void Column<R>::run() const {
                         }
                     }
                     if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
                         const R qlr = R(qlj), qir = R(qij);
                         const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
@@ -705,6 +729,8 @@ UWSHCU_HD void Column<R>::run() const {
                         wtw = wtw * expfac + (delbog + (R(1) - expfac) * (bogbot + delbog / (R(-2) * drage * dpe))) / (rho0j * drage);
                     else
                         wtw = wtw + dpe * (bogbot + bogtop) / rho0j;
+                    near(wtw, wtwb);
+                    near(bogtop, R(1));
                     if (wtw > R(0)) {
                         thlue = R(0.5) * (W(W_THLU, km1) + thluk);
                         qtue = R(0.5) * (W(W_QTU, km1) + qtuk);
@@ -723,6 +749,7 @@ UWSHCU_HD void Column<R>::run() const {
                     if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
                     const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
                     R ufrck = umfk / (rhos0j * wuk);
+                    near(ufrck - rmaxfrac, rmaxfrac);
                     if (ufrck > rmaxfrac) {
                         limit_ufrc = 1;
                         ufrck = rmaxfrac;
@@ -787,6 +814,7 @@ UWSHCU_HD void Column<R>::run() const {
             {
                 const R ptop = R(in(a.ps0, kpen - 1)) + ppen;
                 if (T.conden(ptop, thlu_top, qtu_top, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                near(qlj + qij - criqc, criqc);
                 const R exntop = R(upow((double)ptop / C.p00, C.rovcp));
                 if ((R(qlj) + R(qij)) > criqc) {
                     const R qlr = R(qlj), qir = R(qij);
@@ -842,6 +870,7 @@ UWSHCU_HD void Column<R>::run() const {
                 if (x < lim2) limit_emf = 1;
                 const R emfk = fmax(fmax(x, R(-0.1) * rhos0j), lim2);
                 W(W_EMF, k) = emfk;
+                near(fabs(emfk) - fabs(emf1), fabs(emf1));
                 if (fabs(emfk) > fabs(emf1)) {
                     W(W_THLUE, k) = (W(W_THLUE, k + 1) * emf1 + Er(A_THL0, k + 1) * (emfk - emf1)) / emfk;
                     W(W_QTUE, k) = (W(W_QTUE, k + 1) * emf1 + Er(A_QT0, k + 1) * (emfk - emf1)) / emfk;
@@ -1328,6 +1357,7 @@ L333:
     path = 0;
 
 Ldone:
+    if (a.amb_flag) a.amb_flag[i] = amb;
     if (a.diag) {
         const size_t L = (size_t)a.ldc;
         int* d = a.diag + i;

@@ -16,7 +16,8 @@ ESTBL_LEN = 250
 NDIAG = 12
 
 FP64 = 0
-MIXED = 1
+MIXED = 1             # FP32 plume pass + FP64 conservation/branch check + FP64 recompute
+MIXED_UNCHECKED = 2   # the FP32 pass alone
 
 ERRORS = {
     0: "ok",
@@ -58,7 +59,8 @@ ARRAY_ARGS = INPUT_NAMES + ("cush",) + OUTPUT_NAMES + ("dpdry0", "wu")
 SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params", "uwshcu_ctx_create",
            "uwshcu_ctx_destroy", "uwshcu_ctx_set_workspace_limit", "uwshcu_workspace_bytes_per_column",
            "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
-           "uwshcu_set_dropin_params", "uwshcu_dropin_last_status")
+           "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
+           "uwshcu_mixed_recomputed")
 
 _lib = None
 
@@ -99,6 +101,8 @@ def lib():
     L.uwshcu_status.argtypes = [vp, vp, ip]
     L.uwshcu_set_dropin_params.argtypes = [C.POINTER(Params)]
     L.uwshcu_dropin_last_status.argtypes = [ip]
+    L.uwshcu_ctx_set_mixed_tolerance.argtypes = [vp, dp]
+    L.uwshcu_mixed_recomputed.argtypes = [vp, vp, ip]
     _lib = L
     return L
 

@@ -17,7 +17,7 @@ import ctypes as C
 import torch
 
 from . import _capi
-from ._capi import FP64, MIXED, UwshcuError  # noqa: F401
+from ._capi import FP64, MIXED, MIXED_UNCHECKED, UwshcuError  # noqa: F401
 
 LAYER_OUT = ("qvten", "qlten", "qiten", "sten", "uten", "vten", "qrten", "qsten", "evapc", "cufrc", "qcu", "qlu",
              "qiu", "qc")
@@ -105,6 +105,22 @@ class Context:
         bad = C.c_int(-1)
         rc = _capi.lib().uwshcu_status(self._h, C.c_void_p(sp), C.byref(bad))
         return rc, bad.value
+
+    def mixed_recomputed(self, stream=None) -> int:
+        """Columns the last MIXED call recomputed in FP64 (synchronises the stream)."""
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        n = C.c_int(0)
+        rc = _capi.lib().uwshcu_mixed_recomputed(self._h, C.c_void_p(sp), C.byref(n))
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_mixed_recomputed")
+        return n.value
+
+    def set_mixed_tolerance(self, tol: float) -> None:
+        rc = _capi.lib().uwshcu_ctx_set_mixed_tolerance(self._h, C.c_double(tol))
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_ctx_set_mixed_tolerance")
 
     def check(self, stream=None):
         rc, bad = self.status(stream)

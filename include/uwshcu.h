@@ -61,6 +61,15 @@ int uwshcu_ctx_create(const uwshcu_consts *c, const double *estbl, int device, u
 int uwshcu_ctx_destroy(uwshcu_ctx *ctx);
 int uwshcu_ctx_set_workspace_limit(uwshcu_ctx *ctx, size_t bytes);
 
+/* Relative tolerance of the FP64 water/energy conservation check that selects the
+ * columns UWSHCU_MIXED recomputes in FP64 (default 1e-4).  No counterpart in the
+ * reference: its mixed-precision variant has no error control. */
+int uwshcu_ctx_set_mixed_tolerance(uwshcu_ctx *ctx, double tol);
+
+/* Synchronise `stream` and return the number of columns the last UWSHCU_MIXED call
+ * on ctx recomputed in FP64. */
+int uwshcu_mixed_recomputed(uwshcu_ctx *ctx, void *stream, int *count);
+
 /* Bytes of workspace needed per column. */
 size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision);
 

@@ -35,7 +35,12 @@ typedef struct uwshcu_params {
 // Precision modes.
 enum {
     UWSHCU_FP64  = 0,   // all arithmetic in FP64
-    UWSHCU_MIXED = 1    // FP32 bulk; FP64 saturation adjustment, LCL, CIN and fix-ups
+    // FP32 plume arithmetic; FP64 saturation adjustment, LCL, CIN, cloud-base closure
+    // and moisture fix-up; then every column that fails the FP64 water/energy
+    // conservation check (relative tolerance 1e-4 by default) or took an FP32 branch
+    // decision within rounding of its branch point is recomputed in FP64 on the GPU.
+    UWSHCU_MIXED = 1,
+    UWSHCU_MIXED_UNCHECKED = 2   // the FP32 pass alone (no check, no recompute)
 };
 
 // Return codes (negative values are conditions the reference treats as fatal).

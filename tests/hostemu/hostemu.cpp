@@ -6,7 +6,7 @@
 #include <cstring>
 #include <vector>
 
-#include "uwshcu_column.h"
+#include "uwshcu_check.h"
 
 using namespace uwshcu;
 
@@ -21,7 +21,7 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
                                double* qrten, double* qsten, double* precip, double* snow, double* evapc,
                                double* cufrc, double* qcu, double* qlu, double* qiu, double* cbmf, double* qc,
                                double* rliq, double* cnt, double* cnb, const double* dpdry0, double* wu, int* diag,
-                               int* bad_col) {
+                               int* bad_col, int* nfix) {
     FwdArgs a{};
     a.ncol = ncol; a.mkx = pver; a.ncnst = ncnst; a.ldc = pcols; a.col0 = 0; a.dt = dt; a.par = *par;
     a.ps0 = ps0; a.zs0 = zs0; a.p0 = p0; a.z0 = z0; a.dp0 = dp0; a.dpdry0 = dpdry0; a.u0 = u0; a.v0 = v0;
@@ -33,17 +33,37 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     a.cnt = cnt; a.cnb = cnb; a.wu = wu; a.diag = diag;
     int err[2] = {0, -1};
     a.err = err;
-    const size_t esz = precision == UWSHCU_FP64 ? sizeof(double) : sizeof(float);
+    // precision: UWSHCU_FP64, UWSHCU_MIXED (FP32 pass, conservation/branch check, FP64
+    // recompute of the listed columns -- the product's mixed pipeline, run serially
+    // here), or UWSHCU_MIXED_UNCHECKED (the FP32 pass alone).
+    const bool fp32 = precision != UWSHCU_FP64;
     a.wld = ncol;
-    std::vector<char> ws(ws_bytes_per_column(pver, ncnst, esz) * (size_t)ncol + 64);
+    std::vector<char> ws(ws_bytes_per_column(pver, ncnst, sizeof(double)) * (size_t)ncol + 64);
     std::memset(ws.data(), 0xff, ws.size());   // NaN pattern: unwritten workspace reads show up
     a.ws = ws.data();
+    std::vector<int> amb(ncol, 0);
+    std::vector<double> cush_in(cush, cush + ncol);
+    a.amb_flag = fp32 ? amb.data() : nullptr;
     const Thermo T{c, estbl};
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ncol; ++i) {
-        if (precision == UWSHCU_FP64) Column<double>(a, T, i).run();
-        else Column<float>(a, T, i).run();
+        if (!fp32) Column<double>(a, T, i, i).run();
+        else Column<float>(a, T, i, i).run();
     }
+    int nlist = 0;
+    if (precision == UWSHCU_MIXED) {
+        std::vector<int> list;
+        for (int i = 0; i < ncol; ++i)
+            if (amb[i] || column_fails_check(a, *c, i, 1.e-4)) list.push_back(i);
+        nlist = (int)list.size();
+        a.amb_flag = nullptr;
+        a.cush_in = cush_in.data();
+        a.colmap = list.data();
+        a.colcount = &nlist;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int j = 0; j < nlist; ++j) Column<double>(a, T, list[j], j).run();
+    }
+    if (nfix) *nfix = nlist;
     if (bad_col) *bad_col = err[0] ? err[1] : -1;
     return err[0];
 }

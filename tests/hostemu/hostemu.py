@@ -56,7 +56,7 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False):
+def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False, stats=None):
     """Run the host-compiled column code on a generator chunk.
 
     Returns (outputs dict, diag [ncol, NDIAG], rc, bad_col) like ``oracle.run``.
@@ -86,7 +86,10 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
     args += [_p(out[k]) for k in ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "qiten", "sten",
                                   "uten", "vten", "trten", "qrten", "qsten", "precip", "snow", "evapc", "cufrc",
                                   "qcu", "qlu", "qiu", "cbmf", "qc", "rliq", "cnt", "cnb")]
-    args += [_p(np.ascontiguousarray(f["dpdry0"])), _p(out["wu"]), _p(diag), C.byref(bad)]
+    nfix = C.c_int(0)
+    args += [_p(np.ascontiguousarray(f["dpdry0"])), _p(out["wu"]), _p(diag), C.byref(bad), C.byref(nfix)]
     rc = lib(checked).hostemu_forward(*args)
+    if stats is not None:
+        stats["recomputed"] = nfix.value
     out["cush"] = cush
     return out, np.ascontiguousarray(diag.T), rc, bad.value

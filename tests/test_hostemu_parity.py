@@ -111,3 +111,23 @@ def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision):
     np.testing.assert_array_equal(ad, bd)
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_mixed_pipeline_recomputes_listed_columns_in_fp64():
+    """UWSHCU_MIXED: an FP32 plume pass, then an FP64 recompute of every column that
+    fails the conservation check or took an FP32 branch decision within rounding.
+    Columns it changed must equal the FP64 result bit for bit; the others must equal
+    the FP32 pass."""
+    ch = make_chunk(2048, 35, 3, seed=31, inversion_frac=0.3)
+    ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    st = {}
+    mix, _, _, _ = hostemu.run(ch, precision=1, stats=st)
+    raw, _, _, _ = hostemu.run(ch, precision=2)
+    changed = np.zeros(ch.ncol, bool)
+    for k in ref:
+        changed |= (np.asarray(mix[k]) != np.asarray(raw[k])).reshape(-1, ch.ncol).any(axis=0)
+    assert 0 < changed.sum() <= st["recomputed"] < ch.ncol
+    for k in ref:
+        np.testing.assert_array_equal(mix[k][..., changed], ref[k][..., changed], err_msg=k)
+        np.testing.assert_array_equal(mix[k][..., ~changed], raw[k][..., ~changed], err_msg=k)
+
