@@ -6,6 +6,17 @@ step = one call of the forward path over the whole chunk.  Timing: W untimed
 warm-up steps, then K steps between two synchronisations, with HIP events
 recorded on the stream the kernels run on.
 
+Other modes (same JSON contract; metric and config name the mode):
+  --precision mixed   the mixed-precision forward (FP32 plume arithmetic, FP64
+                      saturation/CIN/closure/fix-ups, FP64 recompute of columns that
+                      fail the conservation or branch check); reports its error against
+                      the FP64 path on the same chunk and the recomputed fraction.
+                      The 1 036 800-column workload: --ncol 1036800.
+  --adjoint           forward + adjoint (gradients of the calibration loss
+                      sum (sten/cp - tau_T)^2 + (qvten - tau_q)^2 with respect to t0,
+                      qv0, u0, v0, tke, pblh, rpen, rle, rkm); one step = FP64 forward
+                      plus the adjoint.  The 65 536-column workload: --ncol 65536.
+
 The JSON line also carries:
   roofline      HBM view of the forward kernel: algorithmic bytes per launch (the
                 FP64 inputs the physics reads plus the full output set, per column
@@ -46,7 +57,26 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=8192, help="columns timed on the CPU oracle")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint", action="store_true", help="time forward + adjoint")
     return ap.parse_args()
+
+
+def timed(step, steps, warmup, stream):
+    """Run warmup + steps calls of step(); per-step milliseconds from HIP events on stream."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for s in range(steps):
+        step()
+        ev[s + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return np.array([ev[s].elapsed_time(ev[s + 1]) for s in range(steps)]), wall
 
 
 def main():
@@ -70,32 +100,61 @@ def main():
     cush = torch.empty_like(inp["cush"])
     stream = torch.cuda.current_stream()
 
-    def step():
+    def fwd(precision):
         cush.copy_(inp["cush"])
-        ctx.forward_into(inp, out, cush, dt=ch.dt, precision=prec, stream=stream)
+        ctx.forward_into(inp, out, cush, dt=ch.dt, precision=precision, stream=stream)
 
-    for _ in range(args.warmup):
-        step()
-    ctx.check(stream)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for s in range(args.steps):
-        step()
-        ev[s + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    extra = {}
+    if args.adjoint:
+        # the calibration loss' cotangents; the adjoint rewrites out/cush like the forward
+        rng = np.random.default_rng(args.seed + 1)
+        dev = torch.device("cuda:0")
+        tau_t = torch.from_numpy(rng.normal(0, 1e-5, (args.pver, args.ncol))).to(dev)
+        tau_q = torch.from_numpy(rng.normal(0, 1e-8, (args.pver, args.ncol))).to(dev)
+        grads = {k: torch.empty((args.pver, args.ncol), dtype=torch.float64, device=dev)
+                 for k in ("t0", "qv0", "u0", "v0")}
+        grads["tke"] = torch.empty((args.pver + 1, args.ncol), dtype=torch.float64, device=dev)
+        grads["pblh"] = torch.empty(args.ncol, dtype=torch.float64, device=dev)
+        grads["par_sum"] = torch.empty(3, dtype=torch.float64, device=dev)
+        aout = ctx.alloc_outputs(args.ncol, args.pver, args.ncnst)
+        acush = torch.empty_like(cush)
+        cpair = 1.00464e3
+
+        def step():
+            fwd(_capi.FP64)
+            cot = {"sten": 2.0 * (out["sten"] / cpair - tau_t) / cpair, "qvten": 2.0 * (out["qvten"] - tau_q)}
+            acush.copy_(inp["cush"])
+            ctx.adjoint_into(inp, aout, acush, cot, grads, dt=ch.dt, stream=stream)
+        per_step_ms, wall = timed(lambda: fwd(_capi.FP64), args.steps, args.warmup, stream)
+        fwd_ms = float(np.mean(per_step_ms))
+        per_step_ms, wall = timed(step, args.steps, args.warmup, stream)
+        ctx.check(stream)
+        extra["adjoint"] = {"fp64_forward_ms": fwd_ms, "forward_plus_adjoint_ms": float(np.mean(per_step_ms)),
+                            "ratio_to_fp64_forward": fwd_ms / float(np.mean(per_step_ms)),
+                            "param_gradient": grads["par_sum"].cpu().tolist()}
+    else:
+        per_step_ms, wall = timed(lambda: fwd(prec), args.steps, args.warmup, stream)
     rc, bad = ctx.status(stream)
     if rc:
         raise _capi.UwshcuError(rc, bad)
-    per_step_ms = [ev[s].elapsed_time(ev[s + 1]) for s in range(args.steps)]
     ms = float(np.mean(per_step_ms))
     cols_per_s = args.ncol / (ms * 1e-3)
     diag = out["diag"].cpu().numpy()
     conv_frac = float((diag[7] > 0).mean())
+    if prec == _capi.MIXED and not args.adjoint:
+        nfix = ctx.mixed_recomputed(stream)
+        mix = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
+        mix["cush"] = cush.cpu().numpy()
+        fwd(_capi.FP64)
+        f64 = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
+        f64["cush"] = cush.cpu().numpy()
+        fwd(_capi.MIXED)          # leave the mixed result in place
+        extra["mixed"] = mixed_error(mix, f64, nfix, args.ncol)
     bpc = algorithmic_bytes_per_column(args.pver, args.ncnst)
     achieved = bpc * args.ncol / (ms * 1e-3) / 1e9
+    mode = "fp64 forward" if prec == _capi.FP64 else "mixed forward"
+    if args.adjoint:
+        mode = "fp64 forward + adjoint"
     res = {
         "metric": METRIC,
         "value": cols_per_s,
@@ -107,10 +166,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == _capi.FP64 else "f32+f64",
+        "dtype": "f64" if (prec == _capi.FP64 or args.adjoint) else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.precision} forward of compute_uwshcu_inv, {args.ncol} columns x {args.pver} levels x "
+            "workload": f"{mode} of compute_uwshcu_inv, {args.ncol} columns x {args.pver} levels x "
                         f"{args.ncnst} constituents, dt={ch.dt:g} s",
             "ncol": args.ncol, "pver": args.pver, "ncnst": args.ncnst,
             "convecting_fraction": conv_frac, "inversion_frac": args.inversion_frac, "seed": args.seed,
@@ -124,9 +183,29 @@ def main():
         "step_ms_median": float(np.median(per_step_ms)),
         "wall_s_timed": wall,
     }
+    res.update(extra)
     if not args.no_cpu:
         res.update(cpu_leg(args, ch, out, cush))
     print(json.dumps(res), flush=True)
+
+
+def mixed_error(mix, f64, nfix, ncol):
+    """Error of the mixed-precision outputs against the FP64 path on the same chunk."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import parity
+    worst = np.zeros(ncol)
+    for k in f64:
+        worst = np.maximum(worst, parity.column_rel_err(mix[k], f64[k]))
+    rel_l2 = {}
+    for k in ("qvten", "qlten", "qiten", "sten", "uten", "vten", "umf", "precip"):
+        den = float((f64[k] ** 2).sum())
+        rel_l2[k] = float(np.sqrt(((mix[k] - f64[k]) ** 2).sum() / den)) if den > 0 else 0.0
+    return {"recomputed_fp64_columns": int(nfix), "recomputed_fraction": nfix / ncol,
+            "tendency_rel_l2_vs_fp64": rel_l2,
+            "column_rel_err_vs_fp64": {"median": float(np.median(worst)), "p90": float(np.quantile(worst, 0.9)),
+                                       "p99": float(np.quantile(worst, 0.99)), "max": float(worst.max())},
+            "reference_harness_rms_vs_fp64": parity.reference_rms_errors(mix, f64)}
 
 
 def cpu_leg(args, ch, out, cush):

@@ -1,0 +1,92 @@
+// uwshcu_adjoint.h -- tape set-up and reverse sweep for one column (host and device).
+//
+// The adjoint of compute_uwshcu_inv is taken by recording the column's forward pass
+// on a tape (Column<AD, AD>, see uwshcu_ad.h) and sweeping it backwards.  The
+// independent variables are t0, qv0, u0, v0 (layers), tke (interfaces) and the
+// tunables rpen, rle, rkm; the first ad_inputs(mkx) tape nodes are their leaves.
+// pblh enters the scheme only through the integer inversion level kinv
+// (uwshcu.F90:1455-1468), so its derivative is zero wherever it exists.
+#ifndef UWSHCU_ADJOINT_H
+#define UWSHCU_ADJOINT_H
+
+#include "uwshcu_column.h"
+
+namespace uwshcu {
+
+UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin) {
+    t.nodes = nodes;
+    t.cap = cap;
+    t.overflow = nin > cap ? 1 : 0;
+    t.n = nin < cap ? nin : cap;
+    for (int q = 0; q < t.n; ++q) {
+        TapeNode nd;
+        nd.a = -1;
+        nd.b = -1;
+        nd.da = 0.0;
+        nd.db = 0.0;
+        nodes[q] = nd;
+    }
+}
+
+// Cotangents of the outputs, in the caller's layout (nullable per field).
+struct AdCotangents {
+    const double* f[ADF_TR];         // umf .. qc, as AdField
+    const double* trten;
+    const double* scal[ADC_COUNT];   // precip, snow, cbmf, rliq, cnt, cnb, cush
+};
+
+// Gradients of the inputs, in the caller's layout (nullable per field); par is
+// [3][ldc]: the column's contribution to d/d(rpen, rle, rkm).
+struct AdGradients {
+    double *t0, *qv0, *u0, *v0, *tke, *pblh, *par;
+};
+
+// Reverse sweep of one column: seed the outputs' adjoints from the cotangents,
+// propagate back through the tape, write the input gradients.
+UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes, size_t wld, int slot, int mkx,
+                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
+    const int KS = mkx + 1;
+    for (int q = 0; q < t.n; ++q) adj[q] = 0.0;
+    auto node = [&](int f, int k) { return nodes[((size_t)f * KS + k) * wld + slot]; };
+    auto row = [&](int k) { return (size_t)(mkx - k) * ldc + col; };
+    for (int f = 0; f < ADF_TR; ++f) {
+        if (!cot.f[f]) continue;
+        for (int k = (f <= ADF_WU ? 0 : 1); k <= mkx; ++k) {
+            const int nd = node(f, k);
+            if (nd >= 0) adj[nd] += cot.f[f][row(k)];
+        }
+    }
+    if (cot.trten)
+        for (int m = 1; m <= ncnst; ++m)
+            for (int k = 1; k <= mkx; ++k) {
+                const int nd = node(ADF_TR + m - 1, k);
+                if (nd >= 0) adj[nd] += cot.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + col];
+            }
+    for (int q = 0; q < ADC_COUNT; ++q) {
+        if (!cot.scal[q]) continue;
+        const int nd = node(ADF_TR + ncnst + q, 0);
+        if (nd >= 0) adj[nd] += cot.scal[q][col];
+    }
+    const int nin = ad_inputs(mkx);
+    for (int q = t.n - 1; q >= nin; --q) {
+        const double w = adj[q];
+        if (w == 0.0) continue;
+        const TapeNode nd = t.nodes[q];
+        if (nd.a >= 0) adj[nd.a] += w * nd.da;
+        if (nd.b >= 0) adj[nd.b] += w * nd.db;
+    }
+    for (int k = 1; k <= mkx; ++k) {
+        if (g.t0) g.t0[row(k)] = adj[ADI_T0 * KS + k];
+        if (g.qv0) g.qv0[row(k)] = adj[ADI_QV0 * KS + k];
+        if (g.u0) g.u0[row(k)] = adj[ADI_U0 * KS + k];
+        if (g.v0) g.v0[row(k)] = adj[ADI_V0 * KS + k];
+    }
+    if (g.tke)
+        for (int k = 0; k <= mkx; ++k) g.tke[row(k)] = adj[ADI_TKE * KS + k];
+    if (g.pblh) g.pblh[col] = 0.0;
+    if (g.par)
+        for (int j = 0; j < 3; ++j) g.par[(size_t)j * ldc + col] = adj[ADI_PARAMS * KS + j];
+}
+
+}  // namespace uwshcu
+#endif

@@ -1,0 +1,72 @@
+// uwshcu_adjoint.hip -- adjoint kernels: record the FP64 forward pass of each column
+// on a tape (Column<AD, AD>), then sweep the tape backwards from the outputs'
+// cotangents to the gradients of t0, qv0, u0, v0, tke, pblh and (rpen, rle, rkm).
+// One thread per column; tapes live in HBM, one contiguous region per column slot.
+// Compiled with -ffp-contract=off: the recorded values equal the FP64 forward path's.
+#include "uwshcu_adjoint.h"
+#include "uwshcu_kernels.h"
+
+namespace uwshcu {
+
+__global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                                                           const double* __restrict__ estbl, TapeNode* tape_base,
+                                                           long long cap, int* node_out) {
+    __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
+    for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
+    __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    TapeCtx& tape = args.ad.tapes[j];
+    ad_begin(tape, tape_base + (size_t)j * (size_t)cap, (int)cap, ad_inputs(args.mkx));
+    const ThermoT<AD> T{consts, s_estbl};
+    const Column<AD, AD> col(args, T, args.col0 + j, j);
+    col.run();
+    col.ad_export(node_out);
+    if (tape.overflow) record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
+}
+
+__global__ __launch_bounds__(64) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
+                                                          long long ldc, const TapeCtx* tapes, const int* node_out,
+                                                          double* adj_base, long long cap, AdCotangents cot,
+                                                          AdGradients g) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    ad_reverse(tapes[j], adj_base + (size_t)j * (size_t)cap, node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc,
+               col0 + j, cot, g);
+}
+
+// Sum of the per-column parameter gradients par[3][ldc] over ncol columns, in FP64:
+// one workgroup per parameter, a strided partial sum per thread, then a tree in LDS.
+__global__ __launch_bounds__(256) void param_sum_kernel(const double* par, long long ldc, int ncol, double* sum) {
+    __shared__ double s[256];
+    const int p = blockIdx.x;
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) acc += par[(size_t)p * ldc + c];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sum[p] = s[0];
+}
+
+hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                          TapeNode* tape_base, long long cap, int* node_out, double* adj_base,
+                          const AdCotangents& cot, const AdGradients& g, hipStream_t stream) {
+    const int block = 64;
+    const int grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(adjoint_record_kernel, dim3(grid), dim3(block), 0, stream, args, n, dconsts, destbl, tape_base,
+                       cap, node_out);
+    hipLaunchKernelGGL(adjoint_sweep_kernel, dim3(grid), dim3(block), 0, stream, n, args.col0, args.mkx, args.ncnst,
+                       args.wld, (long long)args.ldc, (const TapeCtx*)args.ad.tapes, (const int*)node_out, adj_base,
+                       cap, cot, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_param_sum(const double* par, long long ldc, int ncol, double* sum, hipStream_t stream) {
+    hipLaunchKernelGGL(param_sum_kernel, dim3(3), dim3(256), 0, stream, par, ldc, ncol, sum);
+    return hipGetLastError();
+}
+
+}  // namespace uwshcu

@@ -38,6 +38,11 @@ struct uwshcu_ctx {
     int* d_count = nullptr;
     size_t mixed_cols = 0;
     double mixed_tol = 1.e-4;
+    // adjoint: tape nodes per column (0 = 8192 per level interface) and a per-column
+    // parameter-gradient buffer for callers that only want the sums
+    long long tape_cap = 0;
+    double* d_parcol = nullptr;
+    size_t parcol_cols = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
 };
@@ -203,6 +208,83 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     return UWSHCU_OK;
 }
 
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst, double dt,
+                const Chunk& ch, int* diag, const uwshcu_cotangents* cot, const uwshcu_gradients* grad,
+                hipStream_t stream) {
+    using namespace uwshcu;
+    if (!ctx || !cot || !grad || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 ||
+        ncnst > UWSHCU_NCNST_MAX)
+        return UWSHCU_ERR_ARG;
+    if (ncol == 0) return UWSHCU_OK;
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    const int KS = pver + 1;
+    const long long cap = ctx->tape_cap > 0 ? ctx->tape_cap : 8192LL * KS;
+    // per-column pieces: AD workspace, AD outputs, output nodes, tape header, tape, adjoints
+    const size_t b_ws = ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD));
+    const size_t b_out = (size_t)ad_fields(ncnst) * KS * sizeof(AD);
+    const size_t b_nodes = (size_t)ad_fields(ncnst) * KS * sizeof(int);
+    const size_t per_col = b_ws + b_out + b_nodes + sizeof(TapeCtx) + (size_t)cap * (sizeof(TapeNode) + sizeof(double));
+    long long nb = (long long)(ctx->ws_limit / (per_col + 1024));
+    if (nb > ncol) nb = ncol;
+    if (nb < 1) return UWSHCU_ERR_NOMEM;
+    if (nb > 64) nb = (nb / 64) * 64;
+    const size_t o_out = align256(b_ws * nb), o_nodes = o_out + align256(b_out * nb);
+    const size_t o_tapes = o_nodes + align256(b_nodes * nb), o_tape = o_tapes + align256(sizeof(TapeCtx) * nb);
+    const size_t o_adj = o_tape + align256((size_t)cap * sizeof(TapeNode) * nb);
+    const size_t total = o_adj + align256((size_t)cap * sizeof(double) * nb);
+    int rc = ensure_ws(ctx, total);
+    if (rc) return rc;
+    char* base = (char*)ctx->d_ws;
+    double* parcol = grad->par_col;
+    if (!parcol && grad->par_sum) {
+        if ((size_t)pcols > ctx->parcol_cols) {
+            if (ctx->d_parcol) {
+                if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+                (void)hipFree(ctx->d_parcol);
+                ctx->d_parcol = nullptr;
+                ctx->parcol_cols = 0;
+            }
+            if (!hip_ok(hipMalloc((void**)&ctx->d_parcol, 3 * (size_t)pcols * sizeof(double)))) return UWSHCU_ERR_NOMEM;
+            ctx->parcol_cols = (size_t)pcols;
+        }
+        parcol = ctx->d_parcol;
+    }
+    AdCotangents ct;
+    static_assert(sizeof(AdCotangents) == sizeof(uwshcu_cotangents), "cotangent layouts differ");
+    std::memcpy(&ct, cot, sizeof(ct));
+    const AdGradients g{grad->t0, grad->qv0, grad->u0, grad->v0, grad->tke, grad->pblh, parcol};
+
+    FwdArgs a{};
+    a.ncol = ncol; a.mkx = pver; a.ncnst = ncnst; a.ldc = pcols; a.dt = dt;
+    uwshcu_default_params(&a.par);
+    if (par) a.par = *par;
+    a.ps0 = ch.ps0; a.zs0 = ch.zs0; a.p0 = ch.p0; a.z0 = ch.z0; a.dp0 = ch.dp0; a.dpdry0 = ch.dpdry0;
+    a.u0 = ch.u0; a.v0 = ch.v0; a.qv0 = ch.qv0; a.ql0 = ch.ql0; a.qi0 = ch.qi0; a.t0 = ch.t0; a.s0 = ch.s0;
+    a.tr0 = ch.tr0; a.tke = ch.tke; a.pblh = ch.pblh; a.cush = ch.cush;
+    a.umf = ch.umf; a.slflx = ch.slflx; a.qtflx = ch.qtflx; a.flxprc1 = ch.flxprc1; a.flxsnow1 = ch.flxsnow1;
+    a.qvten = ch.qvten; a.qlten = ch.qlten; a.qiten = ch.qiten; a.sten = ch.sten; a.uten = ch.uten;
+    a.vten = ch.vten; a.trten = ch.trten; a.qrten = ch.qrten; a.qsten = ch.qsten; a.precip = ch.precip;
+    a.snow = ch.snow; a.evapc = ch.evapc; a.cufrc = ch.cufrc; a.qcu = ch.qcu; a.qlu = ch.qlu; a.qiu = ch.qiu;
+    a.cbmf = ch.cbmf; a.qc = ch.qc; a.rliq = ch.rliq; a.cnt = ch.cnt; a.cnb = ch.cnb; a.wu = ch.wu;
+    a.diag = diag;
+    a.err = ctx->d_err;
+    a.ws = base;
+    a.wld = nb;
+    a.ad.out = (AD*)(base + o_out);
+    a.ad.tapes = (TapeCtx*)(base + o_tapes);
+    for (long long c0 = 0; c0 < ncol; c0 += nb) {
+        const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
+        a.col0 = (int)c0;
+        if (!hip_ok(launch_adjoint(a, n, ctx->d_consts, ctx->d_estbl, (TapeNode*)(base + o_tape), cap,
+                                   (int*)(base + o_nodes), (double*)(base + o_adj), ct, g, stream)))
+            return UWSHCU_ERR_HIP;
+    }
+    if (grad->par_sum && !hip_ok(launch_param_sum(parcol, pcols, ncol, grad->par_sum, stream))) return UWSHCU_ERR_HIP;
+    return UWSHCU_OK;
+}
+
 std::mutex g_dropin_mu;
 uwshcu_ctx* g_dropin_ctx = nullptr;
 uwshcu_params g_dropin_par{10.0, 0.1, 14.0};
@@ -328,6 +410,7 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_amb) (void)hipFree(ctx->d_amb);
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     if (ctx->d_count) (void)hipFree(ctx->d_count);
+    if (ctx->d_parcol) (void)hipFree(ctx->d_parcol);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return UWSHCU_OK;
@@ -349,6 +432,13 @@ int uwshcu_mixed_recomputed(uwshcu_ctx* ctx, void* stream, int* count) {
     if (!hip_ok(hipMemcpyAsync(count, ctx->d_count, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream)) ||
         !hip_ok(hipStreamSynchronize((hipStream_t)stream)))
         return UWSHCU_ERR_HIP;
+    return UWSHCU_OK;
+}
+
+int uwshcu_ctx_set_tape_capacity(uwshcu_ctx* ctx, long long nodes_per_column) {
+    if (!ctx || nodes_per_column < 0 || nodes_per_column > (1LL << 30)) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tape_cap = nodes_per_column;
     return UWSHCU_OK;
 }
 
@@ -384,6 +474,34 @@ int uwshcu_forward_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* p
              umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
              qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
     return run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream);
+}
+
+int uwshcu_adjoint_device(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,
+                          double dt, const double* ps0, const double* zs0, const double* p0, const double* z0,
+                          const double* dp0, const double* u0, const double* v0, const double* qv0,
+                          const double* ql0, const double* qi0, const double* t0, const double* s0,
+                          const double* tr0, const double* tke, const double* cldfrc, const double* concldfrc,
+                          const double* pblh, double* cush, double* umf, double* slflx, double* qtflx,
+                          double* flxprc1, double* flxsnow1, double* qvten, double* qlten, double* qiten,
+                          double* sten, double* uten, double* vten, double* trten, double* qrten, double* qsten,
+                          double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
+                          double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
+                          const double* dpdry0, double* wu, int* diag, const uwshcu_cotangents* cot,
+                          const uwshcu_gradients* grad, void* stream) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!ctx) return UWSHCU_ERR_ARG;
+    const void* req[] = {ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tke, pblh, cush, umf, slflx,
+                         qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, qrten, qsten, precip,
+                         snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu};
+    for (const void* p : req)
+        if (!p) return UWSHCU_ERR_ARG;
+    if (ncnst > 0 && (!tr0 || !trten)) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Chunk ch{ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tr0, tke, pblh, dpdry0, cush,
+             umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
+             qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
+    return run_adjoint(ctx, par, pcols, ncol, pver, ncnst, dt, ch, diag, cot, grad, (hipStream_t)stream);
 }
 
 int uwshcu_status(uwshcu_ctx* ctx, void* stream, int* bad_col) {

@@ -58,10 +58,17 @@ enum WsArray : int {
 };
 // Per-constituent blocks follow the fixed plume arrays: SSTR0, TRU, TRUE, TRFLX (ncnst arrays each).
 UWSHCU_HD inline int ws_arrays(int ncnst) { return WS_FIXED + 4 * ncnst; }
-// Workspace bytes per column: FP64 environment block, then the R-typed plume block.
-UWSHCU_HD inline size_t ws_bytes_per_column(int mkx, int ncnst, size_t r_size) {
-    return (size_t)(mkx + 1) * ((size_t)ENV_ARRAYS * sizeof(double) + (size_t)ws_arrays(ncnst) * r_size);
+// Workspace bytes per column: environment block of the FP64 type (d_size bytes per
+// value), then the R-typed plume block.
+UWSHCU_HD inline size_t ws_bytes_per_column(int mkx, int ncnst, size_t r_size, size_t d_size = sizeof(double)) {
+    return (size_t)(mkx + 1) * ((size_t)ENV_ARRAYS * d_size + (size_t)ws_arrays(ncnst) * r_size);
 }
+
+// Reverse-mode state of one launch (adjoint kernel only; null pointers otherwise).
+struct AdArgs {
+    AD* out;           // tape values of the outputs (layout: AdField below)
+    TapeCtx* tapes;    // one per workspace slot
+};
 
 struct FwdArgs {
     int ncol, mkx, ncnst, ldc;
@@ -82,27 +89,49 @@ struct FwdArgs {
     const int* colcount;
     const double* cush_in;   // optional: read the input cush here instead of from cush
     int* amb_flag;           // optional [ldc]: 1 where an FP32 branch decision was within rounding
+    AdArgs ad;               // adjoint kernel only
 };
+
+// Reverse-mode state of one launch (adjoint kernel only).  Output fields are held as
+// tape values while the column runs: field f, level k of workspace slot j lives at
+// out[(f * (mkx+1) + k) * wld + j]; fields are ADF_* below, then trten(m) at
+// ADF_TR + m - 1, then the column scalars at ADF_TR + ncnst + ADC_*.
+enum AdField : int {
+    ADF_UMF, ADF_SLFLX, ADF_QTFLX, ADF_FLXPRC1, ADF_FLXSNOW1, ADF_WU,
+    ADF_QVTEN, ADF_QLTEN, ADF_QITEN, ADF_STEN, ADF_UTEN, ADF_VTEN, ADF_QRTEN, ADF_QSTEN, ADF_EVAPC,
+    ADF_CUFRC, ADF_QCU, ADF_QLU, ADF_QIU, ADF_QC,
+    ADF_TR
+};
+enum AdColumnField : int { ADC_PRECIP, ADC_SNOW, ADC_CBMF, ADC_RLIQ, ADC_CNT, ADC_CNB, ADC_CUSH, ADC_COUNT };
+UWSHCU_HD inline int ad_fields(int ncnst) { return ADF_TR + ncnst + ADC_COUNT; }
+// Independent variables: node f*(mkx+1)+k for t0, qv0, u0, v0 (f = 0..3, k = 1..mkx)
+// and tke (f = 4, k = 0..mkx), then rpen, rle, rkm.
+enum AdInput : int { ADI_T0, ADI_QV0, ADI_U0, ADI_V0, ADI_TKE, ADI_PARAMS };
+UWSHCU_HD inline int ad_inputs(int mkx) { return ADI_PARAMS * (mkx + 1) + 3; }
 
 // Exit reasons reported in diag row 0 (same numbering as the oracle).
 enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
 
-template <class R>
+// R: bulk (plume) arithmetic type; D: the FP64 type (double, or AD in the adjoint).
+template <class R, class D = double>
 struct Column {
+    static constexpr bool kAD = is_ad<D>::value;
     const FwdArgs& a;
-    const Thermo& T;
+    const ThermoT<D>& T;
     const uwshcu_consts& C;
     const int i, iw, mkx, KS, ncnst;
     const size_t ldc, wld;
-    double* __restrict__ env;   // ENV_ARRAYS * KS rows of wld doubles
+    D* __restrict__ env;        // ENV_ARRAYS * KS rows of wld D values
     R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
+    TapeCtx* tape;              // adjoint kernel: this column's tape
 
-    UWSHCU_HD Column(const FwdArgs& args, const Thermo& th, int col, int slot)
+    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D>& th, int col, int slot)
         : a(args), T(th), C(*th.c), i(col), iw(slot), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
-          ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<double*>(args.ws)),
-          ws(reinterpret_cast<R*>(reinterpret_cast<double*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
-                                                                         (size_t)args.wld)) {}
+          ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<D*>(args.ws)),
+          ws(reinterpret_cast<R*>(reinterpret_cast<D*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
+                                                                    (size_t)args.wld)),
+          tape(args.ad.tapes ? args.ad.tapes + slot : nullptr) {}
 
     // caller arrays: UW level k <-> CAM row mkx-k (layers k=1..mkx and interfaces k=0..mkx)
     UWSHCU_HD inline size_t row(int k) const { return (size_t)(mkx - k) * ldc + i; }
@@ -112,23 +141,60 @@ struct Column {
                p == a.flxprc1 || p == a.flxsnow1 || p == a.wu;
     }
 #endif
-    UWSHCU_HD inline double in(const double* p, int k) const {
+    // tape value of output field f at level k (adjoint kernel)
+    UWSHCU_HD inline D& adout(int f, int k) const {
+        if constexpr (kAD) return a.ad.out[((size_t)f * KS + k) * wld + iw];
+        else return *(D*)nullptr;
+    }
+    UWSHCU_HD inline int ad_field(const double* p) const {
+        const double* const f[] = {a.umf, a.slflx, a.qtflx, a.flxprc1, a.flxsnow1, a.wu, a.qvten, a.qlten, a.qiten,
+                                   a.sten, a.uten, a.vten, a.qrten, a.qsten, a.evapc, a.cufrc, a.qcu, a.qlu, a.qiu,
+                                   a.qc};
+        for (int q = 0; q < ADF_TR; ++q)
+            if (p == f[q]) return q;
+        if (p == a.precip) return ADF_TR + ncnst + ADC_PRECIP;
+        if (p == a.snow) return ADF_TR + ncnst + ADC_SNOW;
+        if (p == a.cbmf) return ADF_TR + ncnst + ADC_CBMF;
+        if (p == a.rliq) return ADF_TR + ncnst + ADC_RLIQ;
+        if (p == a.cnt) return ADF_TR + ncnst + ADC_CNT;
+        if (p == a.cnb) return ADF_TR + ncnst + ADC_CNB;
+        return ADF_TR + ncnst + ADC_CUSH;
+    }
+    UWSHCU_HD inline D in(const double* p, int k) const {
         UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
-        return p[row(k)];
+        if constexpr (kAD) {
+            const int base = p == a.t0 ? ADI_T0 : p == a.qv0 ? ADI_QV0 : p == a.u0 ? ADI_U0 : p == a.v0 ? ADI_V0
+                           : p == a.tke ? ADI_TKE : -1;
+            if (base >= 0) return D(p[row(k)], base * KS + k, tape);
+        }
+        return D(p[row(k)]);
     }
-    UWSHCU_HD inline double& out(double* p, int k) const {
+    UWSHCU_HD inline D& out(double* p, int k) const {
         UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
-        return p[row(k)];
+        if constexpr (kAD) return adout(ad_field(p), k);
+        else return p[row(k)];
     }
-    UWSHCU_HD inline double trin(int m, int k) const {   // m 1-based
+    // column scalars (precip, snow, cbmf, rliq, cnt, cnb, cush)
+    UWSHCU_HD inline D& cout(double* p) const {
+        if constexpr (kAD) return adout(ad_field(p), 0);
+        else return p[i];
+    }
+    UWSHCU_HD inline D trin(int m, int k) const {   // m 1-based
         UW_CHECK(m >= 1 && m <= ncnst && k >= 1 && k <= mkx);
-        return a.tr0[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
+        return D(a.tr0[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i]);
     }
-    UWSHCU_HD inline double& trout(int m, int k) const {
+    UWSHCU_HD inline D& trout(int m, int k) const {
         UW_CHECK(m >= 1 && m <= ncnst && k >= 1 && k <= mkx);
-        return a.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
+        if constexpr (kAD) return adout(ADF_TR + m - 1, k);
+        else return a.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i];
     }
-    UWSHCU_HD inline double& E(int arr, int k) const {
+    // tunables: j = 0 rpen, 1 rle, 2 rkm
+    UWSHCU_HD inline D param(int j) const {
+        const double v = j == 0 ? a.par.rpen : j == 1 ? a.par.rle : a.par.rkm;
+        if constexpr (kAD) return D(v, ADI_PARAMS * KS + j, tape);
+        else return v;
+    }
+    UWSHCU_HD inline D& E(int arr, int k) const {
         UW_CHECK(arr >= 0 && arr < ENV_ARRAYS && k >= 0 && k <= mkx && iw >= 0 && (long long)iw < (long long)wld);
         return env[((size_t)arr * KS + k) * wld + iw];
     }
@@ -141,16 +207,16 @@ struct Column {
         return ws[((size_t)(WS_FIXED + blk * ncnst + (m - 1)) * KS + k) * wld + iw];
     }
 
-    UWSHCU_HD inline R thv_of(double thj, double qvj, double qlj, double qij) const {
+    UWSHCU_HD inline R thv_of(D thj, D qvj, D qlj, D qij) const {
         return R(thj * (1.0 + C.zvir * qvj - qlj - qij));
     }
 
     // minmod slopes (uwshcu.F90:4831-4861) of field(k) -> E(dst, k), k = 1..mkx, in FP64
     template <class F>
     UWSHCU_HD inline void slope(F field, int dst) const {
-        double below = (field(2) - field(1)) / (in(a.p0, 2) - in(a.p0, 1));
+        D below = (field(2) - field(1)) / (in(a.p0, 2) - in(a.p0, 1));
         for (int k = 2; k <= mkx; ++k) {
-            const double above = (field(k) - field(k - 1)) / (in(a.p0, k) - in(a.p0, k - 1));
+            const D above = (field(k) - field(k - 1)) / (in(a.p0, k) - in(a.p0, k - 1));
             E(dst, k - 1) = (above > 0.0) ? fmax(0.0, fmin(above, below)) : fmin(0.0, fmax(above, below));
             below = above;
         }
@@ -175,15 +241,15 @@ struct Column {
         const int thvb = eb == A_THL0 ? A_THV0BOT : B_THV0BOT, thvt = thvb + 1;
         const int thvlb = eb == A_THL0 ? A_THVL0BOT : B_THVL0BOT, thvlt = thvlb + 1;
         for (int k = 1; k <= mkx; ++k) {
-            double th, qv, ql, qi;
-            const double pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
-            const double thl0bot = E(thl, k) + E(ssthl, k) * (pb - pm);
-            const double qt0bot = E(qt, k) + E(ssqt, k) * (pb - pm);
+            D th, qv, ql, qi;
+            const D pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
+            const D thl0bot = E(thl, k) + E(ssthl, k) * (pb - pm);
+            const D qt0bot = E(qt, k) + E(ssqt, k) * (pb - pm);
             if (T.conden(pb, thl0bot, qt0bot, th, qv, ql, qi)) return false;
             E(thvb, k) = th * (1.0 + C.zvir * qv - ql - qi);
             E(thvlb, k) = thl0bot * (1.0 + C.zvir * qt0bot);
-            const double thl0top = E(thl, k) + E(ssthl, k) * (pt - pm);
-            const double qt0top = E(qt, k) + E(ssqt, k) * (pt - pm);
+            const D thl0top = E(thl, k) + E(ssthl, k) * (pt - pm);
+            const D qt0top = E(qt, k) + E(ssqt, k) * (pt - pm);
             if (T.conden(pt, thl0top, qt0top, th, qv, ql, qi)) return false;
             E(thvt, k) = th * (1.0 + C.zvir * qv - ql - qi);
             E(thvlt, k) = thl0top * (1.0 + C.zvir * qt0top);
@@ -194,10 +260,10 @@ struct Column {
     // Saturation state of the layer-k mean environment, evaluated once per column
     // (uwshcu.F90:2390-2397 evaluate it in every buoyancy-sorting iteration with the
     // same arguments).  Returns false when the saturation adjustment fails.
-    UWSHCU_HD bool env_mid(int k, double& thv0j, double& qs0) const {
-        double& tv = E(E_THV0MID, k);
+    UWSHCU_HD bool env_mid(int k, D& thv0j, D& qs0) const {
+        D& tv = E(E_THV0MID, k);
         if (tv == 0.0) {
-            double th, qv, ql, qi, es, qs;
+            D th, qv, ql, qi, es, qs;
             if (T.conden(in(a.p0, k), E(A_THL0, k), E(A_QT0, k), th, qv, ql, qi)) {
                 tv = -1.0;
             } else {
@@ -257,21 +323,45 @@ struct Column {
             out(a.qiu, k) = 0.0; out(a.qc, k) = 0.0;
             for (int m = 1; m <= ncnst; ++m) trout(m, k) = 0.0;
         }
-        a.precip[i] = 0.0; a.snow[i] = 0.0; a.cush[i] = -1.0; a.cbmf[i] = 0.0; a.rliq[i] = 0.0;
-        a.cnt[i] = (double)(mk + 1) - 1.0;        // cnt = 1 flipped: mkx+1-1
-        a.cnb[i] = (double)(mk + 1) - (double)mk; // cnb = mkx flipped
+        cout(a.precip) = 0.0; cout(a.snow) = 0.0; cout(a.cush) = -1.0; cout(a.cbmf) = 0.0; cout(a.rliq) = 0.0;
+        cout(a.cnt) = D(mk + 1) - 1.0;        // cnt = 1 flipped: mkx+1-1
+        cout(a.cnb) = D(mk + 1) - D(mk); // cnb = mkx flipped
     }
 
     UWSHCU_HD void run() const;
+
+    // Adjoint kernel, after run(): copy the recorded outputs' values to the caller's
+    // arrays and their tape nodes to nodes[(f * (mkx+1) + k) * wld + slot].
+    UWSHCU_HD void ad_export(int* nodes) const {
+        if constexpr (kAD) {
+            auto put = [&](int f, int k, double* dst) {
+                const D& v = adout(f, k);
+                *dst = v.v;
+                nodes[((size_t)f * KS + k) * wld + iw] = v.i;
+            };
+            double* const fields[] = {a.umf, a.slflx, a.qtflx, a.flxprc1, a.flxsnow1, a.wu, a.qvten, a.qlten,
+                                      a.qiten, a.sten, a.uten, a.vten, a.qrten, a.qsten, a.evapc, a.cufrc, a.qcu,
+                                      a.qlu, a.qiu, a.qc};
+            for (int f = 0; f < ADF_TR; ++f)
+                for (int k = (f <= ADF_WU ? 0 : 1); k <= mkx; ++k) put(f, k, &fields[f][row(k)]);
+            for (int m = 1; m <= ncnst; ++m)
+                for (int k = 1; k <= mkx; ++k)
+                    put(ADF_TR + m - 1, k, &a.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + i]);
+            double* const scal[] = {a.precip, a.snow, a.cbmf, a.rliq, a.cnt, a.cnb, a.cush};
+            for (int q = 0; q < ADC_COUNT; ++q) put(ADF_TR + ncnst + q, 0, &scal[q][i]);
+        } else {
+            (void)nodes;
+        }
+    }
 };
 
-template <class R>
-UWSHCU_HD void Column<R>::run() const {
+template <class R, class D>
+UWSHCU_HD void Column<R, D>::run() const {
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
     const R xlv = R(C.xlv), xls = R(C.xls), cp = R(C.cp), zvir = R(C.zvir), r = R(C.r), g = R(C.g);
-    const R rle = R(a.par.rle), rkm = R(a.par.rkm), rpen = R(a.par.rpen);
+    const R rle = R(param(1)), rkm = R(param(2)), rpen = R(param(0));
     const R dt = R(a.dt);
     const double dtd = a.dt;
     const int ixl = C.ixnumliq, ixi = C.ixnumice;
@@ -284,18 +374,20 @@ UWSHCU_HD void Column<R>::run() const {
     // in FP64 by the mixed-precision driver.  No effect in FP64.
     constexpr bool kFp32 = sizeof(R) == sizeof(float);
     int amb = 0;
-    auto near = [&](double dist, double scale) {
-        if (kFp32 && fabs(dist) <= 1.e-5 * fabs(scale)) amb = 1;
+    auto near = [&](auto dist, auto scale) {
+        if constexpr (kFp32) {
+            if (fabs(dist) <= 1.e-5 * fabs(scale)) amb = 1;
+        }
     };
 
     int exit_code = EX_NONE, path = 0, exit_iter = 0;
     int kinv = 0, klcl = 0, klfc = 0, krel = 0, kbup = 0, kpen = 0;
     int limit_ufrc = 0, limit_ppen = 0, limit_emf = 0;
-    double th, qvj, qlj, qij;
+    D th, qvj, qlj, qij;
 
     // ---- wet bulb for the evaporation limiter (uwshcu.F90:1216; findsp 5164-5454) ----
     for (int k = 1; k <= mkx; ++k) {
-        double qsp, enerr;
+        D qsp, enerr;
         const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
         if (doit == 0) record_error(a.err, UWSHCU_ERR_FINDSP_CONV, i);
         else if (doit == 2 && enerr > 1.e-4) record_error(a.err, UWSHCU_ERR_FINDSP_ENTH, i);
@@ -306,8 +398,8 @@ UWSHCU_HD void Column<R>::run() const {
     for (int k = 1; k <= mkx; ++k) E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
     for (int k = 0; k <= mkx; ++k) E(A_EXNS0, k) = upow(in(a.ps0, k) / C.p00, C.rovcp);
     for (int k = 1; k <= mkx; ++k) {
-        const double qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
-        const double qt0 = (qv0 + ql0 + qi0);
+        const D qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
+        const D qt0 = (qv0 + ql0 + qi0);
         E(A_QT0, k) = qt0;
         E(A_THL0, k) = (t0 - C.xlv * ql0 / C.cp - C.xls * qi0 / C.cp) / E(A_EXN0, k);
     }
@@ -321,18 +413,18 @@ UWSHCU_HD void Column<R>::run() const {
     zero_pass_state();
     for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
 
-    double cush = a.cush_in ? a.cush_in[i] : a.cush[i];
-    const double pblh = a.pblh[i];
+    D cush = D(a.cush_in ? a.cush_in[i] : a.cush[i]);
+    const D pblh = D(a.pblh[i]);
 
     // iteration-1 values restored for the implicit-CIN pass (uwshcu.F90:1693-1713)
     int kinv_o = 0, klcl_o = 0, klfc_o = 0;
-    double plcl_o = 0, plfc_o = 0, tkeavg_o = 0, thvlmin_o = 0, qtsrc_o = 0, thvlsrc_o = 0, thlsrc_o = 0;
-    double usrc_o = 0, vsrc_o = 0, thv0lcl_o = 0, cin_i = 0, cinlcl_i = 0, ke = 0;
-    double cush_s = 0;
+    D plcl_o = 0, plfc_o = 0, tkeavg_o = 0, thvlmin_o = 0, qtsrc_o = 0, thvlsrc_o = 0, thlsrc_o = 0;
+    D usrc_o = 0, vsrc_o = 0, thv0lcl_o = 0, cin_i = 0, cinlcl_i = 0, ke = 0;
+    D cush_s = 0;
 
     // locals that persist across stages
-    double plcl = 0, plfc = 0, tkeavg = 0, thvlmin = 0, qtsrc = 0, thvlsrc = 0, thlsrc = 0, usrc = 0, vsrc = 0;
-    double thv0lcl = 0, cin = 0, cinlcl = 0;
+    D plcl = 0, plfc = 0, tkeavg = 0, thvlmin = 0, qtsrc = 0, thvlsrc = 0, thlsrc = 0, usrc = 0, vsrc = 0;
+    D thv0lcl = 0, cin = 0, cinlcl = 0;
     R prel = 0, thv0rel = 0, cbmf = 0, wlcl = 0, ufrclcl = 0, ppen = 0, thlu_top = 0, qtu_top = 0;
     R rho0j = 0, drage = 0, bogbot = 0, bogtop = 0, wtwb = 0;
     R trsrc_tmp = 0;
@@ -346,7 +438,7 @@ UWSHCU_HD void Column<R>::run() const {
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
         const int e_thvb = (iter == 1) ? A_THV0BOT : B_THV0BOT, e_thvt = e_thvb + 1;
         const int e_thvlb = (iter == 1) ? A_THVL0BOT : B_THVL0BOT, e_thvlt = e_thvlb + 1;
-        const double tscaleh = cush;
+        const D tscaleh = cush;
         cush = -1.0;
 
         // ---- S1 kinv (uwshcu.F90:1455-1468) ----
@@ -358,11 +450,11 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S2 PBL averages (uwshcu.F90:1484-1499) ----
         {
-            double dpsum = 0.0;
+            D dpsum = 0.0;
             tkeavg = 0.0;
             thvlmin = 1000.0;
             for (int k = 0; k <= kinv - 1; ++k) {
-                double dpi;
+                D dpi;
                 if (k == 0) dpi = in(a.ps0, 0) - in(a.p0, 1);
                 else if (k == kinv - 1) dpi = in(a.p0, kinv - 1) - in(a.ps0, kinv - 1);
                 else dpi = in(a.p0, k) - in(a.p0, k + 1);
@@ -391,8 +483,8 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S5 thv0lcl (uwshcu.F90:1559-1567) ----
         {
-            const double thl0lcl = E(e_thl, klcl) + E(e_ssthl, klcl) * (plcl - in(a.p0, klcl));
-            const double qt0lcl = E(e_qt, klcl) + E(e_ssqt, klcl) * (plcl - in(a.p0, klcl));
+            const D thl0lcl = E(e_thl, klcl) + E(e_ssthl, klcl) * (plcl - in(a.p0, klcl));
+            const D qt0lcl = E(e_qt, klcl) + E(e_ssqt, klcl) * (plcl - in(a.p0, klcl));
             if (T.conden(plcl, thl0lcl, qt0lcl, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
             thv0lcl = th * (1.0 + C.zvir * qvj - qlj - qij);
         }
@@ -403,7 +495,7 @@ UWSHCU_HD void Column<R>::run() const {
         plfc = 0.0;
         klfc = mkx;
         if (klcl >= kinv) {
-            double thvubot = 0.0, thvutop = 0.0;
+            D thvubot = 0.0, thvutop = 0.0;
             for (int k = kinv; k <= mkx - 1; ++k) {
                 if (k < klcl) {
                     thvubot = thvlsrc;
@@ -432,9 +524,9 @@ UWSHCU_HD void Column<R>::run() const {
             cinlcl = 0.0;
             for (int k = kinv; k <= mkx - 1; ++k) {
                 if (T.conden(in(a.ps0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-                const double thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
+                const D thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
                 if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-                const double thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
+                const D thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
                 T.getbuoy(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                 if (plfc > 0.0) { klfc = k; break; }
             }
@@ -451,10 +543,10 @@ UWSHCU_HD void Column<R>::run() const {
             thvlmin_o = thvlmin; qtsrc_o = qtsrc; thvlsrc_o = thvlsrc; thlsrc_o = thlsrc; usrc_o = usrc;
             vsrc_o = vsrc; thv0lcl_o = thv0lcl;
         } else {
-            const double del_CIN = cin - cin_i;
+            const D del_CIN = cin - cin_i;
             if (cin_i + cin > 0.0) near(del_CIN, cin_i + cin);
             if (del_CIN > 0.0) {
-                const double alpha = compute_alpha(del_CIN, ke);
+                const D alpha = compute_alpha(del_CIN, ke);
                 cin = cin_i + alpha * del_CIN;
                 cinlcl = cinlcl_i;
                 kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
@@ -464,7 +556,7 @@ UWSHCU_HD void Column<R>::run() const {
             } else {
                 // iteration-1 outputs stand (uwshcu.F90:1897-2010): they are already in the
                 // output arrays; flxprc1/flxsnow1 were never written and stay zero.
-                a.cush[i] = cush_s;
+                cout(a.cush) = cush_s;
                 path = 1;
                 goto Ldone;
             }
@@ -486,35 +578,35 @@ UWSHCU_HD void Column<R>::run() const {
         // ---- S9 cloud-base closure (uwshcu.F90:2092-2184), FP64 ----
         R wtw;
         {
-            const double wcrit = sqrt(2.0 * cin * 1.0);
-            const double sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
-            double mu = wcrit / sigmaw / 1.4142;
+            const D wcrit = sqrt(2.0 * cin * 1.0);
+            const D sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
+            D mu = wcrit / sigmaw / 1.4142;
             if (mu >= 3.0) { exit_code = EX_MU3; goto L333; }
-            const double rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
-            double cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
-            const double cbmflimit = 0.9 * in(a.dp0, kinv - 1) / C.g / dtd;
-            double mumin0 = 0.0;
+            const D rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
+            D cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
+            const D cbmflimit = 0.9 * in(a.dp0, kinv - 1) / C.g / dtd;
+            D mumin0 = 0.0;
             if (cbmf_d > cbmflimit) mumin0 = sqrt(-ulog(2.5066 * cbmflimit / rho0inv / sigmaw));
             mu = fmax(fmax(mu, mumin0), 0.906);
-            const double mulcl = sqrt(2.0 * cinlcl * 1.0) / 1.4142 / sigmaw;
-            double mulclstar;
+            const D mulcl = sqrt(2.0 * cinlcl * 1.0) / 1.4142 / sigmaw;
+            D mulclstar;
             {
-                const double e1 = uexp(-(mu * mu)) / 2.5066, ef = uerfc(mu);
+                const D e1 = uexp(-(mu * mu)) / 2.5066, ef = uerfc(mu);
                 mulclstar = sqrt(fmax(0.0, 2.0 * (e1 * e1) * (1.0 / (ef * ef) - 0.25 / (0.10 * 0.10))));
             }
             if (mulcl > 1.e-8 && mulcl > mulclstar) {
-                const double mumin2 = compute_mumin2(mulcl, 0.10, mu);
+                const D mumin2 = compute_mumin2<D>(mulcl, D(0.10), mu);
                 if (mu > mumin2) record_error(a.err, UWSHCU_ERR_MUMIN2, i);
                 mu = fmax(mu, mumin2);
                 if (mu == mumin2) limit_ufrc = 1;
             }
             if (mu == 0.906) limit_ufrc = 1;
             cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
-            const double winv = sigmaw * (2.0 / 2.5066) * uexp(-(mu * mu)) / uerfc(mu);
-            const double wtw_d = winv * winv - 2.0 * cinlcl * 1.0;
+            const D winv = sigmaw * (2.0 / 2.5066) * uexp(-(mu * mu)) / uerfc(mu);
+            const D wtw_d = winv * winv - 2.0 * cinlcl * 1.0;
             if (wtw_d <= 0.0) { exit_code = EX_WTW; goto L333; }
-            const double wlcl_d = sqrt(wtw_d);
-            const double ufrclcl_d = cbmf_d / wlcl_d / rho0inv;
+            const D wlcl_d = sqrt(wtw_d);
+            const D ufrclcl_d = cbmf_d / wlcl_d / rho0inv;
             if (ufrclcl_d <= 0.0001) { exit_code = EX_UFRC; goto L333; }
             cbmf = R(cbmf_d);
             wlcl = R(wlcl_d);
@@ -527,8 +619,8 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S10 release-level updraft (uwshcu.F90:2205-2255) ----
         W(W_EMF, krel - 1) = R(0);
-        out(a.umf, krel - 1) = (double)cbmf;
-        out(a.wu, krel - 1) = (double)wrel;
+        out(a.umf, krel - 1) = D(cbmf);
+        out(a.wu, krel - 1) = D(wrel);
         W(W_THLU, krel - 1) = R(thlsrc);
         W(W_QTU, krel - 1) = R(qtsrc);
         if (T.conden(prel, thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
@@ -583,13 +675,13 @@ UWSHCU_HD void Column<R>::run() const {
                 for (int iter_xc = 1; iter_xc <= 2; ++iter_xc) {
                     wtw = R(out(a.wu, km1)) * R(out(a.wu, km1));
                     R thv0j;
-                    double es, qs;
+                    D es, qs;
                     if (k == krel) {
                         if (T.conden(pe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                         thv0j = thv_of(th, qvj, qlj, qij);
-                        T.qsat_es_qs((double)(thle * exne), pe, es, qs);
+                        T.qsat_es_qs(D(thle * exne), pe, es, qs);
                     } else {
-                        double tv;
+                        D tv;
                         if (!env_mid(k, tv, qs)) { exit_code = EX_CONDEN; goto L333; }
                         thv0j = R(tv);
                     }
@@ -607,7 +699,7 @@ UWSHCU_HD void Column<R>::run() const {
                         if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     }
                     const R thvj = thv_of(th, qvj, qlj, qij);
-                    T.qsat_es_qs((double)(thlue * exne), pe, es, qs);
+                    T.qsat_es_qs(D(thlue * exne), pe, es, qs);
                     const R excessu = qtue - R(qs);
                     near(excessu, qtue);
                     near(excess0, qte);
@@ -716,7 +808,7 @@ UWSHCU_HD void Column<R>::run() const {
                     W(W_QTU, k) = qtuk;
                     W(W_UU, k) = uuk;
                     W(W_VU, k) = vuk;
-                    out(a.umf, k) = (double)umfk;
+                    out(a.umf, k) = D(umfk);
                     const R thvuk = thv_of(th, qvj, qlj, qij);
                     W(W_THVU, k) = thvuk;
                     bogbot = rbuoy * (W(W_THVU, km1) / thvebot - R(1));
@@ -745,7 +837,7 @@ UWSHCU_HD void Column<R>::run() const {
                 if (wtw <= R(0)) { kpen = k; break; }   // go to 45
                 {
                     const R wuk = sqrt(wtw);
-                    out(a.wu, k) = (double)wuk;
+                    out(a.wu, k) = D(wuk);
                     if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
                     const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
                     R ufrck = umfk / (rhos0j * wuk);
@@ -754,7 +846,7 @@ UWSHCU_HD void Column<R>::run() const {
                         limit_ufrc = 1;
                         ufrck = rmaxfrac;
                         umfk = rmaxfrac * rhos0j * wuk;
-                        out(a.umf, k) = (double)umfk;
+                        out(a.umf, k) = D(umfk);
                         W(W_FDR, k) = ferk - ulog(umfk / umf_km1) / dpe;
                     }
                     W(W_UFRC, k) = ufrck;
@@ -815,7 +907,7 @@ UWSHCU_HD void Column<R>::run() const {
                 const R ptop = R(in(a.ps0, kpen - 1)) + ppen;
                 if (T.conden(ptop, thlu_top, qtu_top, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 near(qlj + qij - criqc, criqc);
-                const R exntop = R(upow((double)ptop / C.p00, C.rovcp));
+                const R exntop = R(upow(D(ptop) / C.p00, C.rovcp));
                 if ((R(qlj) + R(qij)) > criqc) {
                     const R qlr = R(qlj), qir = R(qij);
                     const R dw = ((qlr + qir) - criqc) * qlr / (qlr + qir);
@@ -829,7 +921,7 @@ UWSHCU_HD void Column<R>::run() const {
                     W(W_DITEN, kpen) = R(0);
                 }
                 const R rhos0j = R(in(a.ps0, kpen - 1)) / (r * R(0.5) * (Er(A_THV0BOT, kpen) + Er(A_THV0TOP, kpen - 1)) * Er(A_EXNS0, kpen - 1));
-                cush = (double)(R(in(a.zs0, kpen - 1)) - ppen / rhos0j / g);
+                cush = D(R(in(a.zs0, kpen - 1)) - ppen / rhos0j / g);
                 scaleh = R(cush);
             }
         }
@@ -914,10 +1006,10 @@ UWSHCU_HD void Column<R>::run() const {
             const R dbot = R(in(a.ps0, kinv - 1) - in(a.p0, kinv - 1));
             fluxbelowinv(R(qtsrc), Er(A_QT0, kinv), Er(A_QT0, kinv + 1) + Er(A_SSQT0, kinv + 1) * dtop,
                          Er(A_QT0, kinv - 1) + Er(A_SSQT0, kinv - 1) * dbot,
-                         [&](int k, R x) { out(a.qtflx, k) = (double)x; });
+                         [&](int k, R x) { out(a.qtflx, k) = D(x); });
             fluxbelowinv(R(thlsrc), Er(A_THL0, kinv), Er(A_THL0, kinv + 1) + Er(A_SSTHL0, kinv + 1) * dtop,
                          Er(A_THL0, kinv - 1) + Er(A_SSTHL0, kinv - 1) * dbot,
-                         [&](int k, R x) { out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * x); });
+                         [&](int k, R x) { out(a.slflx, k) = D(cp * Er(A_EXNS0, k) * x); });
             fluxbelowinv(R(usrc), R(in(a.u0, kinv)), R(in(a.u0, kinv + 1)) + Er(A_SSU0, kinv + 1) * dtop,
                          R(in(a.u0, kinv - 1)) + Er(A_SSU0, kinv - 1) * dbot,
                          [&](int k, R x) { W(W_UFLX, k) = x; });
@@ -935,8 +1027,8 @@ UWSHCU_HD void Column<R>::run() const {
             for (int k = kinv; k <= krel - 1; ++k) {
                 const int kp1 = k + 1;
                 const R d = R(in(a.ps0, k) - in(a.p0, kp1));
-                out(a.qtflx, k) = (double)(cbmf * (R(qtsrc) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
-                out(a.slflx, k) = (double)(cbmf * (R(thlsrc) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)) * cp * Er(A_EXNS0, k));
+                out(a.qtflx, k) = D(cbmf * (R(qtsrc) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
+                out(a.slflx, k) = D(cbmf * (R(thlsrc) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)) * cp * Er(A_EXNS0, k));
                 uplus = uplus + PGFc * Er(A_SSU0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
                 vplus = vplus + PGFc * Er(A_SSV0, k) * R(in(a.ps0, k) - in(a.ps0, k - 1));
                 W(W_UFLX, k) = cbmf * (R(usrc) + uplus - (R(in(a.u0, kp1)) + Er(A_SSU0, kp1) * d));
@@ -949,8 +1041,8 @@ UWSHCU_HD void Column<R>::run() const {
             const int kp1 = k + 1;
             const R d = R(in(a.ps0, k) - in(a.p0, kp1));
             const R umfk = R(out(a.umf, k));
-            out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * umfk * (W(W_THLU, k) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)));
-            out(a.qtflx, k) = (double)(umfk * (W(W_QTU, k) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
+            out(a.slflx, k) = D(cp * Er(A_EXNS0, k) * umfk * (W(W_THLU, k) - (Er(A_THL0, kp1) + Er(A_SSTHL0, kp1) * d)));
+            out(a.qtflx, k) = D(umfk * (W(W_QTU, k) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
             W(W_UFLX, k) = umfk * (W(W_UU, k) - (R(in(a.u0, kp1)) + Er(A_SSU0, kp1) * d));
             W(W_VFLX, k) = umfk * (W(W_VU, k) - (R(in(a.v0, kp1)) + Er(A_SSV0, kp1) * d));
             for (int m = 1; m <= ncnst; ++m)
@@ -959,8 +1051,8 @@ UWSHCU_HD void Column<R>::run() const {
         for (int k = kbup; k <= kpen - 1; ++k) {
             const R d = R(in(a.ps0, k) - in(a.p0, k));
             const R emfk = W(W_EMF, k);
-            out(a.slflx, k) = (double)(cp * Er(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (Er(A_THL0, k) + Er(A_SSTHL0, k) * d)));
-            out(a.qtflx, k) = (double)(emfk * (W(W_QTUE, k) - (Er(A_QT0, k) + Er(A_SSQT0, k) * d)));
+            out(a.slflx, k) = D(cp * Er(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (Er(A_THL0, k) + Er(A_SSTHL0, k) * d)));
+            out(a.qtflx, k) = D(emfk * (W(W_QTUE, k) - (Er(A_QT0, k) + Er(A_SSQT0, k) * d)));
             W(W_UFLX, k) = emfk * (W(W_UUE, k) - (R(in(a.u0, k)) + Er(A_SSU0, k) * d));
             W(W_VFLX, k) = emfk * (W(W_VUE, k) - (R(in(a.v0, k)) + Er(A_SSV0, k) * d));
             for (int m = 1; m <= ncnst; ++m)
@@ -1021,8 +1113,8 @@ UWSHCU_HD void Column<R>::run() const {
         // ---- S16 tendencies (uwshcu.F90:3488-3794) ----
         for (int k = 1; k <= kpen; ++k) {
             const R gdp = g / R(in(a.dp0, k));
-            out(a.uten, k) = (double)((W(W_UFLX, k - 1) - W(W_UFLX, k)) * g / R(in(a.dp0, k)));
-            out(a.vten, k) = (double)((W(W_VFLX, k - 1) - W(W_VFLX, k)) * g / R(in(a.dp0, k)));
+            out(a.uten, k) = D((W(W_UFLX, k - 1) - W(W_UFLX, k)) * g / R(in(a.dp0, k)));
+            out(a.vten, k) = D((W(W_VFLX, k - 1) - W(W_VFLX, k)) * g / R(in(a.dp0, k)));
             (void)gdp;
         }
         R rliq = R(0), rainflx = R(0), snowflx = R(0);
@@ -1040,8 +1132,8 @@ UWSHCU_HD void Column<R>::run() const {
                 W(W_DWTEN, k) = dwk;
                 W(W_DITEN, k) = dik;
                 const R qrk = frc_rasn * dwk, qsk = frc_rasn * dik;
-                out(a.qrten, k) = (double)qrk;
-                out(a.qsten, k) = (double)qsk;
+                out(a.qrten, k) = D(qrk);
+                out(a.qsten, k) = D(qsk);
                 rainflx = rainflx + qrk * dp0k / g;
                 snowflx = snowflx + qsk * dp0k / g;
                 R slten = (R(out(a.slflx, km1)) - R(out(a.slflx, k))) * g / dp0k;
@@ -1116,7 +1208,7 @@ UWSHCU_HD void Column<R>::run() const {
                     qc_im = qc_im - g * umf_k * qi0k / dps;
                     nc_lm = nc_lm - g * umf_k * trz(ixl, k) / dps;
                     nc_im = nc_im - g * umf_k * trz(ixi, k) / dps;
-                    double ql_e, qi_e;
+                    D ql_e, qi_e;
                     if (T.conden(in(a.p0, k), W(W_THLUE, k), W(W_QTUE, k), th, qvj, ql_e, qi_e)) { exit_code = EX_KBUPCONDEN; goto L333; }
                     const R nl_e = (ql_e > 0.0 && has_nl) ? WT(2, ixl, k) : R(0);
                     const R ni_e = (qi_e > 0.0 && has_ni) ? WT(2, ixi, k) : R(0);
@@ -1128,16 +1220,16 @@ UWSHCU_HD void Column<R>::run() const {
                 }
                 const R qltenk = qc_l + frc_rasn * dwk + (fmax(R(0), ql0k + (qc_lm + W(W_QLSINK, k)) * dt) - ql0k) / dt;
                 const R qitenk = qc_i + frc_rasn * dik + (fmax(R(0), qi0k + (qc_im + W(W_QISINK, k)) * dt) - qi0k) / dt;
-                if (has_nl) trout(ixl, k) = (double)fmax(nc_lm + W(W_NLSINK, k), -trz(ixl, k) / dt);
-                if (has_ni) trout(ixi, k) = (double)fmax(nc_im + W(W_NISINK, k), -trz(ixi, k) / dt);
-                out(a.qlten, k) = (double)qltenk;
-                out(a.qiten, k) = (double)qitenk;
-                out(a.qvten, k) = (double)(qtten - qltenk - qitenk);
-                out(a.sten, k) = (double)(slten + xlv * qltenk + xls * qitenk);
+                if (has_nl) trout(ixl, k) = D(fmax(nc_lm + W(W_NLSINK, k), -trz(ixl, k) / dt));
+                if (has_ni) trout(ixi, k) = D(fmax(nc_im + W(W_NISINK, k), -trz(ixi, k) / dt));
+                out(a.qlten, k) = D(qltenk);
+                out(a.qiten, k) = D(qitenk);
+                out(a.qvten, k) = D(qtten - qltenk - qitenk);
+                out(a.sten, k) = D(slten + xlv * qltenk + xls * qitenk);
                 W(W_QCL, k) = qc_l;
                 W(W_QCI, k) = qc_i;
                 const R qck = qc_l + qc_i;
-                out(a.qc, k) = (double)qck;
+                out(a.qc, k) = D(qck);
                 rliq = rliq + qck * dp0k / g / R(1000);
             }
         }
@@ -1152,7 +1244,7 @@ UWSHCU_HD void Column<R>::run() const {
                 const R dp0k = R(in(a.dp0, k));
                 const R t0k = R(in(a.t0, k)), qv0k = R(in(a.qv0, k));
                 R snowmlt = (t0k > R(273.16)) ? fmax(R(0), flxsnow_k * g / dp0k) : R(0);
-                double es, qs;
+                D es, qs;
                 T.qsat_es_qs(in(a.t0, k), in(a.p0, k), es, qs);
                 const R subsat = fmax((R(1) - qv0k / R(qs)), R(0));
                 R evprain = kevp * subsat * sqrt(flxrain_k + snowmlt * dp0k / g);
@@ -1170,7 +1262,7 @@ UWSHCU_HD void Column<R>::run() const {
                     evprain = tmp1;
                     evpsnow = tmp2;
                 }
-                out(a.evapc, k) = (double)(evprain + evpsnow);
+                out(a.evapc, k) = D(evprain + evpsnow);
                 evpint_rain = evpint_rain + evprain * dp0k / g;
                 evpint_snow = evpint_snow + evpsnow * dp0k / g;
                 const R qrk = R(out(a.qrten, k)), qsk = R(out(a.qsten, k));
@@ -1183,16 +1275,16 @@ UWSHCU_HD void Column<R>::run() const {
                 flxrain_k = flxrain_km1;
                 flxsnow_k = flxsnow_km1;
                 if (final_pass) {
-                    out(a.flxprc1, k - 1) = (double)(flxrain_km1 + flxsnow_km1);
-                    out(a.flxsnow1, k - 1) = (double)flxsnow_km1;
+                    out(a.flxprc1, k - 1) = D(flxrain_km1 + flxsnow_km1);
+                    out(a.flxsnow1, k - 1) = D(flxsnow_km1);
                 }
                 const R qlt = R(out(a.qlten, k)) - qrk;
                 const R qit = R(out(a.qiten, k)) - qsk;
                 const R qvt = R(out(a.qvten, k)) + evprain + evpsnow;
-                out(a.qlten, k) = (double)qlt;
-                out(a.qiten, k) = (double)qit;
-                out(a.qvten, k) = (double)qvt;
-                out(a.sten, k) = (double)(R(out(a.sten, k)) - xlv * evprain - xls * evpsnow - (xls - xlv) * snowmlt);
+                out(a.qlten, k) = D(qlt);
+                out(a.qiten, k) = D(qit);
+                out(a.qvten, k) = D(qvt);
+                out(a.sten, k) = D(R(out(a.sten, k)) - xlv * evprain - xls * evpsnow - (xls - xlv) * snowmlt);
             }
         }
         const R precip = (flxrain_k + flxsnow_k) / R(1000);
@@ -1200,29 +1292,29 @@ UWSHCU_HD void Column<R>::run() const {
 
         // ---- S18 reserved condensate (uwshcu.F90:3966-3981) ----
         for (int k = 1; k <= kpen; ++k) {
-            out(a.qlten, k) = (double)(R(out(a.qlten, k)) - W(W_QCL, k));
-            out(a.qiten, k) = (double)(R(out(a.qiten, k)) - W(W_QCI, k));
-            out(a.sten, k) = (double)(R(out(a.sten, k)) - (xls - xlv) * W(W_QCI, k));
+            out(a.qlten, k) = D(R(out(a.qlten, k)) - W(W_QCL, k));
+            out(a.qiten, k) = D(R(out(a.qiten, k)) - W(W_QCI, k));
+            out(a.sten, k) = D(R(out(a.sten, k)) - (xls - xlv) * W(W_QCI, k));
         }
 
         // ---- S19 positive moisture, FP64 (uwshcu.F90:3995-4001, 5096-5162) ----
         {
             const double qvmin = C.qmin[1], qlmin = C.qmin[2], qimin = C.qmin[3];
             const double xlvd = C.xlv, xlsd = C.xls;
-            double dqv = 0.0;
-            double carry_qv = 0.0, carry_ten = 0.0;   // adjustments of qv(k-1) and qvten(k-1) from level k
+            D dqv = 0.0;
+            D carry_qv = 0.0, carry_ten = 0.0;   // adjustments of qv(k-1) and qvten(k-1) from level k
             bool have_carry = false;
             for (int k = mkx; k >= 1; --k) {
-                double qvten = out(a.qvten, k), qlten = out(a.qlten, k), qiten = out(a.qiten, k), sten = out(a.sten, k);
-                double qv = in(a.qv0, k) + qvten * dtd;
-                double ql = in(a.ql0, k) + qlten * dtd;
-                double qi = in(a.qi0, k) + qiten * dtd;
+                D qvten = out(a.qvten, k), qlten = out(a.qlten, k), qiten = out(a.qiten, k), sten = out(a.sten, k);
+                D qv = in(a.qv0, k) + qvten * dtd;
+                D ql = in(a.ql0, k) + qlten * dtd;
+                D qi = in(a.qi0, k) + qiten * dtd;
                 if (have_carry) {
                     qv = qv - carry_qv;
                     qvten = qvten - carry_ten;
                 }
-                const double dql = fmax(0.0, 1.0 * qlmin - ql);
-                const double dqi = fmax(0.0, 1.0 * qimin - qi);
+                const D dql = fmax(0.0, 1.0 * qlmin - ql);
+                const D dqi = fmax(0.0, 1.0 * qimin - qi);
                 qlten = qlten + dql / dtd;
                 qiten = qiten + dqi / dtd;
                 qvten = qvten - (dql + dqi) / dtd;
@@ -1234,7 +1326,7 @@ UWSHCU_HD void Column<R>::run() const {
                 qvten = qvten + dqv / dtd;
                 qv = qv + dqv;
                 if (k != 1) {
-                    const double dpk = in(a.dp0, k), dpkm1 = in(a.dp0, k - 1);
+                    const D dpk = in(a.dp0, k), dpkm1 = in(a.dp0, k - 1);
                     carry_qv = dqv * dpk / dpkm1;
                     carry_ten = dqv * dpk / dpkm1 / dtd;
                     have_carry = true;
@@ -1247,17 +1339,17 @@ UWSHCU_HD void Column<R>::run() const {
                 E(E_QVSTAR, k) = qv;
             }
             if (dqv > 1.e-20) {
-                double sum = 0.0;
+                D sum = 0.0;
                 for (int k = 1; k <= mkx; ++k) {
-                    const double qv = E(E_QVSTAR, k);
+                    const D qv = E(E_QVSTAR, k);
                     if (qv > 2.0 * qvmin) sum = sum + qv * in(a.dp0, k);
                 }
-                const double aa = dqv * in(a.dp0, 1) / fmax(1.e-20, sum);
+                const D aa = dqv * in(a.dp0, 1) / fmax(1.e-20, sum);
                 if (aa < 0.5) {
                     for (int k = 1; k <= mkx; ++k) {
-                        const double qv = E(E_QVSTAR, k);
+                        const D qv = E(E_QVSTAR, k);
                         if (qv > 2.0 * qvmin) {
-                            const double dum = aa * qv;
+                            const D dum = aa * qv;
                             out(a.qvten, k) = out(a.qvten, k) - dum / dtd;
                         }
                     }
@@ -1285,7 +1377,7 @@ UWSHCU_HD void Column<R>::run() const {
                 W(W_TRFU, k - 1) = fmax(R(0), -dum);
             }
             for (int k = 1; k <= mkx; ++k) {
-                trout(m, k) = (double)((WT(3, m, k - 1) - WT(3, m, k) + W(W_TRFD, k - 1) - W(W_TRFD, k) +
+                trout(m, k) = D((WT(3, m, k - 1) - WT(3, m, k) + W(W_TRFD, k - 1) - W(W_TRFD, k) +
                                         W(W_TRFU, k - 1) - W(W_TRFU, k)) * g / pdelx(k));
             }
         }
@@ -1310,10 +1402,10 @@ UWSHCU_HD void Column<R>::run() const {
                         qiuk = R(0.5) * (qiubelow + criqc * qir / (qlr + qir));
                     }
                 }
-                out(a.qcu, k) = (double)qcuk;
-                out(a.qlu, k) = (double)qluk;
-                out(a.qiu, k) = (double)qiuk;
-                out(a.cufrc, k) = (double)cufrck;
+                out(a.qcu, k) = D(qcuk);
+                out(a.qlu, k) = D(qluk);
+                out(a.qiu, k) = D(qiuk);
+                out(a.cufrc, k) = D(cufrck);
                 qcubelow = qlr + qir;
                 qlubelow = qlr;
                 qiubelow = qir;
@@ -1321,22 +1413,22 @@ UWSHCU_HD void Column<R>::run() const {
         }
 
         // column scalars of this pass
-        a.precip[i] = (double)precip;
-        a.snow[i] = (double)snow;
-        a.cbmf[i] = (double)cbmf;
-        a.rliq[i] = (double)rliq;
-        a.cnt[i] = (double)(mkx + 1 - kpen);
-        a.cnb[i] = (double)(mkx + 1 - (krel - 1));
+        cout(a.precip) = D(precip);
+        cout(a.snow) = D(snow);
+        cout(a.cbmf) = D(cbmf);
+        cout(a.rliq) = D(rliq);
+        cout(a.cnt) = D(mkx + 1 - kpen);
+        cout(a.cnb) = D(mkx + 1 - (krel - 1));
 
         // ---- S22 provisional state and environment set B (uwshcu.F90:4169-4329) ----
         if (iter == 1) {
             cush_s = cush;
             for (int k = 1; k <= mkx; ++k) {
-                const double qv0s = in(a.qv0, k) + out(a.qvten, k) * dtd;
-                const double ql0s = in(a.ql0, k) + out(a.qlten, k) * dtd;
-                const double qi0s = in(a.qi0, k) + out(a.qiten, k) * dtd;
-                const double t0s = in(a.t0, k) + out(a.sten, k) * dtd / C.cp;
-                const double qt0 = (qv0s + ql0s + qi0s);
+                const D qv0s = in(a.qv0, k) + out(a.qvten, k) * dtd;
+                const D ql0s = in(a.ql0, k) + out(a.qlten, k) * dtd;
+                const D qi0s = in(a.qi0, k) + out(a.qiten, k) * dtd;
+                const D t0s = in(a.t0, k) + out(a.sten, k) * dtd / C.cp;
+                const D qt0 = (qv0s + ql0s + qi0s);
                 E(B_QT0, k) = qt0;
                 E(B_THL0, k) = (t0s - C.xlv * ql0s / C.cp - C.xls * qi0s / C.cp) / E(A_EXN0, k);
             }
@@ -1348,7 +1440,7 @@ UWSHCU_HD void Column<R>::run() const {
     }
 
     // successful implicit-CIN pass
-    a.cush[i] = cush;
+    cout(a.cush) = cush;
     path = 2;
     goto Ldone;
 

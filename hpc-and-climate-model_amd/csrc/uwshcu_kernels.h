@@ -3,7 +3,7 @@
 #define UWSHCU_KERNELS_H
 
 #include <hip/hip_runtime.h>
-#include "uwshcu_column.h"
+#include "uwshcu_adjoint.h"
 
 namespace uwshcu {
 
@@ -17,6 +17,15 @@ hipError_t launch_forward_mixed(const FwdArgs& args, int n, const uwshcu_consts*
 // set or whose outputs fail the FP64 conservation check at relative tolerance tol.
 hipError_t launch_check_columns(const FwdArgs& args, const uwshcu_consts* dconsts, double tol, int* list, int* count,
                                 hipStream_t stream);
+// Adjoint over columns [args.col0, args.col0 + n): record each column's FP64 forward
+// pass on its tape (slot j at tape_base + j*cap; args.ad points at the AD output
+// values and the tape headers), write the forward outputs, then sweep the tapes from
+// the cotangents to the gradients.  adj_base holds n*cap doubles of scratch.
+hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                          TapeNode* tape_base, long long cap, int* node_out, double* adj_base,
+                          const AdCotangents& cot, const AdGradients& g, hipStream_t stream);
+// sum[p] = sum over columns c < ncol of par[p*ldc + c], p = 0..2, in FP64.
+hipError_t launch_param_sum(const double* par, long long ldc, int ncol, double* sum, hipStream_t stream);
 // Set the two status words to zero (a fill kernel, so that graph capture records no memset node).
 hipError_t launch_clear_status(int* err, hipStream_t stream);
 

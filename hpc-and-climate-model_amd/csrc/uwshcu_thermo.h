@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include "uwshcu_types.h"
 #include "uwshcu_math.h"
+#include "uwshcu_ad.h"
 
 // Every primitive is callable from host code too: the test suite builds the column
 // code for the CPU to check its logic against the oracle before it runs on a GPU.
@@ -25,23 +26,24 @@ UWSHCU_HD inline void record_error(int* err, int code, int col) {
     if (__atomic_compare_exchange_n(&err[0], &expected, code, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) err[1] = col;
 }
 
-struct Thermo {
+template <class D>
+struct ThermoT {
     const uwshcu_consts* c;   // kernel-argument copy
-    const double* estbl;      // LDS copy of the table
+    const double* estbl;      // LDS copy of the table (FP64 values)
 
-    UWSHCU_HD inline double exnf(double p) const { return upow(p / c->p00, c->rovcp); }  // uwshcu.F90:50-54
+    UWSHCU_HD inline D exnf(D p) const { return upow(p / c->p00, c->rovcp); }  // uwshcu.F90:50-54
 
     // wv_saturation.F90:64-80
-    UWSHCU_HD inline double estblf(double td) const {
-        double e = fmax(fmin(td, c->tmax), c->tmin);
-        double d = e - c->tmin;
-        int i = (int)d;                        // int(e-tmin)+1, 1-based -> i (0-based)
-        double ai = (double)i;                 // aint(e-tmin) for e >= tmin
+    UWSHCU_HD inline D estblf(D td) const {
+        D e = fmax(fmin(td, c->tmax), c->tmin);
+        D d = e - c->tmin;
+        int i = (int)val(d);                   // int(e-tmin)+1, 1-based -> i (0-based)
+        const double ai = (double)i;           // aint(e-tmin) for e >= tmin
         return (c->tmin + ai - e + 1.0) * estbl[i] - (c->tmin + ai - e) * estbl[i + 1];
     }
 
     // vqsatd with len=1, es and qs only (wv_saturation.F90:530-546)
-    UWSHCU_HD inline void qsat_es_qs(double t, double p, double& es, double& qs) const {
+    UWSHCU_HD inline void qsat_es_qs(D t, D p, D& es, D& qs) const {
         const double omeps = 1.0 - c->epsqs;
         es = estblf(t);
         qs = c->epsqs * es / (p - omeps * es);
@@ -50,10 +52,10 @@ struct Thermo {
     }
 
     // vqsatd with len=1 including gam = (L/cp) dqs/dT (wv_saturation.F90:482-609)
-    UWSHCU_HD inline void qsat_full(double t, double p, double& es, double& qs, double& gam) const {
+    UWSHCU_HD inline void qsat_full(D t, D p, D& es, D& qs, D& gam) const {
         const double omeps = 1.0 - c->epsqs;
         qsat_es_qs(t, p, es, qs);
-        double hlatsb, hlatvp, hltalt, desdt;
+        D hlatsb, hlatvp, hltalt, desdt;
         if (!c->icephs || c->ttrice == 0.0) {
             hlatvp = c->hlatv - 2369.0 * (t - c->tmelt);
             hlatsb = c->icephs ? c->hlatv + c->hlatf : c->hlatv;
@@ -61,13 +63,13 @@ struct Thermo {
             desdt = hltalt * es / (c->rgasv * t * t);
         } else {
             const double trinv = 1.0 / c->ttrice;
-            const double tc = t - c->tmelt;
+            const D tc = t - c->tmelt;
             const bool lflg = (tc >= -c->ttrice && tc < 0.0);
-            const double weight = fmin(-tc * trinv, 1.0);
+            const D weight = fmin(-tc * trinv, 1.0);
             hlatsb = c->hlatv + weight * c->hlatf;
             hlatvp = c->hlatv - 2369.0 * tc;
             hltalt = (t < c->tmelt) ? hlatsb : hlatvp;
-            double tterm = 0.0;
+            D tterm = 0.0;
             if (lflg) tterm = c->pcf[0] + tc * (c->pcf[1] + tc * (c->pcf[2] + tc * (c->pcf[3] + tc * c->pcf[4])));
             desdt = hltalt * es / (c->rgasv * t * t) + tterm * trinv;
         }
@@ -76,15 +78,15 @@ struct Thermo {
     }
 
     // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
-    UWSHCU_HD UWSHCU_OUTLINE int conden(double p, double thl, double qt, double& th, double& qv, double& ql,
-                                          double& qi) const {
-        const double ex = exnf(p);
-        const double tc = thl * ex;
-        const double nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
-        const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
-        double temps = tc, es, qs;
+    UWSHCU_HD UWSHCU_OUTLINE int conden(D p, D thl, D qt, D& th, D& qv, D& ql,
+                                          D& qi) const {
+        const D ex = exnf(p);
+        const D tc = thl * ex;
+        const D nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
+        const D leff = (1.0 - nu) * c->xlv + nu * c->xls;
+        D temps = tc, es, qs;
         qsat_es_qs(temps, p, es, qs);
-        double rvls = qs;
+        D rvls = qs;
         if (qs >= qt) {
             qv = qt;
             ql = 0.0;
@@ -98,7 +100,7 @@ struct Thermo {
             qsat_es_qs(temps, p, es, qs);
             rvls = qs;
         }
-        const double qc = fmax(qt - qs, 0.0);
+        const D qc = fmax(qt - qs, 0.0);
         qv = qt - qc;
         ql = qc * (1.0 - nu);
         qi = nu * qc;
@@ -107,29 +109,29 @@ struct Thermo {
     }
 
     // uwshcu.F90:4863-4929
-    UWSHCU_HD double qsinvert(double qt, double thl, double psfc) const {
+    UWSHCU_HD D qsinvert(D qt, D thl, D psfc) const {
         const double psmin = 100.0 * 100.0, dpsmax = 1.0;
-        double es, qs, gam;
-        const double Ti = thl * upow(psfc / c->p00, c->rovcp);
+        D es, qs, gam;
+        const D Ti = thl * upow(psfc / c->p00, c->rovcp);
         qsat_full(Ti, psfc, es, qs, gam);
-        const double rhi = qt / qs;
+        const D rhi = qt / qs;
         if (rhi <= 0.01) return psmin;
-        const double TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - ulog(rhi) / 2840.0);
-        const double PiLCL = TLCL / thl;
-        double ps = c->p00 * upow(PiLCL, 1.0 / c->rovcp);
+        const D TLCL = 55.0 + 1.0 / (1.0 / (Ti - 55.0) - ulog(rhi) / 2840.0);
+        const D PiLCL = TLCL / thl;
+        D ps = c->p00 * upow(PiLCL, 1.0 / c->rovcp);
         for (int it = 0; it < 10; ++it) {
-            const double Pis = upow(ps / c->p00, c->rovcp);
-            const double Ts = thl * Pis;
+            const D Pis = upow(ps / c->p00, c->rovcp);
+            const D Ts = thl * Pis;
             qsat_full(Ts, ps, es, qs, gam);
-            const double err = qt - qs;
-            const double nu = fmax(fmin((268.0 - Ts) / 20.0, 1.0), 0.0);
-            const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
-            const double dlnqsdT = gam * (c->cp / leff) / qs;
-            const double dTdPis = thl;
-            const double dPisdps = c->rovcp * Pis / ps;
-            const double dlnqsdps = -1.0 / (ps - (1.0 - c->ep2) * es);
-            const double derrdps = -qs * (dlnqsdT * dTdPis * dPisdps + dlnqsdps);
-            const double dps = -err / derrdps;
+            const D err = qt - qs;
+            const D nu = fmax(fmin((268.0 - Ts) / 20.0, 1.0), 0.0);
+            const D leff = (1.0 - nu) * c->xlv + nu * c->xls;
+            const D dlnqsdT = gam * (c->cp / leff) / qs;
+            const D dTdPis = thl;
+            const D dPisdps = c->rovcp * Pis / ps;
+            const D dlnqsdps = -1.0 / (ps - (1.0 - c->ep2) * es);
+            const D derrdps = -qs * (dlnqsdT * dTdPis * dPisdps + dlnqsdps);
+            const D dps = -err / derrdps;
             ps = ps + dps;
             if (ps < 0.0) return psmin;
             if (fabs(dps) <= dpsmax) return ps;
@@ -138,32 +140,33 @@ struct Thermo {
     }
 
     // uwshcu.F90:4704-4715
-    UWSHCU_HD inline double single_cin(double pbot, double thv0bot, double ptop, double thv0top,
-                                                 double thvubot, double thvutop) const {
+    UWSHCU_HD inline D single_cin(D pbot, D thv0bot, D ptop, D thv0top,
+                                                 D thvubot, D thvutop) const {
         return ((1.0 - thvubot / thv0bot) + (1.0 - thvutop / thv0top)) * (pbot - ptop) /
                (pbot / (c->r * thv0bot * exnf(pbot)) + ptop / (c->r * thv0top * exnf(ptop)));
     }
 
     // uwshcu.F90:4672-4702
-    UWSHCU_HD inline void getbuoy(double pbot, double thv0bot, double ptop, double thv0top,
-                                            double thvubot, double thvutop, double& plfc, double& cin) const {
+    UWSHCU_HD inline void getbuoy(D pbot, D thv0bot, D ptop, D thv0top,
+                                            D thvubot, D thvutop, D& plfc, D& cin) const {
         if (thvubot > thv0bot && thvutop > thv0top) {
             plfc = pbot;
             return;
         }
-        const double den = (pbot / (c->r * thv0bot * exnf(pbot)) + ptop / (c->r * thv0top * exnf(ptop)));
+        const D den = (pbot / (c->r * thv0bot * exnf(pbot)) + ptop / (c->r * thv0top * exnf(ptop)));
         if (thvubot <= thv0bot && thvutop <= thv0top) {
             cin = cin - ((thvubot / thv0bot - 1.0) + (thvutop / thv0top - 1.0)) * (pbot - ptop) / den;
         } else if (thvubot > thv0bot && thvutop <= thv0top) {
-            const double frc = (thvutop / thv0top - 1.0) / ((thvutop / thv0top - 1.0) - (thvubot / thv0bot - 1.0));
+            const D frc = (thvutop / thv0top - 1.0) / ((thvutop / thv0top - 1.0) - (thvubot / thv0bot - 1.0));
             cin = cin - (thvutop / thv0top - 1.0) * ((ptop + frc * (pbot - ptop)) - ptop) / den;
         } else {
-            const double frc = (thvubot / thv0bot - 1.0) / ((thvubot / thv0bot - 1.0) - (thvutop / thv0top - 1.0));
+            const D frc = (thvubot / thv0bot - 1.0) / ((thvubot / thv0bot - 1.0) - (thvutop / thv0top - 1.0));
             plfc = pbot - frc * (pbot - ptop);
             cin = cin - (thvubot / thv0bot - 1.0) * (pbot - plfc) / den;
         }
     }
 };
+using Thermo = ThermoT<double>;
 
 // uwshcu.F90:4787-4829
 template <class R>
@@ -187,10 +190,11 @@ UWSHCU_HD inline void roots(R a, R b, R c, R& r1, R& r2, int& status) {
 }
 
 // uwshcu.F90:4931-4950
-UWSHCU_HD inline double compute_alpha(double del_CIN, double ke) {
-    double x0 = 0.0;
+template <class D>
+UWSHCU_HD inline D compute_alpha(D del_CIN, D ke) {
+    D x0 = 0.0;
     for (int it = 0; it < 10; ++it) {
-        const double e = uexp(-x0 * ke * del_CIN);
+        const D e = uexp(-x0 * ke * del_CIN);
         x0 = x0 - (e - x0) / (-ke * del_CIN * e - 1.0);
     }
     return x0;
@@ -241,31 +245,31 @@ UWSHCU_HD inline R compute_ppen(R wtwb, R D, R bogbot, R bogtop, R rho0j, R dpen
 // early exit fires only when no unconverged point remains), so each (column,level)
 // is solved independently.  Returns the doit code: 0 not converged, 1 skipped,
 // 2 converged, 4 too cold; enerr is the relative enthalpy error of converged points.
-UWSHCU_HD inline int findsp_point(const Thermo& T, double q, double t, double p, double& qsp_out,
-                                            double& enerr) {
+template <class D>
+UWSHCU_HD inline int findsp_point(const ThermoT<D>& T, D q, D t, D p, D& qsp_out, D& enerr) {
     const uwshcu_consts* c = T.c;
     const double omeps = 1.0 - c->epsqs, trinv = 1.0 / c->ttrice;
     const double a1 = 7.5 * ulog(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;
     const double dttol = 1.e-4, dqtol = 1.e-4, tt0 = 273.15;
-    const double tlim = fmin(fmax(t, 173.0), 373.0);
-    double es = T.estblf(tlim);
-    const double denom = p - omeps * es;
-    double qs = c->epsqs * es / denom;
-    double tsp, qsp, enin, enout = 1.0;
+    const D tlim = fmin(fmax(t, 173.0), 373.0);
+    D es = T.estblf(tlim);
+    const D denom = p - omeps * es;
+    D qs = c->epsqs * es / denom;
+    D tsp, qsp, enin, enout = 1.0;
     int doit = 0;
     if (p > 5.0 * es && qs > 0.0 && qs < 0.5) {
         qs = fmin(c->epsqs * es / denom, 1.0);
-        const double tc = tlim - tt0;
-        const double weight = fmin(-tc * trinv, 1.0);
-        const double hlatsb = c->hlatv + weight * c->hlatf;
-        const double hlatvp = c->hlatv - 2369.0 * tc;
-        const double hltalt = (tlim < tt0) ? hlatsb : hlatvp;
+        const D tc = tlim - tt0;
+        const D weight = fmin(-tc * trinv, 1.0);
+        const D hlatsb = c->hlatv + weight * c->hlatf;
+        const D hlatvp = c->hlatv - 2369.0 * tc;
+        const D hltalt = (tlim < tt0) ? hlatsb : hlatvp;
         enin = c->cpws * tlim + hltalt * q;
-        const double tmp = q - qs;
-        const double c1 = hltalt * c3;
-        const double c2 = (tlim + 36.0) * (tlim + 36.0);
-        const double r1b = c2 / (c2 + c1 * qs);
-        const double qvd = r1b * tmp;
+        const D tmp = q - qs;
+        const D c1 = hltalt * c3;
+        const D c2 = (tlim + 36.0) * (tlim + 36.0);
+        const D r1b = c2 / (c2 + c1 * qs);
+        const D qvd = r1b * tmp;
         tsp = tlim + ((hltalt / c->cpws) * qvd);
         es = T.estblf(tsp);
         qsp = fmin(c->epsqs * es / (p - omeps * es), 1.0);
@@ -278,24 +282,24 @@ UWSHCU_HD inline int findsp_point(const Thermo& T, double q, double t, double p,
     for (int l = 0; l < 8 && doit == 0; ++l) {
         es = T.estblf(tsp);
         qs = fmin(c->epsqs * es / (p - omeps * es), 1.0);
-        const double tc = tsp - tt0;
+        const D tc = tsp - tt0;
         const bool lflg = (tc >= -c->ttrice && tc < 0.0);
-        const double weight = fmin(-tc * trinv, 1.0);
-        const double hlatsb = c->hlatv + weight * c->hlatf;
-        const double hlatvp = c->hlatv - 2369.0 * tc;
-        const double hltalt = (tsp < tt0) ? hlatsb : hlatvp;
-        double tterm = 0.0;
+        const D weight = fmin(-tc * trinv, 1.0);
+        const D hlatsb = c->hlatv + weight * c->hlatf;
+        const D hlatvp = c->hlatv - 2369.0 * tc;
+        const D hltalt = (tsp < tt0) ? hlatsb : hlatvp;
+        D tterm = 0.0;
         if (lflg) tterm = c->pcf[0] + tc * (c->pcf[1] + tc * (c->pcf[2] + tc * (c->pcf[3] + tc * c->pcf[4])));
-        const double desdt = hltalt * es / (c->rgasv * tsp * tsp) + tterm * trinv;
-        const double dqsdt = (c->epsqs + omeps * qs) / (p - omeps * es) * desdt;
-        const double g = enin - (c->cpws * tsp + hltalt * qsp);
-        const double dgdt = -(c->cpws + hltalt * dqsdt);
-        const double t1 = tsp - g / dgdt;
-        const double dtr = fabs(t1 - tsp) / t1;
+        const D desdt = hltalt * es / (c->rgasv * tsp * tsp) + tterm * trinv;
+        const D dqsdt = (c->epsqs + omeps * qs) / (p - omeps * es) * desdt;
+        const D g = enin - (c->cpws * tsp + hltalt * qsp);
+        const D dgdt = -(c->cpws + hltalt * dqsdt);
+        const D t1 = tsp - g / dgdt;
+        const D dtr = fabs(t1 - tsp) / t1;
         tsp = fmax(t1, c->tmin);
         es = T.estblf(tsp);
-        const double q1 = fmin(c->epsqs * es / (p - omeps * es), 1.0);
-        const double dq = fabs(q1 - qsp) / fmax(q1, 1.e-12);
+        const D q1 = fmin(c->epsqs * es / (p - omeps * es), 1.0);
+        const D dq = fabs(q1 - qsp) / fmax(q1, 1.e-12);
         qsp = q1;
         if (dtr < dttol && dq < dqtol) doit = 2;
         enout = c->cpws * tsp + hltalt * qsp;

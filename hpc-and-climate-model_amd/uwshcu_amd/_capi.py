@@ -28,6 +28,7 @@ ERRORS = {
     -11: "workspace allocation failed",
     -12: "HIP runtime error",
     -13: "no usable GPU",
+    -14: "adjoint tape capacity exceeded",
 }
 
 
@@ -46,6 +47,21 @@ class Params(C.Structure):
     _fields_ = [("rpen", C.c_double), ("rle", C.c_double), ("rkm", C.c_double)]
 
 
+# Output fields in the order of uwshcu_cotangents (uwshcu_types.h).
+COT_FIELDS = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu", "qvten", "qlten", "qiten", "sten", "uten", "vten",
+              "qrten", "qsten", "evapc", "cufrc", "qcu", "qlu", "qiu", "qc", "trten", "precip", "snow", "cbmf", "rliq",
+              "cnt", "cnb", "cush")
+GRAD_FIELDS = ("t0", "qv0", "u0", "v0", "tke", "pblh", "par_col", "par_sum")
+
+
+class Cotangents(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in COT_FIELDS]
+
+
+class Gradients(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in GRAD_FIELDS]
+
+
 # Argument names of the forward entry points, after (ctx, precision, par, pcols, ncol,
 # pver, ncnst, dt), in the order of compute_uwshcu_inv (uwshcu.F90:251-265).
 INPUT_NAMES = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke",
@@ -60,7 +76,7 @@ SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params
            "uwshcu_ctx_destroy", "uwshcu_ctx_set_workspace_limit", "uwshcu_workspace_bytes_per_column",
            "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
-           "uwshcu_mixed_recomputed")
+           "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity")
 
 _lib = None
 
@@ -103,6 +119,9 @@ def lib():
     L.uwshcu_dropin_last_status.argtypes = [ip]
     L.uwshcu_ctx_set_mixed_tolerance.argtypes = [vp, dp]
     L.uwshcu_mixed_recomputed.argtypes = [vp, vp, ip]
+    L.uwshcu_adjoint_device.argtypes = [vp, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp] + \
+        [vp] * len(ARRAY_ARGS) + [vp, C.POINTER(Cotangents), C.POINTER(Gradients), vp]
+    L.uwshcu_ctx_set_tape_capacity.argtypes = [vp, C.c_longlong]
     _lib = L
     return L
 

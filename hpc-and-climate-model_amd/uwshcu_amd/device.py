@@ -127,6 +127,53 @@ class Context:
         if rc:
             raise UwshcuError(rc, bad)
 
+    def adjoint_into(self, inp: dict, out: dict, cush: torch.Tensor, cot: dict, grads: dict, *,
+                     ncol: int | None = None, dt: float = 1800.0, params: _capi.Params | None = None,
+                     stream=None) -> None:
+        """Launch the adjoint (asynchronous): recompute the FP64 forward on tapes,
+        write ``out``/``cush`` like :meth:`forward_into`, and write the gradients named
+        in ``grads`` (t0, qv0, u0, v0, tke, pblh, par_col [3, pcols], par_sum [3])
+        from the output cotangents ``cot`` (missing = zero)."""
+        pver, pcols = inp["p0"].shape
+        ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
+        ncol = pcols if ncol is None else ncol
+        par = params or _capi.default_params()
+        for k in _capi.INPUT_NAMES + ("dpdry0",):
+            t = inp[k]
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+        ptrs = {k: inp[k].data_ptr() for k in _capi.INPUT_NAMES + ("dpdry0",)}
+        ptrs["cush"] = cush.data_ptr()
+        for k in _capi.OUTPUT_NAMES + ("wu",):
+            ptrs[k] = out[k].data_ptr()
+        keep = []
+
+        def dptr(t):
+            if t is None:
+                return None
+            t = t.to(dtype=torch.float64).contiguous()
+            keep.append(t)
+            return t.data_ptr()
+        ct = _capi.Cotangents(*[dptr(cot.get(k)) for k in _capi.COT_FIELDS])
+        gr = _capi.Gradients(*[(grads[k].data_ptr() if grads.get(k) is not None else None)
+                               for k in _capi.GRAD_FIELDS])
+        diag = out.get("diag")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        args = [self._h, C.byref(par), pcols, ncol, pver, ncnst, C.c_double(dt)]
+        args += [C.c_void_p(ptrs[k]) for k in _capi.ARRAY_ARGS]
+        args += [C.c_void_p(diag.data_ptr() if diag is not None else 0), C.byref(ct), C.byref(gr), C.c_void_p(sp)]
+        rc = _capi.lib().uwshcu_adjoint_device(*args)
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_adjoint_device")
+        self._keep = keep   # cotangent copies stay alive until the next call
+
+    def set_tape_capacity(self, nodes_per_column: int) -> None:
+        rc = _capi.lib().uwshcu_ctx_set_tape_capacity(self._h, C.c_longlong(int(nodes_per_column)))
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_ctx_set_tape_capacity")
+
     def compute_uwshcu_inv(self, inp: dict, *, dt: float = 1800.0, precision: int = FP64,
                            params: _capi.Params | None = None, diag: bool = False, ncol: int | None = None) -> dict:
         """Run the chunk and return the outputs (synchronous; raises on a fatal condition)."""

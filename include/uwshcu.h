@@ -92,6 +92,32 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
     double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
 
+/* Adjoint (vector-Jacobian product) of the FP64 forward on device arrays.  No
+ * counterpart in the reference, which has no derivative code.  Re-runs the FP64
+ * forward of every column while recording it on a per-column tape (the branch
+ * pattern is the forward's own), writes the forward outputs exactly as
+ * uwshcu_forward_device(UWSHCU_FP64) does, then sweeps each tape backwards from the
+ * cotangents `cot` and writes the requested gradients `grad` (uwshcu_types.h).
+ * Tapes take uwshcu_ctx_set_tape_capacity nodes per column (default 8192*(pver+1),
+ * 32 bytes each with the sweep's scratch) and columns run in batches that fit the
+ * workspace limit; a column whose tape overflows sets UWSHCU_ERR_TAPE in
+ * uwshcu_status.  Stream-ordered like uwshcu_forward_device. */
+int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag,
+    const uwshcu_cotangents *cot, const uwshcu_gradients *grad, void *stream);
+
+/* Tape capacity of the adjoint in nodes per column (0 restores the default). */
+int uwshcu_ctx_set_tape_capacity(uwshcu_ctx *ctx, long long nodes_per_column);
+
 /* Synchronise `stream` and return the status of the forward calls issued on it
  * since the previous uwshcu_status call (0 or a negative fatal code); *bad_col
  * (nullable) receives the offending column. */

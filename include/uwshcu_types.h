@@ -43,6 +43,24 @@ enum {
     UWSHCU_MIXED_UNCHECKED = 2   // the FP32 pass alone (no check, no recompute)
 };
 
+// Cotangents of the outputs for the adjoint: device pointers in the caller's layout
+// (the same layout as the outputs); NULL means zero.
+typedef struct uwshcu_cotangents {
+    const double *umf, *slflx, *qtflx, *flxprc1, *flxsnow1, *wu;
+    const double *qvten, *qlten, *qiten, *sten, *uten, *vten, *qrten, *qsten, *evapc;
+    const double *cufrc, *qcu, *qlu, *qiu, *qc;
+    const double *trten;
+    const double *precip, *snow, *cbmf, *rliq, *cnt, *cnb, *cush;
+} uwshcu_cotangents;
+
+// Gradients written by the adjoint (device pointers, caller's layout; NULL = not
+// wanted): t0, qv0, u0, v0 (pcols, pver); tke (pcols, pver+1); pblh (pcols);
+// par_col (3, pcols): each column's contribution to d/d(rpen, rle, rkm);
+// par_sum [3]: those contributions summed over the columns in FP64.
+typedef struct uwshcu_gradients {
+    double *t0, *qv0, *u0, *v0, *tke, *pblh, *par_col, *par_sum;
+} uwshcu_gradients;
+
 // Return codes (negative values are conditions the reference treats as fatal).
 enum {
     UWSHCU_OK                 = 0,
@@ -52,7 +70,8 @@ enum {
     UWSHCU_ERR_ARG            = -10,  // invalid sizes or null pointers
     UWSHCU_ERR_NOMEM          = -11,  // workspace allocation failed
     UWSHCU_ERR_HIP            = -12,  // a HIP runtime call failed
-    UWSHCU_ERR_NO_DEVICE      = -13   // no usable GPU
+    UWSHCU_ERR_NO_DEVICE      = -13,  // no usable GPU
+    UWSHCU_ERR_TAPE           = -14   // adjoint: a column's tape exceeded its capacity
 };
 
 #ifdef __cplusplus

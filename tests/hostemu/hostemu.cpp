@@ -6,6 +6,7 @@
 #include <cstring>
 #include <vector>
 
+#include "uwshcu_adjoint.h"
 #include "uwshcu_check.h"
 
 using namespace uwshcu;
@@ -65,5 +66,65 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     }
     if (nfix) *nfix = nlist;
     if (bad_col) *bad_col = err[0] ? err[1] : -1;
+    return err[0];
+}
+
+// Adjoint on the host: records each column's FP64 forward pass on a tape
+// (Column<AD, AD>), writes the forward outputs, seeds the outputs' adjoints with
+// cot[] (ADF_TR layer/interface fields, trten, ADC_COUNT column scalars; null =
+// zero) and writes the input gradients grad[] = {t0, qv0, u0, v0, tke, pblh, par}.
+// tape_len[i] receives the tape length of column i (-1 if it overflowed cap).
+extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, const uwshcu_params* par, int pcols,
+                               int ncol, int pver, int ncnst, double dt, const double* ps0, const double* zs0,
+                               const double* p0, const double* z0, const double* dp0, const double* u0,
+                               const double* v0, const double* qv0, const double* ql0, const double* qi0,
+                               const double* t0, const double* s0, const double* tr0, const double* tke,
+                               const double* pblh, double* cush, double* const* outs, const double* dpdry0,
+                               double* wu, const double* const* cot, double* const* grad, int cap, int* tape_len,
+                               int* diag) {
+    FwdArgs a{};
+    a.ncol = ncol; a.mkx = pver; a.ncnst = ncnst; a.ldc = pcols; a.col0 = 0; a.dt = dt; a.par = *par;
+    a.ps0 = ps0; a.zs0 = zs0; a.p0 = p0; a.z0 = z0; a.dp0 = dp0; a.dpdry0 = dpdry0; a.u0 = u0; a.v0 = v0;
+    a.qv0 = qv0; a.ql0 = ql0; a.qi0 = qi0; a.t0 = t0; a.s0 = s0; a.tr0 = tr0; a.tke = tke; a.pblh = pblh;
+    a.cush = cush;
+    double** o = const_cast<double**>(outs);
+    a.umf = o[0]; a.slflx = o[1]; a.qtflx = o[2]; a.flxprc1 = o[3]; a.flxsnow1 = o[4]; a.qvten = o[5];
+    a.qlten = o[6]; a.qiten = o[7]; a.sten = o[8]; a.uten = o[9]; a.vten = o[10]; a.trten = o[11]; a.qrten = o[12];
+    a.qsten = o[13]; a.precip = o[14]; a.snow = o[15]; a.evapc = o[16]; a.cufrc = o[17]; a.qcu = o[18];
+    a.qlu = o[19]; a.qiu = o[20]; a.cbmf = o[21]; a.qc = o[22]; a.rliq = o[23]; a.cnt = o[24]; a.cnb = o[25];
+    a.wu = wu;
+    a.diag = diag;
+    int err[2] = {0, -1};
+    a.err = err;
+    a.wld = ncol;
+    const int KS = pver + 1;
+    std::vector<char> ws(ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD)) * (size_t)ncol + 64);
+    a.ws = ws.data();
+    std::vector<AD> adout((size_t)ad_fields(ncnst) * KS * ncol);
+    std::vector<TapeCtx> tapes(ncol);
+    std::vector<int> nodes((size_t)ad_fields(ncnst) * KS * ncol, -1);
+    a.ad.out = adout.data();
+    a.ad.tapes = tapes.data();
+    AdCotangents ct{};
+    for (int f = 0; f < ADF_TR; ++f) ct.f[f] = cot[f];
+    ct.trten = cot[ADF_TR];
+    for (int q = 0; q < ADC_COUNT; ++q) ct.scal[q] = cot[ADF_TR + 1 + q];
+    AdGradients g{grad[0], grad[1], grad[2], grad[3], grad[4], grad[5], grad[6]};
+    const ThermoT<AD> T{c, estbl};
+    const int nin = ad_inputs(pver);
+#pragma omp parallel
+    {
+        std::vector<TapeNode> buf((size_t)cap);
+        std::vector<double> adj((size_t)cap);
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 0; i < ncol; ++i) {
+            ad_begin(tapes[i], buf.data(), cap, nin);
+            const Column<AD, AD> col(a, T, i, i);
+            col.run();
+            col.ad_export(nodes.data());
+            ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
+            tape_len[i] = tapes[i].overflow ? -1 : tapes[i].n;
+        }
+    }
     return err[0];
 }

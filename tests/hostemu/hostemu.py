@@ -93,3 +93,56 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
         stats["recomputed"] = nfix.value
     out["cush"] = cush
     return out, np.ascontiguousarray(diag.T), rc, bad.value
+
+
+AD_FIELDS = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu", "qvten", "qlten", "qiten", "sten", "uten", "vten",
+             "qrten", "qsten", "evapc", "cufrc", "qcu", "qlu", "qiu", "qc")
+AD_SCALARS = ("precip", "snow", "cbmf", "rliq", "cnt", "cnb", "cush")
+GRAD_NAMES = ("t0", "qv0", "u0", "v0", "tke", "pblh", "par")
+OUT_ORDER = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "qiten", "sten", "uten", "vten",
+             "trten", "qrten", "qsten", "precip", "snow", "evapc", "cufrc", "qcu", "qlu", "qiu", "cbmf", "qc", "rliq",
+             "cnt", "cnb")
+
+
+def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False):
+    """Host adjoint (tape) of the FP64 column code.
+
+    cot: dict output-name -> cotangent array (same shape as the output; missing = 0).
+    Returns (outputs, grads, tape_len, diag): grads has t0, qv0, u0, v0 ([pver, ncol]),
+    tke ([pver+1, ncol]), pblh ([ncol]) and par ([3, ncol]: per-column d/d(rpen, rle, rkm)).
+    """
+    c = consts or _capi.default_consts()
+    tab = (C.c_double * 250)(*_capi.build_estbl(c))
+    par = params or _capi.default_params()
+    f = chunk.fields
+    n, L, ncnst = chunk.ncol, chunk.pver, chunk.ncnst
+    out = {k: np.zeros((L + 1, n)) for k in ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu")}
+    for k in ("qvten", "qlten", "qiten", "sten", "uten", "vten", "qrten", "qsten", "evapc", "cufrc", "qcu",
+              "qlu", "qiu", "qc"):
+        out[k] = np.zeros((L, n))
+    out["trten"] = np.zeros((max(ncnst, 1), L, n))
+    for k in ("precip", "snow", "cbmf", "rliq", "cnt", "cnb"):
+        out[k] = np.zeros(n)
+    cush = f["cush"].copy()
+    ins = [np.ascontiguousarray(f[k]) for k in ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0",
+                                                 "t0", "s0", "tr0", "tke", "pblh")]
+    outs = (C.c_void_p * len(OUT_ORDER))(*[out[k].ctypes.data for k in OUT_ORDER])
+    cots = [np.ascontiguousarray(cot[k], dtype=np.float64) if k in cot else None
+            for k in AD_FIELDS + ("trten",) + AD_SCALARS]
+    cotp = (C.c_void_p * len(cots))(*[(x.ctypes.data if x is not None else None) for x in cots])
+    grads = {"t0": np.zeros((L, n)), "qv0": np.zeros((L, n)), "u0": np.zeros((L, n)), "v0": np.zeros((L, n)),
+             "tke": np.zeros((L + 1, n)), "pblh": np.zeros(n), "par": np.zeros((3, n))}
+    gp = (C.c_void_p * 7)(*[grads[k].ctypes.data for k in GRAD_NAMES])
+    tlen = np.zeros(n, dtype=np.int32)
+    diag = np.zeros((NDIAG, n), dtype=np.int32)
+    args = [C.byref(c), tab, C.byref(par), C.c_int(n), C.c_int(n), C.c_int(L), C.c_int(ncnst),
+            C.c_double(chunk.dt)] + [_p(a) for a in ins] + [_p(cush), outs, _p(np.ascontiguousarray(f["dpdry0"])),
+                                                           _p(out["wu"]), cotp, gp, C.c_int(cap), _p(tlen), _p(diag)]
+    L_ = lib(checked)
+    L_.hostemu_adjoint.restype = C.c_int
+    rc = L_.hostemu_adjoint(*args)
+    assert rc == 0, rc
+    out["cush"] = cush
+    if ncnst == 0:
+        out["trten"] = out["trten"][:0]
+    return out, grads, tlen, np.ascontiguousarray(diag.T)

@@ -1,0 +1,105 @@
+"""GPU adjoint: the device tape and sweep against the same code on the host, the
+torch.autograd binding, batching and the tape-capacity error."""
+import numpy as np
+import pytest
+
+import hostemu
+import oracle
+from uwshcu_amd.generator import make_chunk
+
+pytestmark = pytest.mark.gpu
+
+COT = ("sten", "qvten", "umf", "precip", "qlten")
+
+
+def _cot(ch, seed):
+    rng = np.random.default_rng(seed)
+    ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    return {k: rng.normal(0.0, 1.0, np.shape(ref[k])) for k in COT}
+
+
+def _gpu_adjoint(ch, cot, workspace_limit=None, cap=None):
+    import torch
+    from uwshcu_amd import device
+    ctx = device.Context(0, workspace_limit=workspace_limit) if workspace_limit else device.default_context()
+    if cap:
+        ctx.set_tape_capacity(cap)
+    inp = device.chunk_to_device(ch, "cuda:0")
+    pver, n = ch.pver, ch.ncol
+    out = ctx.alloc_outputs(n, pver, ch.ncnst, diag=True)
+    cush = inp["cush"].clone()
+    dev = torch.device("cuda:0")
+    grads = {k: torch.zeros((pver, n), dtype=torch.float64, device=dev) for k in ("t0", "qv0", "u0", "v0")}
+    grads["tke"] = torch.zeros((pver + 1, n), dtype=torch.float64, device=dev)
+    grads["pblh"] = torch.ones(n, dtype=torch.float64, device=dev)
+    grads["par_col"] = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    grads["par_sum"] = torch.zeros(3, dtype=torch.float64, device=dev)
+    cd = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in cot.items()}
+    ctx.adjoint_into(inp, out, cush, cd, grads, dt=ch.dt)
+    rc, bad = ctx.status()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["cush"] = cush.cpu().numpy()
+    g = {k: v.cpu().numpy() for k, v in grads.items()}
+    if workspace_limit or cap:
+        ctx.close()
+    return rc, res, g
+
+
+@pytest.mark.parametrize("ncol,pver,ncnst", [(1024, 35, 3), (128, 72, 10)])
+def test_device_adjoint_equals_host_adjoint(ncol, pver, ncnst):
+    ch = make_chunk(ncol, pver, ncnst, seed=60 + pver, ptop=100.0 if pver > 60 else 200.0)
+    cot = _cot(ch, 1)
+    print(f"\n[adjoint {ncol}x{pver}x{ncnst}] device run", flush=True)
+    rc, out, g = _gpu_adjoint(ch, cot)
+    assert rc == 0
+    ref, rdiag, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    np.testing.assert_array_equal(out["diag"].T, rdiag)
+    for k in ref:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    _, hg, tlen, _ = hostemu.adjoint(ch, cot)
+    print(f"[adjoint {ncol}x{pver}x{ncnst}] tape nodes max {tlen.max()} median {int(np.median(tlen))}", flush=True)
+    for k in ("t0", "qv0", "u0", "v0", "tke", "pblh"):
+        np.testing.assert_array_equal(g[k], hg[k], err_msg=k)
+    np.testing.assert_array_equal(g["par_col"], hg["par"])
+    np.testing.assert_allclose(g["par_sum"], hg["par"].sum(axis=1), rtol=1e-12, atol=0)
+
+
+def test_batched_adjoint_gives_the_same_gradients():
+    from uwshcu_amd import _capi  # noqa: F401
+    ch = make_chunk(700, 35, 3, seed=62)
+    cot = _cot(ch, 2)
+    _, _, g1 = _gpu_adjoint(ch, cot)
+    _, _, g2 = _gpu_adjoint(ch, cot, workspace_limit=3 << 30)     # about 300 columns per batch
+    for k in g1:
+        np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
+
+
+def test_tape_capacity_overflow_is_an_error_not_a_fault():
+    ch = make_chunk(256, 35, 3, seed=63)
+    rc, _, _ = _gpu_adjoint(ch, _cot(ch, 3), cap=4000)
+    assert rc == -14
+
+
+def test_autograd_backward_matches_the_adjoint():
+    import torch
+    from uwshcu_amd import device
+    from uwshcu_amd.autograd import uwshcu
+    ch = make_chunk(512, 35, 3, seed=64)
+    dev = torch.device("cuda:0")
+    inp = device.chunk_to_device(ch, "cuda:0")
+    for k in ("t0", "qv0", "u0", "v0", "tke", "pblh"):
+        inp[k].requires_grad_(True)
+    params = torch.tensor([10.0, 0.1, 14.0], dtype=torch.float64, requires_grad=True)
+    rng = np.random.default_rng(4)
+    tau_t = torch.from_numpy(rng.normal(0, 1e-5, (35, 512))).to(dev)
+    tau_q = torch.from_numpy(rng.normal(0, 1e-8, (35, 512))).to(dev)
+    out = uwshcu(inp, params, dt=ch.dt)
+    loss = ((out["sten"] / 1.00464e3 - tau_t) ** 2).sum() + ((out["qvten"] - tau_q) ** 2).sum()
+    loss.backward()
+    cot = {"sten": (2 * (out["sten"] / 1.00464e3 - tau_t) / 1.00464e3).detach().cpu().numpy(),
+           "qvten": (2 * (out["qvten"] - tau_q)).detach().cpu().numpy()}
+    _, hg, _, _ = hostemu.adjoint(ch, cot)
+    for k in ("t0", "qv0", "u0", "v0", "tke"):
+        np.testing.assert_allclose(inp[k].grad.cpu().numpy(), hg[k], rtol=1e-12, atol=1e-300, err_msg=k)
+    np.testing.assert_allclose(params.grad.numpy(), hg["par"].sum(axis=1), rtol=1e-10)
+    assert not inp["pblh"].grad.any()
