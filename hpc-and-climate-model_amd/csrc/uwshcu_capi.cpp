@@ -38,7 +38,7 @@ struct uwshcu_ctx {
     int* d_count = nullptr;
     size_t mixed_cols = 0;
     double mixed_tol = 1.e-4;
-    // adjoint: tape nodes per column (0 = 8192 per level interface) and a per-column
+    // adjoint: tape nodes per column (0 = (2048 + 512 ncnst) per level interface) and a per-column
     // parameter-gradient buffer for callers that only want the sums
     long long tape_cap = 0;
     double* d_parcol = nullptr;
@@ -220,7 +220,8 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     if (ncol == 0) return UWSHCU_OK;
     if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
     const int KS = pver + 1;
-    const long long cap = ctx->tape_cap > 0 ? ctx->tape_cap : 8192LL * KS;
+    // default capacity: ~2.5x the longest tapes measured (48k nodes at 35x3, 230k at 72x10)
+    const long long cap = ctx->tape_cap > 0 ? ctx->tape_cap : (2048LL + 512LL * ncnst) * KS;
     // per-column pieces: AD workspace, AD outputs, output nodes, tape header, tape, adjoints
     const size_t b_ws = ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD));
     const size_t b_out = (size_t)ad_fields(ncnst) * KS * sizeof(AD);

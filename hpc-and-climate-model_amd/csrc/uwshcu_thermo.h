@@ -80,6 +80,12 @@ struct ThermoT {
     // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
     UWSHCU_HD UWSHCU_OUTLINE int conden(D p, D thl, D qt, D& th, D& qv, D& ql,
                                           D& qi) const {
+#ifndef UWSHCU_AD_TAPE_CONDEN
+        if constexpr (is_ad<D>::value) {
+            int id;
+            if (conden_ift(p, thl, qt, th, qv, ql, qi, id)) return id;
+        }
+#endif
         const D ex = exnf(p);
         const D tc = thl * ex;
         const D nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
@@ -106,6 +112,91 @@ struct ThermoT {
         qi = nu * qc;
         th = temps / ex;
         return (fabs((temps - (leff / c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
+    }
+
+    // Adjoint form of conden (D = AD): the values are the FP64 routine's, computed by
+    // the same operations; the derivatives of the saturated branch come from the
+    // implicit function theorem at the Newton point T of
+    //     F(T) = (tc - T) cp/leff + qt - qs(T, p) = 0,   tc = thl (p/p00)^(R/cp),
+    // which the ten Newton steps reach to round-off.  Each output is then one or two
+    // tape nodes instead of the ~300 the ten steps would record.  When the last step
+    // still moved T by more than 1e-12 relative (the iteration hopping between two
+    // segments of the saturation table), returns false and the caller records the
+    // ten steps themselves.
+    UWSHCU_HD bool conden_ift(const D& p, const D& thl, const D& qt, D& th, D& qv, D& ql, D& qi, int& id) const {
+        const double pv = val(p), thlv = val(thl), qtv = val(qt);
+        const double ex = exnf_d(pv);
+        const double tc = thlv * ex;
+        const double nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
+        const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
+        double temps = tc, es, qs;
+        qsat_d(temps, pv, es, qs);
+        double rvls = qs;
+        if (qs >= qtv) {
+            // unsaturated: qv = qt, no condensate, th = (thl ex)/ex
+            const D exa = exnf(p);
+            const D tca = thl * exa;
+            qv = qt;
+            ql = 0.0;
+            qi = 0.0;
+            th = tca / exa;
+            id = 0;
+            return true;
+        }
+        double tprev = temps;
+        for (int it = 0; it < 10; ++it) {
+            tprev = temps;
+            temps = temps + ((tc - temps) * c->cp / leff + qtv - rvls) /
+                            (c->cp / leff + c->ep2 * leff * rvls / c->r / temps / temps);
+            qsat_d(temps, pv, es, qs);
+            rvls = qs;
+        }
+        if (fabs(temps - tprev) > 1.e-12 * temps) return false;
+        const double qc = fmax(qtv - qs, 0.0);
+        const double qvv = qtv - qc, qlv = qc * (1.0 - nu), qiv = nu * qc, thv = temps / ex;
+        id = (fabs((temps - (leff / c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
+        // partial derivatives
+        const double ex_p = c->rovcp * ex / pv;
+        const double tc_thl = ex, tc_p = thlv * ex_p;
+        const double r20 = (268.0 - tc) / 20.0;
+        const double nu_tc = (r20 >= 0.0 && r20 <= 1.0) ? -1.0 / 20.0 : 0.0;
+        const double leff_tc = (c->xls - c->xlv) * nu_tc;
+        const double omeps = 1.0 - c->epsqs, den = pv - omeps * es;
+        double qs_es = c->epsqs * pv / (den * den), qs_p = -c->epsqs * es / (den * den);
+        if (c->epsqs * es / den > 1.0) qs_es = qs_p = 0.0;          // qs clipped at 1
+        const double e = fmax(fmin(temps, c->tmax), c->tmin);
+        const int i = (int)(e - c->tmin);
+        const double es_t = (temps >= c->tmin && temps < c->tmax) ? estbl[i + 1] - estbl[i] : 0.0;
+        const double qs_t = qs_es * es_t;
+        const double f_t = -c->cp / leff - qs_t;
+        const double f_tc = c->cp / leff - (tc - temps) * c->cp * leff_tc / (leff * leff);
+        const double t_thl = -f_tc * tc_thl / f_t, t_qt = -1.0 / f_t, t_p = -(f_tc * tc_p - qs_p) / f_t;
+        const double s_thl = qs_t * t_thl, s_qt = qs_t * t_qt, s_p = qs_t * t_p + qs_p;
+        const bool wet = qtv - qs >= 0.0;
+        const double qc_thl = wet ? -s_thl : 0.0, qc_qt = wet ? 1.0 - s_qt : 0.0, qc_p = wet ? -s_p : 0.0;
+        const double nu_thl = nu_tc * tc_thl, nu_p = nu_tc * tc_p;
+        auto emit = [&](double v, double d_thl, double d_qt, double d_p) {
+            const D part = ad_node(0.0, thl, d_thl, qt, d_qt);
+            return ad_node(v, part, 1.0, p, d_p);
+        };
+        qv = emit(qvv, -qc_thl, 1.0 - qc_qt, -qc_p);
+        ql = emit(qlv, qc_thl * (1.0 - nu) - qc * nu_thl, qc_qt * (1.0 - nu), qc_p * (1.0 - nu) - qc * nu_p);
+        qi = emit(qiv, nu * qc_thl + qc * nu_thl, nu * qc_qt, nu * qc_p + qc * nu_p);
+        th = emit(thv, t_thl / ex, t_qt / ex, t_p / ex - temps * ex_p / (ex * ex));
+        return true;
+    }
+    // FP64 pieces of the above (the D = double forms of exnf and qsat_es_qs)
+    UWSHCU_HD double exnf_d(double p) const { return upow(p / c->p00, c->rovcp); }
+    UWSHCU_HD void qsat_d(double t, double p, double& es, double& qs) const {
+        const double omeps = 1.0 - c->epsqs;
+        const double e = fmax(fmin(t, c->tmax), c->tmin);
+        const double d = e - c->tmin;
+        const int i = (int)d;
+        const double ai = (double)i;
+        es = (c->tmin + ai - e + 1.0) * estbl[i] - (c->tmin + ai - e) * estbl[i + 1];
+        qs = c->epsqs * es / (p - omeps * es);
+        qs = fmin(1.0, qs);
+        if (qs < 0.0) { qs = 1.0; es = p; }
     }
 
     // uwshcu.F90:4863-4929

@@ -98,8 +98,8 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
  * pattern is the forward's own), writes the forward outputs exactly as
  * uwshcu_forward_device(UWSHCU_FP64) does, then sweeps each tape backwards from the
  * cotangents `cot` and writes the requested gradients `grad` (uwshcu_types.h).
- * Tapes take uwshcu_ctx_set_tape_capacity nodes per column (default 8192*(pver+1),
- * 32 bytes each with the sweep's scratch) and columns run in batches that fit the
+ * Tapes take uwshcu_ctx_set_tape_capacity nodes per column (default
+ * (2048 + 512*ncnst)*(pver+1), 32 bytes each with the sweep's scratch) and columns run in batches that fit the
  * workspace limit; a column whose tape overflows sets UWSHCU_ERR_TAPE in
  * uwshcu_status.  Stream-ordered like uwshcu_forward_device. */
 int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,

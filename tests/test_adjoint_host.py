@@ -101,6 +101,13 @@ def test_adjoint_column_code_stays_in_bounds(ncol, pver, ncnst):
         np.testing.assert_array_equal(a[1][k], b[1][k], err_msg=k)
 
 
+@pytest.mark.parametrize("ncol,pver,ncnst", [(512, 35, 3), (128, 72, 10)])
+def test_tapes_fit_the_default_capacity_with_margin(ncol, pver, ncnst):
+    ch = make_chunk(ncol, pver, ncnst, seed=46, ptop=100.0 if pver > 60 else 200.0)
+    _, _, tlen, _ = hostemu.adjoint(ch, {})
+    assert (tlen > 0).all() and tlen.max() * 2 <= (2048 + 512 * ncnst) * (pver + 1)
+
+
 def test_tape_overflow_is_reported():
     ch = make_chunk(64, 35, 3, seed=45)
     _, _, tlen, diag = hostemu.adjoint(ch, {}, cap=2000)
