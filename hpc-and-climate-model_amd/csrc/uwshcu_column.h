@@ -89,6 +89,7 @@ struct FwdArgs {
     const int* colcount;
     const double* cush_in;   // optional: read the input cush here instead of from cush
     int* amb_flag;           // optional [ldc]: 1 where an FP32 branch decision was within rounding
+    int env_ready;           // 1: setup_level1/2 already filled the set-A environment and the wet bulb
     AdArgs ad;               // adjoint kernel only
 };
 
@@ -237,10 +238,15 @@ struct Column {
     // (A_THL0 or B_THL0), from thl0/qt0 already stored in that set.  Returns false on a
     // conden failure.
     UWSHCU_HD bool interface_env(int eb) const {
+        for (int k = 1; k <= mkx; ++k)
+            if (!interface_env_level(eb, k)) return false;
+        return true;
+    }
+    UWSHCU_HD bool interface_env_level(int eb, int k) const {
         const int thl = eb, qt = eb + 1, ssthl = eb + 2, ssqt = eb + 3;
         const int thvb = eb == A_THL0 ? A_THV0BOT : B_THV0BOT, thvt = thvb + 1;
         const int thvlb = eb == A_THL0 ? A_THVL0BOT : B_THVL0BOT, thvlt = thvlb + 1;
-        for (int k = 1; k <= mkx; ++k) {
+        {
             D th, qv, ql, qi;
             const D pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
             const D thl0bot = E(thl, k) + E(ssthl, k) * (pb - pm);
@@ -330,6 +336,51 @@ struct Column {
 
     UWSHCU_HD void run() const;
 
+    // Level-parallel set-up (one thread per column and level), the part of S0 and of
+    // the wet bulb that needs no sequential sweep.  Stage 1, level k = 0..mkx: Exner
+    // functions, thl0, qt0, findsp.  Stage 2, level k = 1..mkx, after stage 1 of the
+    // neighbouring levels: slopes, the set-A interface environment (a failed
+    // saturation adjustment marks A_THV0BOT(k) = -1) and the layer-mean saturation
+    // cache.  Every value equals the one run() computes itself without env_ready.
+    UWSHCU_HD void setup_level1(int k) const {
+        E(A_EXNS0, k) = upow(in(a.ps0, k) / C.p00, C.rovcp);
+        E(E_THV0MID, k) = 0.0;
+        if (k < 1) return;
+        D qsp, enerr;
+        const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
+        if (doit == 0) record_error(a.err, UWSHCU_ERR_FINDSP_CONV, i);
+        else if (doit == 2 && enerr > 1.e-4) record_error(a.err, UWSHCU_ERR_FINDSP_ENTH, i);
+        W(W_QW0, k) = R(qsp);
+        E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
+        const D qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
+        const D qt0 = (qv0 + ql0 + qi0);
+        E(A_QT0, k) = qt0;
+        E(A_THL0, k) = (t0 - C.xlv * ql0 / C.cp - C.xls * qi0 / C.cp) / E(A_EXN0, k);
+    }
+    UWSHCU_HD void setup_level2(int k) const {
+        // minmod slope at level k as slope() computes it (the top level copies mkx-1)
+        const int j = k < mkx ? k : mkx - 1;
+        auto mm = [&](auto field) {
+            const auto above = (field(j + 1) - field(j)) / (in(a.p0, j + 1) - in(a.p0, j));
+            const auto below = (j == 1) ? above : (field(j) - field(j - 1)) / (in(a.p0, j) - in(a.p0, j - 1));
+            using V = decltype(above + below);
+            return (above > V(0)) ? fmax(V(0), fmin(above, below)) : fmin(V(0), fmax(above, below));
+        };
+        E(A_SSTHL0, k) = mm([&](int q) { return E(A_THL0, q); });
+        E(A_SSQT0, k) = mm([&](int q) { return E(A_QT0, q); });
+        E(A_SSU0, k) = mm([&](int q) { return in(a.u0, q); });
+        E(A_SSV0, k) = mm([&](int q) { return in(a.v0, q); });
+        for (int m = 1; m <= ncnst; ++m) {
+            const auto above = (R(trin(m, j + 1)) - R(trin(m, j))) / (R(in(a.p0, j + 1)) - R(in(a.p0, j)));
+            const auto below = (j == 1) ? above
+                                        : (R(trin(m, j)) - R(trin(m, j - 1))) / (R(in(a.p0, j)) - R(in(a.p0, j - 1)));
+            WT(0, m, k) = (above > R(0)) ? fmax(R(0), fmin(above, below)) : fmin(R(0), fmax(above, below));
+        }
+        if (!interface_env_level(A_THL0, k)) E(A_THV0BOT, k) = -1.0;
+        D thv0j, qs0;
+        env_mid(k, thv0j, qs0);
+    }
+
     // Adjoint kernel, after run(): copy the recorded outputs' values to the caller's
     // arrays and their tape nodes to nodes[(f * (mkx+1) + k) * wld + slot].
     UWSHCU_HD void ad_export(int* nodes) const {
@@ -386,6 +437,7 @@ UWSHCU_HD void Column<R, D>::run() const {
     D th, qvj, qlj, qij;
 
     // ---- wet bulb for the evaporation limiter (uwshcu.F90:1216; findsp 5164-5454) ----
+    if (!a.env_ready)
     for (int k = 1; k <= mkx; ++k) {
         D qsp, enerr;
         const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
@@ -395,6 +447,7 @@ UWSHCU_HD void Column<R, D>::run() const {
     }
 
     // ---- S0 environment, set A (uwshcu.F90:1255-1299) ----
+    if (!a.env_ready) {
     for (int k = 1; k <= mkx; ++k) E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
     for (int k = 0; k <= mkx; ++k) E(A_EXNS0, k) = upow(in(a.ps0, k) / C.p00, C.rovcp);
     for (int k = 1; k <= mkx; ++k) {
@@ -409,6 +462,7 @@ UWSHCU_HD void Column<R, D>::run() const {
     slope([&](int k) { return in(a.v0, k); }, A_SSV0);
     for (int k = 0; k <= mkx; ++k) E(E_THV0MID, k) = 0.0;
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
+    }
 
     zero_pass_state();
     for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
@@ -430,7 +484,13 @@ UWSHCU_HD void Column<R, D>::run() const {
     R trsrc_tmp = 0;
     (void)trsrc_tmp;
 
-    if (!interface_env(A_THL0)) { exit_code = EX_CONDEN; goto L333; }
+    if (a.env_ready) {
+        for (int k = 1; k <= mkx; ++k)
+            if (E(A_THV0BOT, k) < 0.0) { exit_code = EX_CONDEN; goto L333; }
+    } else if (!interface_env(A_THL0)) {
+        exit_code = EX_CONDEN;
+        goto L333;
+    }
 
     for (int iter = 1; iter <= 2; ++iter) {
         exit_iter = iter;

@@ -34,5 +34,45 @@ void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ const
     c.run();
 }
 
+// Level-parallel set-up: thread t of the launch handles workspace slot t % n and
+// level t / n (+1 for stage 2), so that a wavefront covers consecutive columns.
+template <class R, int STAGE>
+__global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                                                    const double* __restrict__ estbl) {
+    __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
+    int nvalid = n;
+    if (args.colmap) {
+        const int left = *args.colcount - args.col0;
+        nvalid = left < n ? left : n;
+    }
+    for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
+    __syncthreads();
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = (int)(t % n);
+    const int k = (int)(t / n) + (STAGE == 2 ? 1 : 0);
+    if (k > args.mkx || j >= nvalid) return;
+    const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
+    const Thermo T{consts, s_estbl};
+    const Column<R> c(args, T, col, j);
+    if (STAGE == 1) c.setup_level1(k);
+    else c.setup_level2(k);
+}
+
+// The forward pass of columns [col0, col0 + n) (or of the mapped columns): the two
+// level-parallel set-up stages, then the column kernel.
+template <class R>
+hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_consts* dconsts, const double* destbl,
+                                  hipStream_t stream) {
+    FwdArgs args = args0;
+    args.env_ready = 1;
+    const long long t1 = (long long)n * (args.mkx + 1), t2 = (long long)n * args.mkx;
+    hipLaunchKernelGGL((setup_kernel<R, 1>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, stream, args, n,
+                       dconsts, destbl);
+    hipLaunchKernelGGL((setup_kernel<R, 2>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
+                       dconsts, destbl);
+    hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+    return hipGetLastError();
+}
+
 }  // namespace uwshcu
 #endif

@@ -9,10 +9,7 @@ namespace uwshcu {
 
 hipError_t launch_forward_fp64(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                hipStream_t stream) {
-    const int block = 128;
-    const int grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(forward_kernel<double>, dim3(grid), dim3(block), 0, stream, args, n, dconsts, destbl);
-    return hipGetLastError();
+    return launch_column_pipeline<double>(args, n, dconsts, destbl, stream);
 }
 
 __global__ void clear_status_kernel(int* err) {

@@ -11,10 +11,7 @@ namespace uwshcu {
 
 hipError_t launch_forward_mixed(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                 hipStream_t stream) {
-    const int block = 128;
-    const int grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(forward_kernel<float>, dim3(grid), dim3(block), 0, stream, args, n, dconsts, destbl);
-    return hipGetLastError();
+    return launch_column_pipeline<float>(args, n, dconsts, destbl, stream);
 }
 
 // One thread per column of [0, ncol): append failing columns to list.

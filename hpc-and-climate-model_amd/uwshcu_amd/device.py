@@ -197,7 +197,7 @@ _default_ctx = {}
 
 def default_context(device=None) -> Context:
     d = torch.cuda.current_device() if device is None else int(device)
-    if d not in _default_ctx:
+    if d not in _default_ctx or _default_ctx[d]._h is None:
         _default_ctx[d] = Context(d)
     return _default_ctx[d]
 

@@ -11,6 +11,27 @@
 
 using namespace uwshcu;
 
+// 1: run the level-parallel set-up (setup_level1/2 over every column and level)
+// before the columns, as the GPU pipeline does; 0: the columns do it themselves.
+static int g_staged = 0;
+extern "C" void hostemu_set_staged(int on) { g_staged = on; }
+
+template <class R>
+static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
+    if (g_staged) {
+        a.env_ready = 1;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < ncol; ++i)
+            for (int k = 0; k <= pver; ++k) Column<R>(a, T, i, i).setup_level1(k);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < ncol; ++i)
+            for (int k = 1; k <= pver; ++k) Column<R>(a, T, i, i).setup_level2(k);
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ncol; ++i) Column<R>(a, T, i, i).run();
+    a.env_ready = 0;
+}
+
 extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int precision, const uwshcu_params* par,
                                int pcols, int ncol, int pver, int ncnst, double dt, const double* ps0,
                                const double* zs0, const double* p0, const double* z0, const double* dp0,
@@ -46,11 +67,8 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     std::vector<double> cush_in(cush, cush + ncol);
     a.amb_flag = fp32 ? amb.data() : nullptr;
     const Thermo T{c, estbl};
-#pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < ncol; ++i) {
-        if (!fp32) Column<double>(a, T, i, i).run();
-        else Column<float>(a, T, i, i).run();
-    }
+    if (!fp32) run_columns<double>(a, T, ncol, pver);
+    else run_columns<float>(a, T, ncol, pver);
     int nlist = 0;
     if (precision == UWSHCU_MIXED) {
         std::vector<int> list;

@@ -56,7 +56,8 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False, stats=None):
+def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False, stats=None,
+        staged=False):
     """Run the host-compiled column code on a generator chunk.
 
     Returns (outputs dict, diag [ncol, NDIAG], rc, bad_col) like ``oracle.run``.
@@ -88,7 +89,9 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
                                   "qcu", "qlu", "qiu", "cbmf", "qc", "rliq", "cnt", "cnb")]
     nfix = C.c_int(0)
     args += [_p(np.ascontiguousarray(f["dpdry0"])), _p(out["wu"]), _p(diag), C.byref(bad), C.byref(nfix)]
+    lib(checked).hostemu_set_staged(1 if staged else 0)
     rc = lib(checked).hostemu_forward(*args)
+    lib(checked).hostemu_set_staged(0)
     if stats is not None:
         stats["recomputed"] = nfix.value
     out["cush"] = cush

@@ -21,7 +21,8 @@ def _cot(ch, seed):
 def _gpu_adjoint(ch, cot, workspace_limit=None, cap=None):
     import torch
     from uwshcu_amd import device
-    ctx = device.Context(0, workspace_limit=workspace_limit) if workspace_limit else device.default_context()
+    own = bool(workspace_limit or cap)
+    ctx = device.Context(0, workspace_limit=workspace_limit) if own else device.default_context()
     if cap:
         ctx.set_tape_capacity(cap)
     inp = device.chunk_to_device(ch, "cuda:0")
@@ -40,7 +41,7 @@ def _gpu_adjoint(ch, cot, workspace_limit=None, cap=None):
     res = {k: v.cpu().numpy() for k, v in out.items()}
     res["cush"] = cush.cpu().numpy()
     g = {k: v.cpu().numpy() for k, v in grads.items()}
-    if workspace_limit or cap:
+    if own:
         ctx.close()
     return rc, res, g
 

@@ -30,6 +30,19 @@ def test_fp64_column_code_is_bit_identical_to_oracle(ncol, pver, ncnst, inv):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("ncol,pver,ncnst", [(512, 35, 3), (128, 72, 10), (128, 30, 0)])
+def test_level_parallel_setup_gives_identical_results(ncol, pver, ncnst, precision):
+    """The GPU pipeline fills the set-A environment and the wet bulb level-parallel
+    before the column kernel; the outputs must equal the column-serial run bit for bit."""
+    ch = make_chunk(ncol, pver, ncnst, seed=70 + pver, ptop=100.0 if pver > 60 else 200.0)
+    a, ad, _, _ = hostemu.run(ch, precision=precision)
+    b, bd, _, _ = hostemu.run(ch, precision=precision, staged=True)
+    np.testing.assert_array_equal(ad, bd)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_fp64_column_code_with_other_tunables():
     from uwshcu_amd import _capi
     ch = make_chunk(256, 30, 3, seed=21)
@@ -100,13 +113,14 @@ def test_libm_sensitivity_at_72_levels_and_10_constituents():
     assert np.median(w) < 1e-12
 
 
+@pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("ncol,pver,ncnst,inv", [(192, 30, 3, 0.25), (128, 72, 10, 0.2), (128, 35, 0, 0.1)])
-def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision):
+def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision, staged):
     """Build with every array and workspace index asserted (an assertion aborts the
     process), and check the results equal the unchecked build's."""
     ch = make_chunk(ncol, pver, ncnst, seed=pver + ncnst, inversion_frac=inv, ptop=100.0 if pver > 60 else 200.0)
-    a, ad, _, _ = hostemu.run(ch, precision=precision, checked=True)
+    a, ad, _, _ = hostemu.run(ch, precision=precision, checked=True, staged=staged)
     b, bd, _, _ = hostemu.run(ch, precision=precision)
     np.testing.assert_array_equal(ad, bd)
     for k in a:
