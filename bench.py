@@ -143,6 +143,9 @@ def main():
     conv_frac = float((diag[7] > 0).mean())
     if prec == _capi.MIXED and not args.adjoint:
         nfix = ctx.mixed_recomputed(stream)
+        f64_ms, _ = timed(lambda: fwd(_capi.FP64), args.steps, 1, stream)
+        extra["fp64_forward_ms_same_chunk"] = float(np.mean(f64_ms))
+        extra["speedup_vs_fp64"] = float(np.mean(f64_ms)) / ms
         mix = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
         mix["cush"] = cush.cpu().numpy()
         fwd(_capi.FP64)
