@@ -251,12 +251,12 @@ struct Column {
             const D pb = in(a.ps0, k - 1), pt = in(a.ps0, k), pm = in(a.p0, k);
             const D thl0bot = E(thl, k) + E(ssthl, k) * (pb - pm);
             const D qt0bot = E(qt, k) + E(ssqt, k) * (pb - pm);
-            if (T.conden(pb, thl0bot, qt0bot, th, qv, ql, qi)) return false;
+            if (T.conden_x(pb, E(A_EXNS0, k - 1), thl0bot, qt0bot, th, qv, ql, qi)) return false;
             E(thvb, k) = th * (1.0 + C.zvir * qv - ql - qi);
             E(thvlb, k) = thl0bot * (1.0 + C.zvir * qt0bot);
             const D thl0top = E(thl, k) + E(ssthl, k) * (pt - pm);
             const D qt0top = E(qt, k) + E(ssqt, k) * (pt - pm);
-            if (T.conden(pt, thl0top, qt0top, th, qv, ql, qi)) return false;
+            if (T.conden_x(pt, E(A_EXNS0, k), thl0top, qt0top, th, qv, ql, qi)) return false;
             E(thvt, k) = th * (1.0 + C.zvir * qv - ql - qi);
             E(thvlt, k) = thl0top * (1.0 + C.zvir * qt0top);
         }
@@ -270,7 +270,7 @@ struct Column {
         D& tv = E(E_THV0MID, k);
         if (tv == 0.0) {
             D th, qv, ql, qi, es, qs;
-            if (T.conden(in(a.p0, k), E(A_THL0, k), E(A_QT0, k), th, qv, ql, qi)) {
+            if (T.conden_x(in(a.p0, k), E(A_EXN0, k), E(A_THL0, k), E(A_QT0, k), th, qv, ql, qi)) {
                 tv = -1.0;
             } else {
                 tv = th * (1.0 + C.zvir * qv - ql - qi);
@@ -568,13 +568,13 @@ UWSHCU_HD void Column<R, D>::run() const {
                     cinlcl = fmax(cin, 0.0);
                     cin = cinlcl;
                     thvubot = thvlsrc;
-                    if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
                     T.getbuoy(plcl, thv0lcl, in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                     if (plfc > 0.0) { klfc = k; break; }
                 } else {
                     thvubot = thvutop;
-                    if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
                     T.getbuoy(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                     if (plfc > 0.0) { klfc = k; break; }
@@ -583,9 +583,9 @@ UWSHCU_HD void Column<R, D>::run() const {
         } else {
             cinlcl = 0.0;
             for (int k = kinv; k <= mkx - 1; ++k) {
-                if (T.conden(in(a.ps0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                if (T.conden_x(in(a.ps0, k - 1), E(A_EXNS0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 const D thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
-                if (T.conden(in(a.ps0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 const D thvutop = th * (1.0 + C.zvir * qvj - qlj - qij);
                 T.getbuoy(in(a.ps0, k - 1), E(e_thvb, k), in(a.ps0, k), E(e_thvt, k), thvubot, thvutop, plfc, cin);
                 if (plfc > 0.0) { klfc = k; break; }
@@ -713,7 +713,8 @@ UWSHCU_HD void Column<R, D>::run() const {
             wtw = wlcl * wlcl;
             R pe = R(0.5) * (prel + R(in(a.ps0, krel)));
             R dpe = prel - R(in(a.ps0, krel));
-            R exne = R(T.exnf(pe));
+            D exe = T.exnf(D(pe));          // Exner function of pe, FP64
+            R exne = R(exe);
             R thvebot = thv0rel;
             R thle = Er(A_THL0, krel) + Er(A_SSTHL0, krel) * (pe - R(in(a.p0, krel)));
             R qte = Er(A_QT0, krel) + Er(A_SSQT0, krel) * (pe - R(in(a.p0, krel)));
@@ -737,7 +738,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                     R thv0j;
                     D es, qs;
                     if (k == krel) {
-                        if (T.conden(pe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        if (T.conden_x(pe, exe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                         thv0j = thv_of(th, qvj, qlj, qij);
                         T.qsat_es_qs(D(thle * exne), pe, es, qs);
                     } else {
@@ -747,7 +748,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                     }
                     rho0j = pe / (r * thv0j * exne);
                     const R excess0 = qte - R(qs);
-                    if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(pe, exe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
                         const R qlr = R(qlj), qir = R(qij);
@@ -756,7 +757,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                         qtue = qtue - exql - exqi;
                         thlue = thlue + (xlv / cp / exne) * exql + (xls / cp / exne) * exqi;
                         // unchanged arguments give unchanged results, so only re-adjust here
-                        if (T.conden(pe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        if (T.conden_x(pe, exe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     }
                     const R thvj = thv_of(th, qvj, qlj, qij);
                     T.qsat_es_qs(D(thlue * exne), pe, es, qs);
@@ -771,7 +772,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                         const R xsat = excessu / (excessu - excess0);
                         const R thlxsat = thlue + xsat * (thle - thlue);
                         const R qtxsat = qtue + xsat * (qte - qtue);
-                        if (T.conden(pe, thlxsat, qtxsat, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        if (T.conden_x(pe, exe, thlxsat, qtxsat, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                         const R thvxsat = thv_of(th, qvj, qlj, qij);
                         R x_cu = 0, x_en = 0;
                         for (int kk = 1; kk <= 2; ++kk) {
@@ -849,7 +850,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                                           (tre + sst * dpe / R(2) - WT(1, m, km1) + sst / ferk) * ef;
                         }
                     }
-                    if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
                         const R qlr = R(qlj), qir = R(qij);
@@ -859,7 +860,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                         thluk = thluk + (xlv / cp / Er(A_EXNS0, k)) * exql + (xls / cp / Er(A_EXNS0, k)) * exqi;
                         W(W_DWTEN, k) = exql;
                         W(W_DITEN, k) = exqi;
-                        if (T.conden(in(a.ps0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     } else {
                         W(W_DWTEN, k) = R(0);
                         W(W_DITEN, k) = R(0);
@@ -914,6 +915,8 @@ UWSHCU_HD void Column<R, D>::run() const {
                 pe = R(in(a.p0, k + 1));
                 dpe = R(in(a.dp0, k + 1));
                 exne = Er(A_EXN0, k + 1);
+                if constexpr (sizeof(R) == sizeof(double)) exe = E(A_EXN0, k + 1);   // = exnf(pe), pe = p0(k+1)
+                else exe = T.exnf(D(pe));
                 thvebot = Er(A_THV0BOT, k + 1);
                 thle = Er(A_THL0, k + 1);
                 qte = Er(A_QT0, k + 1);
@@ -1162,7 +1165,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                 }
                 const R thl_prog = Er(A_THL0, k) + thlten_sub * dt;
                 const R qt_prog = fmax(Er(A_QT0, k) + qtten_sub * dt, R(1.e-12));
-                if (T.conden(in(a.p0, k), thl_prog, qt_prog, th, qvj, qlj, qij)) { exit_code = EX_SUBCONDEN; goto L333; }
+                if (T.conden_x(in(a.p0, k), E(A_EXN0, k), thl_prog, qt_prog, th, qvj, qlj, qij)) { exit_code = EX_SUBCONDEN; goto L333; }
                 W(W_QLSINK, k) = fmax(qlten_sub, -R(in(a.ql0, k)) / dt);
                 W(W_QISINK, k) = fmax(qiten_sub, -R(in(a.qi0, k)) / dt);
                 W(W_NLSINK, k) = fmax(nlten_sub, -trz(ixl, k) / dt);
@@ -1224,7 +1227,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                     qiubelow = R(qij);
                     qlr_base = qlubelow;
                     qir_base = qiubelow;
-                    if (T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     qlr = R(qlj); qir = R(qij);
                     const R fr = (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
                     qlu_mid = R(0.5) * (qlubelow + qlr) * (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
@@ -1236,7 +1239,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                     qlu_mid = R(0.5) * (qlubelow + qlr) * (-ppen) / R(in(a.ps0, k - 1) - in(a.ps0, k));
                     qiu_mid = R(0.5) * (qiubelow + qir) * (-ppen) / R(in(a.ps0, k - 1) - in(a.ps0, k));
                 } else {
-                    if (T.conden(in(a.ps0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), W(W_THLU, k), W(W_QTU, k), th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     qlr = R(qlj); qir = R(qij);
                     qlu_mid = R(0.5) * (qlubelow + qlr);
                     qiu_mid = R(0.5) * (qiubelow + qir);
@@ -1269,7 +1272,7 @@ UWSHCU_HD void Column<R, D>::run() const {
                     nc_lm = nc_lm - g * umf_k * trz(ixl, k) / dps;
                     nc_im = nc_im - g * umf_k * trz(ixi, k) / dps;
                     D ql_e, qi_e;
-                    if (T.conden(in(a.p0, k), W(W_THLUE, k), W(W_QTUE, k), th, qvj, ql_e, qi_e)) { exit_code = EX_KBUPCONDEN; goto L333; }
+                    if (T.conden_x(in(a.p0, k), E(A_EXN0, k), W(W_THLUE, k), W(W_QTUE, k), th, qvj, ql_e, qi_e)) { exit_code = EX_KBUPCONDEN; goto L333; }
                     const R nl_e = (ql_e > 0.0 && has_nl) ? WT(2, ixl, k) : R(0);
                     const R ni_e = (qi_e > 0.0 && has_ni) ? WT(2, ixi, k) : R(0);
                     const R emfk = W(W_EMF, k);

@@ -78,15 +78,18 @@ struct ThermoT {
     }
 
     // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
-    UWSHCU_HD UWSHCU_OUTLINE int conden(D p, D thl, D qt, D& th, D& qv, D& ql,
-                                          D& qi) const {
+    UWSHCU_HD int conden(D p, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
+        return conden_x(p, exnf(p), thl, qt, th, qv, ql, qi);
+    }
+    // The same with the Exner function of p supplied by the caller (ex must equal
+    // exnf(p); callers pass the set-A values of the model levels).
+    UWSHCU_HD UWSHCU_OUTLINE int conden_x(D p, D ex, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
 #ifndef UWSHCU_AD_TAPE_CONDEN
         if constexpr (is_ad<D>::value) {
             int id;
             if (conden_ift(p, thl, qt, th, qv, ql, qi, id)) return id;
         }
 #endif
-        const D ex = exnf(p);
         const D tc = thl * ex;
         const D nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
         const D leff = (1.0 - nu) * c->xlv + nu * c->xls;
