@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", action="store_true", help="time forward + adjoint")
+    ap.add_argument("--workspace-gib", type=float, default=None,
+                    help="context workspace limit (default: the library's 24 GiB; the adjoint uses 160 GiB)")
     return ap.parse_args()
 
 
@@ -94,7 +96,8 @@ def main():
     torch.cuda.set_device(0)
     prec = _capi.FP64 if args.precision == "fp64" else _capi.MIXED
     ch = make_chunk(args.ncol, args.pver, args.ncnst, seed=args.seed, inversion_frac=args.inversion_frac)
-    ctx = device.Context(0)
+    ws_gib = args.workspace_gib or (160.0 if args.adjoint else None)
+    ctx = device.Context(0, workspace_limit=int(ws_gib * (1 << 30)) if ws_gib else None)
     inp = device.chunk_to_device(ch, "cuda:0")
     out = ctx.alloc_outputs(args.ncol, args.pver, args.ncnst, diag=True)
     cush = torch.empty_like(inp["cush"])

@@ -18,6 +18,7 @@ UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin) {
     t.cap = cap;
     t.overflow = nin > cap ? 1 : 0;
     t.n = nin < cap ? nin : cap;
+    if (!nodes) return;
     for (int q = 0; q < t.n; ++q) {
         TapeNode nd;
         nd.a = -1;

@@ -8,16 +8,33 @@
 
 namespace uwshcu {
 
-__global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
-                                                           const double* __restrict__ estbl, TapeNode* tape_base,
-                                                           long long cap, int* node_out) {
+// Sizing pass: run each column's recording without storing it; counts[col] = nodes.
+__global__ __launch_bounds__(64) void adjoint_count_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                                                          const double* __restrict__ estbl, int* counts) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
     for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
     __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     TapeCtx& tape = args.ad.tapes[j];
-    ad_begin(tape, tape_base + (size_t)j * (size_t)cap, (int)cap, ad_inputs(args.mkx));
+    ad_begin(tape, nullptr, 0x7fffffff, ad_inputs(args.mkx));
+    const ThermoT<AD> T{consts, s_estbl};
+    const Column<AD, AD> col(args, T, args.col0 + j, j);
+    col.run();
+    counts[args.col0 + j] = tape.n;
+}
+
+// Recording pass: slot j's tape holds cap[j] nodes at tape_base + off[j].
+__global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                                                           const double* __restrict__ estbl, TapeNode* tape_base,
+                                                           const long long* off, const int* cap, int* node_out) {
+    __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
+    for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
+    __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    TapeCtx& tape = args.ad.tapes[j];
+    ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j);
     col.run();
@@ -27,12 +44,11 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
 
 __global__ __launch_bounds__(64) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
                                                           long long ldc, const TapeCtx* tapes, const int* node_out,
-                                                          double* adj_base, long long cap, AdCotangents cot,
+                                                          double* adj_base, const long long* off, AdCotangents cot,
                                                           AdGradients g) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    ad_reverse(tapes[j], adj_base + (size_t)j * (size_t)cap, node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc,
-               col0 + j, cot, g);
+    ad_reverse(tapes[j], adj_base + off[j], node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc, col0 + j, cot, g);
 }
 
 // Sum of the per-column parameter gradients par[3][ldc] over ncol columns, in FP64:
@@ -51,16 +67,22 @@ __global__ __launch_bounds__(256) void param_sum_kernel(const double* par, long 
     if (threadIdx.x == 0) sum[p] = s[0];
 }
 
+hipError_t launch_adjoint_count(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                                int* counts, hipStream_t stream) {
+    hipLaunchKernelGGL(adjoint_count_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, args, n, dconsts, destbl, counts);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                          TapeNode* tape_base, long long cap, int* node_out, double* adj_base,
+                          TapeNode* tape_base, double* adj_base, const long long* off, const int* cap, int* node_out,
                           const AdCotangents& cot, const AdGradients& g, hipStream_t stream) {
     const int block = 64;
     const int grid = (n + block - 1) / block;
     hipLaunchKernelGGL(adjoint_record_kernel, dim3(grid), dim3(block), 0, stream, args, n, dconsts, destbl, tape_base,
-                       cap, node_out);
+                       off, cap, node_out);
     hipLaunchKernelGGL(adjoint_sweep_kernel, dim3(grid), dim3(block), 0, stream, n, args.col0, args.mkx, args.ncnst,
                        args.wld, (long long)args.ldc, (const TapeCtx*)args.ad.tapes, (const int*)node_out, adj_base,
-                       cap, cot, g);
+                       off, cot, g);
     return hipGetLastError();
 }
 

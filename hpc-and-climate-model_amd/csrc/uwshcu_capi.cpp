@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "uwshcu.h"
 #include "uwshcu_kernels.h"
@@ -38,11 +40,13 @@ struct uwshcu_ctx {
     int* d_count = nullptr;
     size_t mixed_cols = 0;
     double mixed_tol = 1.e-4;
-    // adjoint: tape nodes per column (0 = (2048 + 512 ncnst) per level interface) and a per-column
+    // adjoint: tape capacity per column (0 = as many as the column needs) and a per-column
     // parameter-gradient buffer for callers that only want the sums
     long long tape_cap = 0;
     double* d_parcol = nullptr;
     size_t parcol_cols = 0;
+    int* d_counts = nullptr;
+    size_t count_cols = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
 };
@@ -220,24 +224,34 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     if (ncol == 0) return UWSHCU_OK;
     if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
     const int KS = pver + 1;
-    // default capacity: ~2.5x the longest tapes measured (48k nodes at 35x3, 230k at 72x10)
-    const long long cap = ctx->tape_cap > 0 ? ctx->tape_cap : (2048LL + 512LL * ncnst) * KS;
-    // per-column pieces: AD workspace, AD outputs, output nodes, tape header, tape, adjoints
+    const long long user_cap = ctx->tape_cap > 0 ? ctx->tape_cap : 0x7fffffffLL;
+    // per-column fixed pieces: AD workspace, AD outputs, output nodes, tape header,
+    // offset and capacity; the tape and the sweep's adjoints take 32 bytes per node
     const size_t b_ws = ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD));
     const size_t b_out = (size_t)ad_fields(ncnst) * KS * sizeof(AD);
     const size_t b_nodes = (size_t)ad_fields(ncnst) * KS * sizeof(int);
-    const size_t per_col = b_ws + b_out + b_nodes + sizeof(TapeCtx) + (size_t)cap * (sizeof(TapeNode) + sizeof(double));
-    long long nb = (long long)(ctx->ws_limit / (per_col + 1024));
-    if (nb > ncol) nb = ncol;
-    if (nb < 1) return UWSHCU_ERR_NOMEM;
-    if (nb > 64) nb = (nb / 64) * 64;
-    const size_t o_out = align256(b_ws * nb), o_nodes = o_out + align256(b_out * nb);
-    const size_t o_tapes = o_nodes + align256(b_nodes * nb), o_tape = o_tapes + align256(sizeof(TapeCtx) * nb);
-    const size_t o_adj = o_tape + align256((size_t)cap * sizeof(TapeNode) * nb);
-    const size_t total = o_adj + align256((size_t)cap * sizeof(double) * nb);
-    int rc = ensure_ws(ctx, total);
-    if (rc) return rc;
-    char* base = (char*)ctx->d_ws;
+    const size_t b_fix = b_ws + b_out + b_nodes + sizeof(TapeCtx) + sizeof(long long) + sizeof(int);
+    const size_t b_node = sizeof(TapeNode) + sizeof(double);
+    auto layout = [&](long long nb, size_t nodes, size_t* o) {   // o: out, nodes, tapes, off, cap, tape, adj, end
+        o[0] = align256(b_ws * nb);
+        o[1] = o[0] + align256(b_out * nb);
+        o[2] = o[1] + align256(b_nodes * nb);
+        o[3] = o[2] + align256(sizeof(TapeCtx) * nb);
+        o[4] = o[3] + align256(sizeof(long long) * nb);
+        o[5] = o[4] + align256(sizeof(int) * nb);
+        o[6] = o[5] + align256(sizeof(TapeNode) * nodes);
+        o[7] = o[6] + align256(sizeof(double) * nodes);
+    };
+    if ((size_t)ncol > ctx->count_cols) {
+        if (ctx->d_counts) {
+            if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+            (void)hipFree(ctx->d_counts);
+            ctx->d_counts = nullptr;
+            ctx->count_cols = 0;
+        }
+        if (!hip_ok(hipMalloc((void**)&ctx->d_counts, (size_t)ncol * sizeof(int)))) return UWSHCU_ERR_NOMEM;
+        ctx->count_cols = (size_t)ncol;
+    }
     double* parcol = grad->par_col;
     if (!parcol && grad->par_sum) {
         if ((size_t)pcols > ctx->parcol_cols) {
@@ -271,18 +285,88 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     a.cbmf = ch.cbmf; a.qc = ch.qc; a.rliq = ch.rliq; a.cnt = ch.cnt; a.cnb = ch.cnb; a.wu = ch.wu;
     a.diag = diag;
     a.err = ctx->d_err;
+
+    // 1. sizing pass in batches of the fixed per-column pieces
+    long long nbc = (long long)(ctx->ws_limit / (b_fix + 64));
+    if (nbc > ncol) nbc = ncol;
+    if (nbc < 1) return UWSHCU_ERR_NOMEM;
+    size_t o[8];
+    layout(nbc, 0, o);
+    int rc = ensure_ws(ctx, o[7]);
+    if (rc) return rc;
+    char* base = (char*)ctx->d_ws;
     a.ws = base;
-    a.wld = nb;
-    a.ad.out = (AD*)(base + o_out);
-    a.ad.tapes = (TapeCtx*)(base + o_tapes);
-    for (long long c0 = 0; c0 < ncol; c0 += nb) {
-        const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
+    a.wld = nbc;
+    a.ad.out = (AD*)(base + o[0]);
+    a.ad.tapes = (TapeCtx*)(base + o[2]);
+    for (long long c0 = 0; c0 < ncol; c0 += nbc) {
+        const int n = (int)((ncol - c0) < nbc ? (ncol - c0) : nbc);
         a.col0 = (int)c0;
-        if (!hip_ok(launch_adjoint(a, n, ctx->d_consts, ctx->d_estbl, (TapeNode*)(base + o_tape), cap,
-                                   (int*)(base + o_nodes), (double*)(base + o_adj), ct, g, stream)))
+        if (!hip_ok(launch_adjoint_count(a, n, ctx->d_consts, ctx->d_estbl, ctx->d_counts, stream)))
             return UWSHCU_ERR_HIP;
     }
+    std::vector<int> counts((size_t)ncol);
+    if (!hip_ok(hipMemcpyAsync(counts.data(), ctx->d_counts, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost,
+                               stream)) ||
+        !hip_ok(hipStreamSynchronize(stream)))
+        return UWSHCU_ERR_HIP;
+    // 2. batches whose exact tapes fit the workspace
+    std::vector<long long> starts{0};
+    {
+        size_t nodes = 0;
+        long long n = 0;
+        for (int c = 0; c < ncol; ++c) {
+            const size_t cn = (size_t)std::min<long long>(counts[c], user_cap);
+            if (n > 0 && (size_t)(n + 1) * (b_fix + 2048) + (nodes + cn) * b_node > ctx->ws_limit) {
+                starts.push_back(c);
+                nodes = 0;
+                n = 0;
+            }
+            nodes += cn;
+            ++n;
+        }
+        starts.push_back(ncol);
+    }
+    size_t need = 0;
+    std::vector<long long> off((size_t)ncol);
+    std::vector<int> cap((size_t)ncol);
+    for (size_t b = 0; b + 1 < starts.size(); ++b) {
+        long long acc = 0;
+        for (long long c = starts[b]; c < starts[b + 1]; ++c) {
+            off[c] = acc;
+            cap[c] = (int)std::min<long long>(counts[c], user_cap);
+            acc += cap[c];
+        }
+        layout(starts[b + 1] - starts[b], (size_t)acc, o);
+        need = std::max(need, o[7]);
+    }
+    if ((rc = ensure_ws(ctx, need))) return rc;
+    base = (char*)ctx->d_ws;
+    for (size_t b = 0; b + 1 < starts.size(); ++b) {
+        const long long c0 = starts[b], n = starts[b + 1] - c0;
+        size_t nodes = 0;
+        for (long long c = c0; c < c0 + n; ++c) nodes += (size_t)cap[c];
+        layout(n, nodes, o);
+        a.ws = base;
+        a.wld = n;
+        a.col0 = (int)c0;
+        a.ad.out = (AD*)(base + o[0]);
+        a.ad.tapes = (TapeCtx*)(base + o[2]);
+        if (!hip_ok(hipMemcpyAsync(base + o[3], off.data() + c0, (size_t)n * sizeof(long long), hipMemcpyHostToDevice,
+                                   stream)) ||
+            !hip_ok(hipMemcpyAsync(base + o[4], cap.data() + c0, (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                                   stream)))
+            return UWSHCU_ERR_HIP;
+        if (!hip_ok(launch_adjoint(a, (int)n, ctx->d_consts, ctx->d_estbl, (TapeNode*)(base + o[5]),
+                                   (double*)(base + o[6]), (const long long*)(base + o[3]),
+                                   (const int*)(base + o[4]), (int*)(base + o[1]), ct, g, stream)))
+            return UWSHCU_ERR_HIP;
+        // the host copies above read off/cap, which live until the next batch's copies
+        if (b + 2 < starts.size() && !hip_ok(hipStreamSynchronize(stream))) return UWSHCU_ERR_HIP;
+    }
     if (grad->par_sum && !hip_ok(launch_param_sum(parcol, pcols, ncol, grad->par_sum, stream))) return UWSHCU_ERR_HIP;
+    // off/cap are host vectors: make sure the last copies have completed before they go
+    if (!hip_ok(hipStreamSynchronize(stream))) return UWSHCU_ERR_HIP;
     return UWSHCU_OK;
 }
 
@@ -412,6 +496,7 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_parcol) (void)hipFree(ctx->d_parcol);
+    if (ctx->d_counts) (void)hipFree(ctx->d_counts);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return UWSHCU_OK;

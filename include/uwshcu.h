@@ -98,10 +98,12 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
  * pattern is the forward's own), writes the forward outputs exactly as
  * uwshcu_forward_device(UWSHCU_FP64) does, then sweeps each tape backwards from the
  * cotangents `cot` and writes the requested gradients `grad` (uwshcu_types.h).
- * Tapes take uwshcu_ctx_set_tape_capacity nodes per column (default
- * (2048 + 512*ncnst)*(pver+1), 32 bytes each with the sweep's scratch) and columns run in batches that fit the
- * workspace limit; a column whose tape overflows sets UWSHCU_ERR_TAPE in
- * uwshcu_status.  Stream-ordered like uwshcu_forward_device. */
+ * A sizing pass first counts each column's tape (about 30 k nodes at 35 levels, 32
+ * bytes each with the sweep's scratch); the call then synchronises `stream` once to
+ * read the counts and records the columns in batches whose exact tapes fit the
+ * workspace limit.  A column longer than uwshcu_ctx_set_tape_capacity (default: no
+ * limit) sets UWSHCU_ERR_TAPE in uwshcu_status and gets no gradient.  The call
+ * returns after its work is queued on `stream`, and its host buffers are released. */
 int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,
     int pcols, int ncol, int pver, int ncnst, double dt,
     const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
@@ -115,7 +117,7 @@ int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag,
     const uwshcu_cotangents *cot, const uwshcu_gradients *grad, void *stream);
 
-/* Tape capacity of the adjoint in nodes per column (0 restores the default). */
+/* Upper bound on the adjoint's tape length in nodes per column (0: no bound). */
 int uwshcu_ctx_set_tape_capacity(uwshcu_ctx *ctx, long long nodes_per_column);
 
 /* Synchronise `stream` and return the status of the forward calls issued on it
