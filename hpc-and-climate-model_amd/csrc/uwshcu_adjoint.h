@@ -13,9 +13,10 @@
 
 namespace uwshcu {
 
-UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin) {
+UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin, int stride = 1) {
     t.nodes = nodes;
     t.cap = cap;
+    t.stride = stride;
     t.overflow = nin > cap ? 1 : 0;
     t.n = nin < cap ? nin : cap;
     if (!nodes) return;
@@ -25,7 +26,7 @@ UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin) {
         nd.b = -1;
         nd.da = 0.0;
         nd.db = 0.0;
-        nodes[q] = nd;
+        nodes[tape_slot(t, q)] = nd;
     }
 }
 
@@ -43,50 +44,52 @@ struct AdGradients {
 };
 
 // Reverse sweep of one column: seed the outputs' adjoints from the cotangents,
-// propagate back through the tape, write the input gradients.
+// propagate back through the tape, write the input gradients.  adj is laid out like
+// the tape's nodes (adj[tape_slot(t, q)] is the adjoint of node q).
 UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes, size_t wld, int slot, int mkx,
                                  int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
     const int KS = mkx + 1;
-    for (int q = 0; q < t.n; ++q) adj[q] = 0.0;
+    for (int q = 0; q < t.n; ++q) adj[tape_slot(t, q)] = 0.0;
+    auto A = [&](int q) -> double& { return adj[tape_slot(t, q)]; };
     auto node = [&](int f, int k) { return nodes[((size_t)f * KS + k) * wld + slot]; };
     auto row = [&](int k) { return (size_t)(mkx - k) * ldc + col; };
     for (int f = 0; f < ADF_TR; ++f) {
         if (!cot.f[f]) continue;
         for (int k = (f <= ADF_WU ? 0 : 1); k <= mkx; ++k) {
             const int nd = node(f, k);
-            if (nd >= 0) adj[nd] += cot.f[f][row(k)];
+            if (nd >= 0) A(nd) += cot.f[f][row(k)];
         }
     }
     if (cot.trten)
         for (int m = 1; m <= ncnst; ++m)
             for (int k = 1; k <= mkx; ++k) {
                 const int nd = node(ADF_TR + m - 1, k);
-                if (nd >= 0) adj[nd] += cot.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + col];
+                if (nd >= 0) A(nd) += cot.trten[((size_t)(m - 1) * mkx + (mkx - k)) * ldc + col];
             }
     for (int q = 0; q < ADC_COUNT; ++q) {
         if (!cot.scal[q]) continue;
         const int nd = node(ADF_TR + ncnst + q, 0);
-        if (nd >= 0) adj[nd] += cot.scal[q][col];
+        if (nd >= 0) A(nd) += cot.scal[q][col];
     }
     const int nin = ad_inputs(mkx);
     for (int q = t.n - 1; q >= nin; --q) {
-        const double w = adj[q];
+        const double w = A(q);
         if (w == 0.0) continue;
-        const TapeNode nd = t.nodes[q];
-        if (nd.a >= 0) adj[nd.a] += w * nd.da;
-        if (nd.b >= 0) adj[nd.b] += w * nd.db;
+        const TapeNode nd = t.nodes[tape_slot(t, q)];
+        if (nd.a >= 0) A(nd.a) += w * nd.da;
+        if (nd.b >= 0) A(nd.b) += w * nd.db;
     }
     for (int k = 1; k <= mkx; ++k) {
-        if (g.t0) g.t0[row(k)] = adj[ADI_T0 * KS + k];
-        if (g.qv0) g.qv0[row(k)] = adj[ADI_QV0 * KS + k];
-        if (g.u0) g.u0[row(k)] = adj[ADI_U0 * KS + k];
-        if (g.v0) g.v0[row(k)] = adj[ADI_V0 * KS + k];
+        if (g.t0) g.t0[row(k)] = A(ADI_T0 * KS + k);
+        if (g.qv0) g.qv0[row(k)] = A(ADI_QV0 * KS + k);
+        if (g.u0) g.u0[row(k)] = A(ADI_U0 * KS + k);
+        if (g.v0) g.v0[row(k)] = A(ADI_V0 * KS + k);
     }
     if (g.tke)
-        for (int k = 0; k <= mkx; ++k) g.tke[row(k)] = adj[ADI_TKE * KS + k];
+        for (int k = 0; k <= mkx; ++k) g.tke[row(k)] = A(ADI_TKE * KS + k);
     if (g.pblh) g.pblh[col] = 0.0;
     if (g.par)
-        for (int j = 0; j < 3; ++j) g.par[(size_t)j * ldc + col] = adj[ADI_PARAMS * KS + j];
+        for (int j = 0; j < 3; ++j) g.par[(size_t)j * ldc + col] = A(ADI_PARAMS * KS + j);
 }
 
 }  // namespace uwshcu

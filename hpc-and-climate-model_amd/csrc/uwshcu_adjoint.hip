@@ -16,15 +16,18 @@ __global__ __launch_bounds__(64) void adjoint_count_kernel(FwdArgs args, int n, 
     __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    TapeCtx& tape = args.ad.tapes[j];
+    // the tape header lives in LDS: every recorded operation updates it
+    __shared__ TapeCtx s_tape[64];
+    TapeCtx& tape = s_tape[threadIdx.x];
     ad_begin(tape, nullptr, 0x7fffffff, ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
-    const Column<AD, AD> col(args, T, args.col0 + j, j);
+    const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
     col.run();
     counts[args.col0 + j] = tape.n;
 }
 
-// Recording pass: slot j's tape holds cap[j] nodes at tape_base + off[j].
+// Recording pass: slot j's tape holds cap[j] nodes from tape_base + off[j] with
+// stride 64: the 64 columns of a wavefront interleave their (reversed) tapes.
 __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                                                            const double* __restrict__ estbl, TapeNode* tape_base,
                                                            const long long* off, const int* cap, int* node_out) {
@@ -33,12 +36,14 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    TapeCtx& tape = args.ad.tapes[j];
-    ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx));
+    __shared__ TapeCtx s_tape[64];
+    TapeCtx& tape = s_tape[threadIdx.x];
+    ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx), 64);
     const ThermoT<AD> T{consts, s_estbl};
-    const Column<AD, AD> col(args, T, args.col0 + j, j);
+    const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
     col.run();
     col.ad_export(node_out);
+    args.ad.tapes[j] = tape;     // the sweep reads the header from HBM
     if (tape.overflow) record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
 }
 

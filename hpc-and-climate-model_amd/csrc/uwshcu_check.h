@@ -30,7 +30,9 @@ UWSHCU_HD inline bool column_fails_check(const FwdArgs& a, const uwshcu_consts& 
         const double qv = at(a.qvten, r), ql = at(a.qlten, r), qi = at(a.qiten, r);
         const double u0 = at(a.u0, r), v0 = at(a.v0, r);
         const double uf = u0 + at(a.uten, r) * dt, vf = v0 + at(a.vten, r) * dt;
-        const double dke = 0.5 * ((uf * uf - u0 * u0) + (vf * vf - v0 * v0)) / dt;
+        // (uf-u0)(uf+u0): exactly zero where the wind tendency is, whether or not the
+        // compiler fuses the products
+        const double dke = 0.5 * ((uf - u0) * (uf + u0) + (vf - v0) * (vf + v0)) / dt;
         w += (qv + ql + qi) * dpg;
         te += (at(a.sten, r) + dke + (C.xlv + C.xlf) * qv + C.xlf * ql) * dpg;
         acc += fabs(at(a.qrten, r)) + fabs(at(a.qsten, r)) + fabs(at(a.evapc, r)) + fabs(at(a.cufrc, r)) +
@@ -47,7 +49,7 @@ UWSHCU_HD inline bool column_fails_check(const FwdArgs& a, const uwshcu_consts& 
     const double eres = te + (precip + rliq - snow) * 1000.0 * C.xlf;
     acc += fabs(a.cbmf[i]) + fabs(a.cush[i]) + fs + fq;
     if (!(finite_value(w) && finite_value(eres) && finite_value(acc))) return true;
-    return fabs(w) > tol * fq || fabs(eres) > tol * fs;
+    return (fq > 0.0 && fabs(w) > tol * fq) || (fs > 0.0 && fabs(eres) > tol * fs);
 }
 
 }  // namespace uwshcu

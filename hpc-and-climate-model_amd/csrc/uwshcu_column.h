@@ -127,12 +127,12 @@ struct Column {
     R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
     TapeCtx* tape;              // adjoint kernel: this column's tape
 
-    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D>& th, int col, int slot)
+    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D>& th, int col, int slot, TapeCtx* own_tape = nullptr)
         : a(args), T(th), C(*th.c), i(col), iw(slot), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
           ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<D*>(args.ws)),
           ws(reinterpret_cast<R*>(reinterpret_cast<D*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
                                                                     (size_t)args.wld)),
-          tape(args.ad.tapes ? args.ad.tapes + slot : nullptr) {}
+          tape(own_tape ? own_tape : args.ad.tapes ? args.ad.tapes + slot : nullptr) {}
 
     // caller arrays: UW level k <-> CAM row mkx-k (layers k=1..mkx and interfaces k=0..mkx)
     UWSHCU_HD inline size_t row(int k) const { return (size_t)(mkx - k) * ldc + i; }
