@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", action="store_true", help="time forward + adjoint")
+    ap.add_argument("--tile", type=int, default=1,
+                    help="generate ncol/tile distinct columns and repeat them tile times on the GPU "
+                         "(for the multi-million-column workloads; columns are independent)")
     ap.add_argument("--workspace-gib", type=float, default=None,
                     help="context workspace limit (default: the library's 24 GiB; the adjoint uses 160 GiB)")
     return ap.parse_args()
@@ -95,10 +98,16 @@ def main():
 
     torch.cuda.set_device(0)
     prec = _capi.FP64 if args.precision == "fp64" else _capi.MIXED
-    ch = make_chunk(args.ncol, args.pver, args.ncnst, seed=args.seed, inversion_frac=args.inversion_frac)
+    if args.ncol % args.tile:
+        sys.exit("bench.py: --ncol must be a multiple of --tile")
+    ptop = 100.0 if args.pver > 60 else 200.0
+    ch = make_chunk(args.ncol // args.tile, args.pver, args.ncnst, seed=args.seed,
+                    inversion_frac=args.inversion_frac, ptop=ptop)
     ws_gib = args.workspace_gib or (160.0 if args.adjoint else None)
     ctx = device.Context(0, workspace_limit=int(ws_gib * (1 << 30)) if ws_gib else None)
     inp = device.chunk_to_device(ch, "cuda:0")
+    if args.tile > 1:
+        inp = {k: v.repeat(*([1] * (v.dim() - 1)), args.tile).contiguous() for k, v in inp.items()}
     out = ctx.alloc_outputs(args.ncol, args.pver, args.ncnst, diag=True)
     cush = torch.empty_like(inp["cush"])
     stream = torch.cuda.current_stream()
@@ -177,7 +186,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if (prec == _capi.FP64 or args.adjoint) else "f32+f64",
-        "data": "synthetic",
+        "data": "synthetic" if args.tile == 1 else f"synthetic, {args.ncol // args.tile} distinct columns tiled {args.tile}x",
         "config": {
             "workload": f"{mode} of compute_uwshcu_inv, {args.ncol} columns x {args.pver} levels x "
                         f"{args.ncnst} constituents, dt={ch.dt:g} s",

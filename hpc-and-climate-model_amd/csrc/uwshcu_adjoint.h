@@ -46,8 +46,18 @@ struct AdGradients {
 // Reverse sweep of one column: seed the outputs' adjoints from the cotangents,
 // propagate back through the tape, write the input gradients.  adj is laid out like
 // the tape's nodes (adj[tape_slot(t, q)] is the adjoint of node q).
+//
+// With a window (win != nullptr, W entries, win[q % W]) the adjoints of the W most
+// recent nodes are kept in it: 82 % of the operands a node refers to lie within 64
+// nodes of it (35-level columns), so most of the sweep's read-modify-writes stay out
+// of HBM.  A node's adjoint enters the window when the sweep comes within W of it,
+// carrying what farther nodes already added in adj; later contributions go to the
+// window.  Every adjoint still receives its contributions in decreasing node order,
+// so the result is the same, bit for bit, as without the window.
+template <class Win>
 UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes, size_t wld, int slot, int mkx,
-                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
+                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g,
+                                 Win win, int W) {
     const int KS = mkx + 1;
     for (int q = 0; q < t.n; ++q) adj[tape_slot(t, q)] = 0.0;
     auto A = [&](int q) -> double& { return adj[tape_slot(t, q)]; };
@@ -72,12 +82,32 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes
         if (nd >= 0) A(nd) += cot.scal[q][col];
     }
     const int nin = ad_inputs(mkx);
-    for (int q = t.n - 1; q >= nin; --q) {
-        const double w = A(q);
-        if (w == 0.0) continue;
-        const TapeNode nd = t.nodes[tape_slot(t, q)];
-        if (nd.a >= 0) A(nd.a) += w * nd.da;
-        if (nd.b >= 0) A(nd.b) += w * nd.db;
+    if (W > 0) {
+        for (int q = t.n - 1; q >= 0 && q >= t.n - W; --q) win(q % W) = A(q);
+        for (int q = t.n - 1; q >= nin; --q) {
+            const double w = win(q % W);
+            if (w != 0.0) {
+                const TapeNode nd = t.nodes[tape_slot(t, q)];
+                if (nd.a >= 0) {
+                    if (q - nd.a < W) win(nd.a % W) += w * nd.da;
+                    else A(nd.a) += w * nd.da;
+                }
+                if (nd.b >= 0) {
+                    if (q - nd.b < W) win(nd.b % W) += w * nd.db;
+                    else A(nd.b) += w * nd.db;
+                }
+            }
+            if (q - W >= 0) win(q % W) = A(q - W);      // node q - W enters the window
+        }
+        for (int q = nin - 1; q >= 0 && q >= nin - W; --q) A(q) = win(q % W);
+    } else {
+        for (int q = t.n - 1; q >= nin; --q) {
+            const double w = A(q);
+            if (w == 0.0) continue;
+            const TapeNode nd = t.nodes[tape_slot(t, q)];
+            if (nd.a >= 0) A(nd.a) += w * nd.da;
+            if (nd.b >= 0) A(nd.b) += w * nd.db;
+        }
     }
     for (int k = 1; k <= mkx; ++k) {
         if (g.t0) g.t0[row(k)] = A(ADI_T0 * KS + k);
@@ -90,6 +120,14 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes
     if (g.pblh) g.pblh[col] = 0.0;
     if (g.par)
         for (int j = 0; j < 3; ++j) g.par[(size_t)j * ldc + col] = A(ADI_PARAMS * KS + j);
+}
+
+struct NoWindow {
+    UWSHCU_HD double& operator()(int) const { return *(double*)nullptr; }
+};
+UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes, size_t wld, int slot, int mkx,
+                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
+    ad_reverse(t, adj, nodes, wld, slot, mkx, ncnst, ldc, col, cot, g, NoWindow{}, 0);
 }
 
 }  // namespace uwshcu

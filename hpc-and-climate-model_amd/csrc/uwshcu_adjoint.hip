@@ -26,8 +26,7 @@ __global__ __launch_bounds__(64) void adjoint_count_kernel(FwdArgs args, int n, 
     counts[args.col0 + j] = tape.n;
 }
 
-// Recording pass: slot j's tape holds cap[j] nodes from tape_base + off[j] with
-// stride 64: the 64 columns of a wavefront interleave their (reversed) tapes.
+// Recording pass: slot j's tape holds cap[j] nodes at tape_base + off[j].
 __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                                                            const double* __restrict__ estbl, TapeNode* tape_base,
                                                            const long long* off, const int* cap, int* node_out) {
@@ -38,7 +37,7 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     if (j >= n) return;
     __shared__ TapeCtx s_tape[64];
     TapeCtx& tape = s_tape[threadIdx.x];
-    ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx), 64);
+    ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
     col.run();
@@ -47,13 +46,20 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     if (tape.overflow) record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
 }
 
+// Backward sweep, one column per thread, with a 64-entry window of recent adjoints
+// per thread in LDS (32 KB per workgroup of 64).
+constexpr int kSweepWindow = 64;
 __global__ __launch_bounds__(64) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
                                                           long long ldc, const TapeCtx* tapes, const int* node_out,
                                                           double* adj_base, const long long* off, AdCotangents cot,
                                                           AdGradients g) {
+    __shared__ double s_win[kSweepWindow][64];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    ad_reverse(tapes[j], adj_base + off[j], node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc, col0 + j, cot, g);
+    const int lane = threadIdx.x;
+    auto win = [&](int s) -> double& { return s_win[s][lane]; };
+    ad_reverse(tapes[j], adj_base + off[j], node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc, col0 + j, cot, g, win,
+               kSweepWindow);
 }
 
 // Sum of the per-column parameter gradients par[3][ldc] over ncol columns, in FP64:

@@ -310,27 +310,22 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
                                stream)) ||
         !hip_ok(hipStreamSynchronize(stream)))
         return UWSHCU_ERR_HIP;
-    // 2. batches of whole wavefronts (64 columns) whose tapes fit the workspace.  The 64
-    // tapes of a wavefront interleave (stride 64), so a wavefront takes 64 x its longest
-    // tape; column c of the batch starts at its wavefront's block + (c % 64).
+    // 2. batches whose exact tapes fit the workspace; column c of a batch starts at
+    // the sum of the earlier columns' tape lengths
     std::vector<int> cap((size_t)ncol);
     for (int c = 0; c < ncol; ++c) cap[c] = (int)std::min<long long>(counts[c], user_cap);
-    const int nw = (ncol + 63) / 64;
-    std::vector<size_t> rows((size_t)nw, 0);
-    for (int c = 0; c < ncol; ++c) rows[c / 64] = std::max(rows[c / 64], (size_t)cap[c]);
     std::vector<long long> starts{0};
     {
         size_t nodes = 0;
         long long n = 0;
-        for (int w = 0; w < nw; ++w) {
-            const long long wc = std::min<long long>(64, ncol - 64LL * w);
-            if (n > 0 && (size_t)(n + wc) * (b_fix + 2048) + (nodes + rows[w] * 64) * b_node > ctx->ws_limit) {
-                starts.push_back(64LL * w);
+        for (int c = 0; c < ncol; ++c) {
+            if (n > 0 && (size_t)(n + 1) * (b_fix + 2048) + (nodes + cap[c]) * b_node > ctx->ws_limit) {
+                starts.push_back(c);
                 nodes = 0;
                 n = 0;
             }
-            nodes += rows[w] * 64;
-            n += wc;
+            nodes += (size_t)cap[c];
+            ++n;
         }
         starts.push_back(ncol);
     }
@@ -339,10 +334,9 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     size_t need = 0;
     for (size_t b = 0; b + 1 < starts.size(); ++b) {
         size_t acc = 0;
-        for (long long w = starts[b] / 64; w * 64 < starts[b + 1]; ++w) {
-            for (long long c = 64 * w; c < std::min<long long>(64 * w + 64, starts[b + 1]); ++c)
-                off[c] = (long long)acc + (c % 64);
-            acc += rows[w] * 64;
+        for (long long c = starts[b]; c < starts[b + 1]; ++c) {
+            off[c] = (long long)acc;
+            acc += (size_t)cap[c];
         }
         batch_nodes.push_back(acc);
         layout(starts[b + 1] - starts[b], acc, o);

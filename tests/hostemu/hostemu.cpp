@@ -14,6 +14,9 @@ using namespace uwshcu;
 // 1: run the level-parallel set-up (setup_level1/2 over every column and level)
 // before the columns, as the GPU pipeline does; 0: the columns do it themselves.
 static int g_staged = 0;
+// backward-sweep window (entries) used by hostemu_adjoint; 0 = none
+static int g_window = 0;
+extern "C" void hostemu_set_window(int w) { g_window = w; }
 extern "C" void hostemu_set_staged(int on) { g_staged = on; }
 
 template <class R>
@@ -140,7 +143,14 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
             const Column<AD, AD> col(a, T, i, i);
             col.run();
             col.ad_export(nodes.data());
-            ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
+            if (g_window > 0) {
+                std::vector<double> wv((size_t)g_window);
+                auto win = [&](int q) -> double& { return wv[(size_t)q]; };
+                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g,
+                           win, g_window);
+            } else {
+                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
+            }
             tape_len[i] = tapes[i].overflow ? -1 : tapes[i].n;
         }
     }

@@ -107,7 +107,7 @@ OUT_ORDER = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "
              "cnt", "cnb")
 
 
-def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False):
+def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False, window=0):
     """Host adjoint (tape) of the FP64 column code.
 
     cot: dict output-name -> cotangent array (same shape as the output; missing = 0).
@@ -143,7 +143,9 @@ def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False):
                                                            _p(out["wu"]), cotp, gp, C.c_int(cap), _p(tlen), _p(diag)]
     L_ = lib(checked)
     L_.hostemu_adjoint.restype = C.c_int
+    L_.hostemu_set_window(int(window))
     rc = L_.hostemu_adjoint(*args)
+    L_.hostemu_set_window(0)
     assert rc == 0, rc
     out["cush"] = cush
     if ncnst == 0:

@@ -112,3 +112,19 @@ def test_tape_overflow_is_reported():
     ch = make_chunk(64, 35, 3, seed=45)
     _, _, tlen, diag = hostemu.adjoint(ch, {}, cap=2000)
     assert (tlen[diag[:, 7] > 0] == -1).all()
+
+
+@pytest.mark.parametrize("window", [1, 7, 64])
+def test_windowed_sweep_is_bit_identical(window):
+    """The device sweep keeps recent adjoints in a window; each adjoint still sums its
+    contributions in decreasing node order, so the gradients equal the plain sweep's."""
+    ch = make_chunk(256, 35, 3, seed=47)
+    ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    rng = np.random.default_rng(5)
+    cot = {k: rng.normal(0, 1, np.shape(ref[k])) for k in ("sten", "qvten", "umf", "precip")}
+    _, g0, _, _ = hostemu.adjoint(ch, cot)
+    _, g1, _, _ = hostemu.adjoint(ch, cot, window=window)
+    _, g2, _, _ = hostemu.adjoint(ch, cot, window=window, checked=True)
+    for k in g0:
+        np.testing.assert_array_equal(g1[k], g0[k], err_msg=k)
+        np.testing.assert_array_equal(g2[k], g0[k], err_msg=k)
