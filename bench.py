@@ -128,15 +128,14 @@ def main():
         grads["tke"] = torch.empty((args.pver + 1, args.ncol), dtype=torch.float64, device=dev)
         grads["pblh"] = torch.empty(args.ncol, dtype=torch.float64, device=dev)
         grads["par_sum"] = torch.empty(3, dtype=torch.float64, device=dev)
-        aout = ctx.alloc_outputs(args.ncol, args.pver, args.ncnst)
-        acush = torch.empty_like(cush)
         cpair = 1.00464e3
 
         def step():
             fwd(_capi.FP64)
             cot = {"sten": 2.0 * (out["sten"] / cpair - tau_t) / cpair, "qvten": 2.0 * (out["qvten"] - tau_q)}
-            acush.copy_(inp["cush"])
-            ctx.adjoint_into(inp, aout, acush, cot, grads, dt=ch.dt, stream=stream)
+            # the adjoint rewrites the forward's outputs (with the same values)
+            cush.copy_(inp["cush"])
+            ctx.adjoint_into(inp, out, cush, cot, grads, dt=ch.dt, stream=stream)
         per_step_ms, wall = timed(lambda: fwd(_capi.FP64), args.steps, args.warmup, stream)
         fwd_ms = float(np.mean(per_step_ms))
         per_step_ms, wall = timed(step, args.steps, args.warmup, stream)
