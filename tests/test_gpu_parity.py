@@ -102,6 +102,12 @@ def test_72_levels_10_constituents():
     _check(make_chunk(4096, 72, 10, seed=2, ptop=100.0), "4096x72x10")
 
 
+def test_launch_large_enough_for_the_two_wavefront_kernel():
+    """From 131072 columns on, the column kernel is the build that lets two wavefronts
+    share a SIMD; its results must be the same bits."""
+    _check(make_chunk(163840, 35, 3, seed=9), "163840x35x3")
+
+
 def test_convecting_fraction_sweep():
     for inv in (0.9, 0.5, 0.05):
         _check(make_chunk(8192, 35, 3, seed=7, inversion_frac=inv), f"8192x35x3 capped {inv}")
