@@ -47,6 +47,9 @@ struct uwshcu_ctx {
     size_t parcol_cols = 0;
     int* d_counts = nullptr;
     size_t count_cols = 0;
+    // columns whose second convection pass runs in a launch of its own
+    int* d_defer = nullptr;
+    size_t defer_cols = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
 };
@@ -158,6 +161,17 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     int rc = ensure_ws(ctx, need);
     if (rc) return rc;
     if (recompute && (rc = ensure_mixed(ctx, (size_t)ncol))) return rc;
+    // defer list (nb entries and a counter) for large launches
+    if (nb >= 2 * 1024 * 64 && (size_t)nb + 1 > ctx->defer_cols) {
+        if (ctx->d_defer) {
+            if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+            (void)hipFree(ctx->d_defer);
+            ctx->d_defer = nullptr;
+            ctx->defer_cols = 0;
+        }
+        if (!hip_ok(hipMalloc((void**)&ctx->d_defer, ((size_t)nb + 1) * sizeof(int)))) return UWSHCU_ERR_NOMEM;
+        ctx->defer_cols = (size_t)nb + 1;
+    }
 
     FwdArgs a{};
     a.ncol = ncol;
@@ -178,6 +192,10 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     a.diag = diag;
     a.err = ctx->d_err;
     a.ws = ctx->d_ws;
+    if (ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
+        a.defer_list = ctx->d_defer + 1;
+        a.defer_count = ctx->d_defer;
+    }
     if (recompute) {
         // the FP32 pass overwrites cush; the FP64 recompute reads the input value
         if (!hip_ok(hipMemcpyAsync(ctx->d_cush_in, ch.cush, (size_t)ncol * sizeof(double), hipMemcpyDeviceToDevice,
@@ -497,6 +515,7 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_parcol) (void)hipFree(ctx->d_parcol);
     if (ctx->d_counts) (void)hipFree(ctx->d_counts);
+    if (ctx->d_defer) (void)hipFree(ctx->d_defer);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return UWSHCU_OK;

@@ -90,6 +90,10 @@ struct FwdArgs {
     const double* cush_in;   // optional: read the input cush here instead of from cush
     int* amb_flag;           // optional [ldc]: 1 where an FP32 branch decision was within rounding
     int env_ready;           // 1: setup_level1/2 already filled the set-A environment and the wet bulb
+    // optional: a column whose implicit-CIN pass needs a second convection pass stops
+    // there, leaves its outputs to be rewritten and appends itself to defer_list
+    int* defer_list;
+    int* defer_count;
     AdArgs ad;               // adjoint kernel only
 };
 
@@ -606,6 +610,10 @@ UWSHCU_HD void Column<R, D>::run() const {
             const D del_CIN = cin - cin_i;
             if (cin_i + cin > 0.0) near(del_CIN, cin_i + cin);
             if (del_CIN > 0.0) {
+                if (a.defer_list) {
+                    a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
+                    return;
+                }
                 const D alpha = compute_alpha(del_CIN, ke);
                 cin = cin_i + alpha * del_CIN;
                 cinlcl = cinlcl_i;

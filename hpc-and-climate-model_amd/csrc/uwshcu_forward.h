@@ -70,23 +70,58 @@ __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const u
     else c.setup_level2(k);
 }
 
+static __global__ void clear_count_kernel(int* count) {
+    if (threadIdx.x == 0) *count = 0;
+}
+
 // The forward pass of columns [col0, col0 + n) (or of the mapped columns): the two
 // level-parallel set-up stages, then the column kernel.
+//
+// Large launches (at least two wavefronts per SIMD) also split off the columns whose
+// implicit-CIN pass convects a second time (a few per cent of them): the column
+// kernel stops those at that point and lists them (defer_list), and a second column
+// kernel reruns the listed columns from the start.  Otherwise nearly every wavefront
+// would carry one or two such columns through a second convection pass while its
+// other lanes idle.
+//
+// args0.defer_list / defer_count (nullable) give the list's storage (n entries).
 template <class R>
 hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_consts* dconsts, const double* destbl,
                                   hipStream_t stream) {
+    int* const defer_list = args0.defer_list;
+    int* const defer_count = args0.defer_count;
     FwdArgs args = args0;
     args.env_ready = 1;
+    args.defer_list = nullptr;
+    args.defer_count = nullptr;
     const long long t1 = (long long)n * (args.mkx + 1), t2 = (long long)n * args.mkx;
     hipLaunchKernelGGL((setup_kernel<R, 1>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
     hipLaunchKernelGGL((setup_kernel<R, 2>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
+    const bool large = n >= 2 * 1024 * 64;
+    const bool split = large && defer_list && defer_count && !args.colmap;
+    if (split) {
+        hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, stream, defer_count);
+        args.defer_list = defer_list;
+        args.defer_count = defer_count;
+    }
     // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build
-    if (n >= 2 * 1024 * 64)
+    if (large)
         hipLaunchKernelGGL(forward_kernel_occ2<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     else
         hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+    if (split) {
+        // rerun the listed columns whole; they set up their own environment in their slots
+        FwdArgs b = args0;
+        b.env_ready = 0;
+        b.defer_list = nullptr;
+        b.defer_count = nullptr;
+        b.colmap = defer_list;
+        b.colcount = defer_count;
+        b.col0 = 0;
+        hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts, destbl);
+    }
     return hipGetLastError();
 }
 

@@ -14,6 +14,10 @@ using namespace uwshcu;
 // 1: run the level-parallel set-up (setup_level1/2 over every column and level)
 // before the columns, as the GPU pipeline does; 0: the columns do it themselves.
 static int g_staged = 0;
+// 1: as the GPU pipeline does for large launches, stop columns that need a second
+// convection pass, list them and rerun them whole afterwards
+static int g_defer = 0;
+extern "C" void hostemu_set_defer(int on) { g_defer = on; }
 // backward-sweep window (entries) used by hostemu_adjoint; 0 = none
 static int g_window = 0;
 extern "C" void hostemu_set_window(int w) { g_window = w; }
@@ -30,9 +34,21 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
         for (int i = 0; i < ncol; ++i)
             for (int k = 1; k <= pver; ++k) Column<R>(a, T, i, i).setup_level2(k);
     }
+    std::vector<int> deferred((size_t)ncol);
+    int ndefer = 0;
+    if (g_defer) {
+        a.defer_list = deferred.data();
+        a.defer_count = &ndefer;
+    }
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ncol; ++i) Column<R>(a, T, i, i).run();
     a.env_ready = 0;
+    a.defer_list = nullptr;
+    a.defer_count = nullptr;
+    if (g_defer) {
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < ndefer; ++j) Column<R>(a, T, deferred[j], j).run();
+    }
 }
 
 extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int precision, const uwshcu_params* par,

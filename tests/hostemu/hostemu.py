@@ -57,8 +57,13 @@ def _p(a):
 
 
 def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False, stats=None,
-        staged=False):
+        staged=False, defer=False):
     """Run the host-compiled column code on a generator chunk.
+
+    ``staged`` runs the level-parallel set-up first, as the GPU pipeline does;
+    ``defer`` stops the columns that need a second convection pass and reruns them
+    afterwards from a list, as the GPU pipeline does for launches of 131072 columns
+    and more.
 
     Returns (outputs dict, diag [ncol, NDIAG], rc, bad_col) like ``oracle.run``.
     The constants and table come from the product library's host functions, which
@@ -89,9 +94,12 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
                                   "qcu", "qlu", "qiu", "cbmf", "qc", "rliq", "cnt", "cnb")]
     nfix = C.c_int(0)
     args += [_p(np.ascontiguousarray(f["dpdry0"])), _p(out["wu"]), _p(diag), C.byref(bad), C.byref(nfix)]
-    lib(checked).hostemu_set_staged(1 if staged else 0)
-    rc = lib(checked).hostemu_forward(*args)
-    lib(checked).hostemu_set_staged(0)
+    L = lib(checked)
+    L.hostemu_set_staged(1 if staged else 0)
+    L.hostemu_set_defer(1 if defer else 0)
+    rc = L.hostemu_forward(*args)
+    L.hostemu_set_staged(0)
+    L.hostemu_set_defer(0)
     if stats is not None:
         stats["recomputed"] = nfix.value
     out["cush"] = cush

@@ -43,6 +43,21 @@ def test_level_parallel_setup_gives_identical_results(ncol, pver, ncnst, precisi
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("ncol,pver,ncnst,inv", [(1024, 35, 3, 0.5), (256, 72, 10, 0.3)])
+def test_deferred_second_pass_columns_give_identical_results(ncol, pver, ncnst, inv, precision):
+    """Large GPU launches stop the columns whose implicit-CIN pass convects a second
+    time, list them and rerun them whole in a second launch.  The outputs and the
+    integer diagnostics must equal the single-launch run bit for bit."""
+    ch = make_chunk(ncol, pver, ncnst, seed=90 + pver, inversion_frac=inv, ptop=100.0 if pver > 60 else 200.0)
+    a, ad, _, _ = hostemu.run(ch, precision=precision, staged=True)
+    b, bd, _, _ = hostemu.run(ch, precision=precision, staged=True, defer=True)
+    assert (ad[:, 7] == 2).any(), "no column took the second pass"
+    np.testing.assert_array_equal(ad, bd)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_fp64_column_code_with_other_tunables():
     from uwshcu_amd import _capi
     ch = make_chunk(256, 30, 3, seed=21)
@@ -118,9 +133,10 @@ def test_libm_sensitivity_at_72_levels_and_10_constituents():
 @pytest.mark.parametrize("ncol,pver,ncnst,inv", [(192, 30, 3, 0.25), (128, 72, 10, 0.2), (128, 35, 0, 0.1)])
 def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision, staged):
     """Build with every array and workspace index asserted (an assertion aborts the
-    process), and check the results equal the unchecked build's."""
+    process), and check the results equal the unchecked build's.  The staged runs
+    also defer the second-pass columns, as large GPU launches do."""
     ch = make_chunk(ncol, pver, ncnst, seed=pver + ncnst, inversion_frac=inv, ptop=100.0 if pver > 60 else 200.0)
-    a, ad, _, _ = hostemu.run(ch, precision=precision, checked=True, staged=staged)
+    a, ad, _, _ = hostemu.run(ch, precision=precision, checked=True, staged=staged, defer=staged)
     b, bd, _, _ = hostemu.run(ch, precision=precision)
     np.testing.assert_array_equal(ad, bd)
     for k in a:
