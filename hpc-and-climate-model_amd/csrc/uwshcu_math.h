@@ -133,5 +133,39 @@ UWSHCU_HD inline float ulog(float x) { return logf(x); }
 UWSHCU_HD inline float upow(float x, float y) { return powf(x, y); }
 UWSHCU_HD inline float uerfc(float x) { return erfcf(x); }
 
+// a / b, correctly rounded, from y = 1.0 / b (itself correctly rounded): q0 = a*y is
+// within one ulp of a/b, the remainder a - b*q0 is exact in one fused multiply-add,
+// and q0 + remainder*y rounds to a/b (Markstein, IBM J. Res. Dev. 34(1), 1990, the
+// theorem behind fused-multiply-add division).  So a divisor that recurs pays for one
+// division, and each quotient costs three dependent operations instead of a full
+// division sequence, with identical bits.  Valid for finite a, finite nonzero b and
+// |a| above 2^-900 or zero (no underflow in the remainder); a = -0 returns +0.
+UWSHCU_HD inline double div_rcp(double a, double b, double y) {
+    const double q0 = a * y;
+    return fma(fma(-b, q0, a), y, q0);
+}
+
+// a / b on the GPU by the operations of the hardware's IEEE FP64 division without its
+// range scaling (v_div_scale) and special-value fix-up (v_div_fixup): reciprocal
+// estimate, two Newton steps, quotient, exact remainder and the final fused
+// correction.  When neither operand nor the quotient is near the ends of the
+// exponent range (|exponents| below 900, no zero, infinite or NaN divisor) the
+// scaling steps are identities and the fix-up returns its input, so the result is the
+// correctly rounded quotient, the bits of `a / b`; two dependent steps shorter.  Host
+// builds use the operator itself.
+UWSHCU_HD inline double div_fast(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+#else
+    return a / b;
+#endif
+}
+
 }  // namespace uwshcu
 #endif

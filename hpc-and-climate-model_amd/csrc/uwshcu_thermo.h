@@ -90,6 +90,7 @@ struct ThermoT {
             if (conden_ift(p, thl, qt, th, qv, ql, qi, id)) return id;
         }
 #endif
+        if constexpr (!is_ad<D>::value) return conden_d(p, ex, thl, qt, th, qv, ql, qi);
         const D tc = thl * ex;
         const D nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
         const D leff = (1.0 - nu) * c->xlv + nu * c->xls;
@@ -115,6 +116,65 @@ struct ThermoT {
         qi = nu * qc;
         th = temps / ex;
         return (fabs((temps - (leff / c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
+    }
+
+    // conden_x for D = double: the same operations, each quotient computed so that it
+    // has the bits of the plain expression's.
+    //  - Divisions by leff, r and (twice) temps go through correctly rounded
+    //    reciprocals (div_rcp, three dependent operations each).
+    //  - The others, and the reciprocals, use div_fast (no range scaling; every
+    //    operand is a temperature, pressure, latent heat or mixing ratio).
+    //  - The Newton loop keeps the two table entries of the last lookup in registers
+    //    and reads the table again only when the index moves, which after the first
+    //    steps it rarely does.
+    UWSHCU_HD int conden_d(double p, double ex, double thl, double qt, double& th, double& qv, double& ql,
+                           double& qi) const {
+        const double omeps = 1.0 - c->epsqs;
+        int ic = -1;
+        double tlo = 0.0, thi = 0.0;
+        auto qsat_c = [&](double t, double& es_, double& qs_) {   // qsat_es_qs with the cached entries
+            const double e = fmax(fmin(t, c->tmax), c->tmin);
+            const int i = (int)(e - c->tmin);
+            if (i != ic) {
+                ic = i;
+                tlo = estbl[i];
+                thi = estbl[i + 1];
+            }
+            const double ai = (double)i;
+            es_ = (c->tmin + ai - e + 1.0) * tlo - (c->tmin + ai - e) * thi;
+            qs_ = div_fast(c->epsqs * es_, p - omeps * es_);
+            qs_ = fmin(1.0, qs_);
+            if (qs_ < 0.0) { qs_ = 1.0; es_ = p; }
+        };
+        const double tc = thl * ex;
+        const double nu = fmax(fmin(div_fast(268.0 - tc, 20.0), 1.0), 0.0);
+        const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
+        double temps = tc, es, qs;
+        qsat_c(temps, es, qs);
+        double rvls = qs;
+        if (qs >= qt) {
+            qv = qt;
+            ql = 0.0;
+            qi = 0.0;
+            th = div_fast(tc, ex);
+            return 0;
+        }
+        const double rleff = div_fast(1.0, leff), rr = div_fast(1.0, c->r);
+        const double cpl = div_rcp(c->cp, leff, rleff);
+        for (int it = 0; it < 10; ++it) {
+            const double rt = div_fast(1.0, temps);
+            const double num = div_rcp((tc - temps) * c->cp, leff, rleff) + qt - rvls;
+            const double d3 = div_rcp(div_rcp(div_rcp(c->ep2 * leff * rvls, c->r, rr), temps, rt), temps, rt);
+            temps = temps + div_fast(num, cpl + d3);
+            qsat_c(temps, es, qs);
+            rvls = qs;
+        }
+        const double qc = fmax(qt - qs, 0.0);
+        qv = qt - qc;
+        ql = qc * (1.0 - nu);
+        qi = nu * qc;
+        th = div_fast(temps, ex);
+        return (fabs((temps - div_fast(leff, c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
     }
 
     // Adjoint form of conden (D = AD): the values are the FP64 routine's, computed by

@@ -80,14 +80,50 @@ def test_portable_math_is_bit_identical_on_gpu_and_host():
     n = 1 << 18
     x = np.concatenate([rng.uniform(-40, 40, n // 2), rng.uniform(-3, 6, n // 2)])
     y = np.exp(rng.uniform(-30, 30, n))
-    go, ho = np.zeros((n, 8)), np.zeros((n, 8))
-    pd = lambda a: a.ctypes.data_as(C.c_void_p)
-    assert L.probe_gpu(n, pd(x), pd(y), pd(go)) == 0
-    assert L.probe_host(n, pd(x), pd(y), pd(ho)) == 0
-    names = ("exp", "log", "pow", "erfc", "sqrt", "div", "fmax-fmin", "floor")
+    go, ho = _probe(L, x, y)
+    names = ("exp", "log", "pow", "erfc", "sqrt", "div", "fmax-fmin", "floor", "div_fast", "div_rcp")
     for j, nm in enumerate(names):
         diff = np.flatnonzero(go[:, j].view(np.int64) != ho[:, j].view(np.int64))
         assert diff.size == 0, f"{nm}: {diff.size} inputs differ, e.g. x={x[diff[0]]!r} y={y[diff[0]]!r}"
+
+
+def _probe(L, x, y):
+    n = x.size
+    go, ho = np.zeros((n, 10)), np.zeros((n, 10))
+    pd = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert L.probe_gpu(n, pd(x), pd(y), pd(go)) == 0
+    assert L.probe_host(n, pd(x), pd(y), pd(ho)) == 0
+    return go, ho
+
+
+def test_division_without_range_scaling_is_the_ieee_quotient():
+    """div_fast (the hardware division sequence without range scaling) and div_rcp
+    (quotient from a correctly rounded reciprocal plus one fused correction) must give
+    the bits of x / y on the GPU: random significands, significands next to all-ones,
+    short significands and quotients next to representable numbers, exponents within
+    +-300."""
+    import torch
+    assert torch.cuda.is_available()
+    L = C.CDLL(os.path.join(HERE, "hostemu", "libmathprobe.so"))
+    rng = np.random.default_rng(12)
+    n = 1 << 21
+    mant = lambda: rng.integers(0, 1 << 52, n, dtype=np.uint64)
+    expo = lambda: (rng.integers(-300, 301, n) + 1023).astype(np.uint64) << np.uint64(52)
+    x = (mant() | expo()).view(np.float64).copy()
+    y = (mant() | expo()).view(np.float64).copy()
+    q = n // 4
+    all_ones = np.uint64((1 << 52) - 1)
+    y[:q] = ((y[:q].view(np.uint64) | all_ones) ^ rng.integers(0, 256, q, dtype=np.uint64)).view(np.float64)
+    x[q:2 * q] = ((x[q:2 * q].view(np.uint64) | all_ones) ^ rng.integers(0, 256, q, dtype=np.uint64)).view(np.float64)
+    y[2 * q:3 * q] = ((y[2 * q:3 * q].view(np.uint64) >> np.uint64(36)) << np.uint64(36)).view(np.float64)
+    x[3 * q:] = np.nextafter(rng.uniform(1, 2, n - 3 * q) * y[3 * q:], np.where(rng.random(n - 3 * q) < 0.5, np.inf, -np.inf))
+    x[rng.random(n) < 0.5] *= -1.0
+    go, ho = _probe(L, x, y)
+    exact = x / y
+    for j, nm in ((5, "div"), (8, "div_fast"), (9, "div_rcp")):
+        for label, got in (("gpu", go[:, j]), ("host", ho[:, j])):
+            diff = np.flatnonzero(got.view(np.int64) != exact.view(np.int64))
+            assert diff.size == 0, f"{nm} on the {label}: {diff.size} quotients differ, e.g. {x[diff[0]]!r} / {y[diff[0]]!r}"
 
 
 def test_c1_chunk_16x30():
