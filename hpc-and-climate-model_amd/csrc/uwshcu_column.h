@@ -338,7 +338,10 @@ struct Column {
         cout(a.cnb) = D(mk + 1) - D(mk); // cnb = mkx flipped
     }
 
-    UWSHCU_HD void run() const;
+    UWSHCU_HD void run() const { run_body(); }
+    // the column pass itself, expanded into each forward kernel so that each gets its
+    // own register allocation (the two builds differ in their register budget)
+    UWSHCU_HD __attribute__((always_inline)) void run_body() const;
 
     // Level-parallel set-up (one thread per column and level), the part of S0 and of
     // the wet bulb that needs no sequential sweep.  Stage 1, level k = 0..mkx: Exner
@@ -411,7 +414,7 @@ struct Column {
 };
 
 template <class R, class D>
-UWSHCU_HD void Column<R, D>::run() const {
+UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);

@@ -26,7 +26,7 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
     const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
     const Thermo T{consts, s_estbl};
     const Column<R> c(args, T, col, j);
-    c.run();
+    c.run_body();
 }
 
 // Two register budgets for the same column code.  forward_kernel lets it use all 512

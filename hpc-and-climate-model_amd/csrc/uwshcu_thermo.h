@@ -82,15 +82,20 @@ struct ThermoT {
         return conden_x(p, exnf(p), thl, qt, th, qv, ql, qi);
     }
     // The same with the Exner function of p supplied by the caller (ex must equal
-    // exnf(p); callers pass the set-A values of the model levels).
-    UWSHCU_HD UWSHCU_OUTLINE int conden_x(D p, D ex, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
+    // exnf(p); callers pass the set-A values of the model levels).  The FP64 form is
+    // expanded at each call site: an out-of-line call would pass the outputs through
+    // scratch memory and reload the constants at every call.  The recording (AD) form
+    // stays one out-of-line copy.
+    UWSHCU_HD __attribute__((always_inline)) int conden_x(D p, D ex, D thl, D qt, D& th, D& qv, D& ql,
+                                                         D& qi) const {
+        if constexpr (is_ad<D>::value) return conden_ad(p, ex, thl, qt, th, qv, ql, qi);
+        else return conden_d(p, ex, thl, qt, th, qv, ql, qi);
+    }
+    UWSHCU_HD UWSHCU_OUTLINE int conden_ad(D p, D ex, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
 #ifndef UWSHCU_AD_TAPE_CONDEN
-        if constexpr (is_ad<D>::value) {
-            int id;
-            if (conden_ift(p, thl, qt, th, qv, ql, qi, id)) return id;
-        }
+        int id;
+        if (conden_ift(p, thl, qt, th, qv, ql, qi, id)) return id;
 #endif
-        if constexpr (!is_ad<D>::value) return conden_d(p, ex, thl, qt, th, qv, ql, qi);
         const D tc = thl * ex;
         const D nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
         const D leff = (1.0 - nu) * c->xlv + nu * c->xls;
@@ -127,8 +132,8 @@ struct ThermoT {
     //  - The Newton loop keeps the two table entries of the last lookup in registers
     //    and reads the table again only when the index moves, which after the first
     //    steps it rarely does.
-    UWSHCU_HD int conden_d(double p, double ex, double thl, double qt, double& th, double& qv, double& ql,
-                           double& qi) const {
+    UWSHCU_HD __attribute__((always_inline)) int conden_d(double p, double ex, double thl, double qt, double& th,
+                                                         double& qv, double& ql, double& qi) const {
         const double omeps = 1.0 - c->epsqs;
         int ic = -1;
         double tlo = 0.0, thi = 0.0;
