@@ -63,7 +63,22 @@ def parse():
                          "(for the multi-million-column workloads; columns are independent)")
     ap.add_argument("--workspace-gib", type=float, default=None,
                     help="context workspace limit (default: the library's 24 GiB; the adjoint uses 160 GiB)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer three times: the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Three device-to-device copies of 256 MiB (32 Mi doubles), nothing else."""
+    import torch
+    n = 32 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_copy": n * 8, "copies": 3}), flush=True)
 
 
 def timed(step, steps, warmup, stream):
@@ -90,6 +105,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the column physics shards by columns; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
 

@@ -49,6 +49,7 @@ struct uwshcu_ctx {
     size_t count_cols = 0;
     // columns whose second convection pass runs in a launch of its own
     int* d_defer = nullptr;
+    double* d_defer_state = nullptr;   // NRESUME x defer_cols doubles: their saved scalars
     size_t defer_cols = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -161,15 +162,23 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     int rc = ensure_ws(ctx, need);
     if (rc) return rc;
     if (recompute && (rc = ensure_mixed(ctx, (size_t)ncol))) return rc;
-    // defer list (nb entries and a counter) for large launches
+    // defer list (nb entries and a counter) and the deferred columns' scalars, for
+    // large launches
     if (nb >= 2 * 1024 * 64 && (size_t)nb + 1 > ctx->defer_cols) {
         if (ctx->d_defer) {
             if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
             (void)hipFree(ctx->d_defer);
+            (void)hipFree(ctx->d_defer_state);
             ctx->d_defer = nullptr;
+            ctx->d_defer_state = nullptr;
             ctx->defer_cols = 0;
         }
         if (!hip_ok(hipMalloc((void**)&ctx->d_defer, ((size_t)nb + 1) * sizeof(int)))) return UWSHCU_ERR_NOMEM;
+        if (!hip_ok(hipMalloc((void**)&ctx->d_defer_state, (size_t)uwshcu::NRESUME * ((size_t)nb + 1) * sizeof(double)))) {
+            (void)hipFree(ctx->d_defer);
+            ctx->d_defer = nullptr;
+            return UWSHCU_ERR_NOMEM;
+        }
         ctx->defer_cols = (size_t)nb + 1;
     }
 
@@ -195,6 +204,7 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     if (ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
         a.defer_list = ctx->d_defer + 1;
         a.defer_count = ctx->d_defer;
+        a.defer_state = ctx->d_defer_state;   // [field][slot], a.wld = nb apart
     }
     if (recompute) {
         // the FP32 pass overwrites cush; the FP64 recompute reads the input value
@@ -516,6 +526,7 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_parcol) (void)hipFree(ctx->d_parcol);
     if (ctx->d_counts) (void)hipFree(ctx->d_counts);
     if (ctx->d_defer) (void)hipFree(ctx->d_defer);
+    if (ctx->d_defer_state) (void)hipFree(ctx->d_defer_state);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return UWSHCU_OK;

@@ -91,9 +91,15 @@ struct FwdArgs {
     int* amb_flag;           // optional [ldc]: 1 where an FP32 branch decision was within rounding
     int env_ready;           // 1: setup_level1/2 already filled the set-A environment and the wet bulb
     // optional: a column whose implicit-CIN pass needs a second convection pass stops
-    // there, leaves its outputs to be rewritten and appends itself to defer_list
+    // there, saves its scalars (NRESUME doubles, [field][slot], wld apart) in
+    // defer_state, leaves its outputs to be rewritten and appends itself to defer_list
     int* defer_list;
     int* defer_count;
+    double* defer_state;
+    // 1: continue each listed column (colmap) from that point, in the workspace slot it
+    // had in the deferring launch (column - slot0), whose set-A environment is intact
+    int resume;
+    int slot0;
     AdArgs ad;               // adjoint kernel only
 };
 
@@ -115,6 +121,9 @@ enum AdInput : int { ADI_T0, ADI_QV0, ADI_U0, ADI_V0, ADI_TKE, ADI_PARAMS };
 UWSHCU_HD inline int ad_inputs(int mkx) { return ADI_PARAMS * (mkx + 1) + 3; }
 
 // Exit reasons reported in diag row 0 (same numbering as the oracle).
+// Scalars of a column deferred at its second convection pass (FwdArgs::defer_state).
+enum { NRESUME = 40 };
+
 enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
 
@@ -138,6 +147,11 @@ struct Column {
                                                                     (size_t)args.wld)),
           tape(own_tape ? own_tape : args.ad.tapes ? args.ad.tapes + slot : nullptr) {}
 
+    // saved scalar f of a deferred column (FwdArgs::defer_state)
+    UWSHCU_HD inline double& rstate(int f) const {
+        UW_CHECK(f >= 0 && f < NRESUME && iw >= 0 && (size_t)iw < wld && a.defer_state);
+        return a.defer_state[(size_t)f * wld + iw];
+    }
     // caller arrays: UW level k <-> CAM row mkx-k (layers k=1..mkx and interfaces k=0..mkx)
     UWSHCU_HD inline size_t row(int k) const { return (size_t)(mkx - k) * ldc + i; }
 #ifdef UWSHCU_BOUNDS_CHECK
@@ -491,6 +505,21 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
     R trsrc_tmp = 0;
     (void)trsrc_tmp;
 
+    // The scalars live across the split of a deferred column: every function-scope
+    // local that the second convection pass reads or carries to the end of the column.
+    // FP32 values (the mixed path) are held exactly as doubles.
+    auto resume_io = [&](auto&& io) {
+        io(0, cush); io(1, cin); io(2, cinlcl); io(3, plcl); io(4, plfc); io(5, tkeavg); io(6, thvlmin);
+        io(7, qtsrc); io(8, thvlsrc); io(9, thlsrc); io(10, usrc); io(11, vsrc); io(12, thv0lcl); io(13, prel);
+        io(14, thv0rel); io(15, cbmf); io(16, wlcl); io(17, ufrclcl); io(18, ppen); io(19, thlu_top);
+        io(20, qtu_top); io(21, rho0j); io(22, drage); io(23, bogbot); io(24, bogtop); io(25, wtwb); io(26, th);
+        io(27, qvj); io(28, qlj); io(29, qij); io(30, kinv); io(31, klcl); io(32, klfc); io(33, krel);
+        io(34, kbup); io(35, kpen); io(36, limit_ufrc); io(37, limit_ppen); io(38, limit_emf); io(39, amb);
+    };
+    if (a.resume) {
+        resume_io([&](int f, auto& x) { x = decltype(+x)(rstate(f)); });
+    }
+
     if (a.env_ready) {
         for (int k = 1; k <= mkx; ++k)
             if (E(A_THV0BOT, k) < 0.0) { exit_code = EX_CONDEN; goto L333; }
@@ -499,7 +528,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
         goto L333;
     }
 
-    for (int iter = 1; iter <= 2; ++iter) {
+    for (int iter = a.resume ? 2 : 1; iter <= 2; ++iter) {
         exit_iter = iter;
         const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
@@ -508,6 +537,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
         const D tscaleh = cush;
         cush = -1.0;
 
+        if (a.resume) {
+            // a deferred column continues here with the scalars of its implicit-CIN restore
+            zero_pass_state();
+        } else {
         // ---- S1 kinv (uwshcu.F90:1455-1468) ----
         kinv = 1;
         for (int k = mkx - 1; k >= 1; --k) {
@@ -613,16 +646,22 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
             const D del_CIN = cin - cin_i;
             if (cin_i + cin > 0.0) near(del_CIN, cin_i + cin);
             if (del_CIN > 0.0) {
-                if (a.defer_list) {
-                    a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
-                    return;
-                }
                 const D alpha = compute_alpha(del_CIN, ke);
                 cin = cin_i + alpha * del_CIN;
                 cinlcl = cinlcl_i;
                 kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
                 thvlmin = thvlmin_o; qtsrc = qtsrc_o; thvlsrc = thvlsrc_o; thlsrc = thlsrc_o; usrc = usrc_o;
                 vsrc = vsrc_o; thv0lcl = thv0lcl_o;
+                if (a.defer_list) {
+                    // the second pass runs in a launch of its own: save, list, stop
+                    cush = tscaleh;
+                    resume_io([&](int f, auto& x) {
+                        if constexpr (std::is_integral<std::remove_reference_t<decltype(x)>>::value) rstate(f) = (double)x;
+                        else rstate(f) = (double)val(x);
+                    });
+                    a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
+                    return;
+                }
                 zero_pass_state();
             } else {
                 // iteration-1 outputs stand (uwshcu.F90:1897-2010): they are already in the
@@ -632,6 +671,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
                 goto Ldone;
             }
         }
+        }   // S1-S7
 
         // From here on the environment is set A (the input state).
         {

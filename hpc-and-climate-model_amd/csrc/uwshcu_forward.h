@@ -24,8 +24,9 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nvalid) return;
     const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
+    const int slot = args.resume ? col - args.slot0 : j;
     const Thermo T{consts, s_estbl};
-    const Column<R> c(args, T, col, j);
+    const Column<R> c(args, T, col, slot);
     c.run_body();
 }
 
@@ -79,12 +80,14 @@ static __global__ void clear_count_kernel(int* count) {
 //
 // Large launches (at least two wavefronts per SIMD) also split off the columns whose
 // implicit-CIN pass convects a second time (a few per cent of them): the column
-// kernel stops those at that point and lists them (defer_list), and a second column
-// kernel reruns the listed columns from the start.  Otherwise nearly every wavefront
-// would carry one or two such columns through a second convection pass while its
-// other lanes idle.
+// kernel stops each of those where its second convection pass begins, saves its
+// scalars and lists it (defer_list, defer_state), and a second column kernel
+// continues the listed columns from there in their own workspace slots.  Otherwise
+// nearly every wavefront would carry one or two such columns through a second
+// convection pass while its other lanes idle.
 //
-// args0.defer_list / defer_count (nullable) give the list's storage (n entries).
+// args0.defer_list / defer_count / defer_state (nullable) give the list's storage
+// (n entries, NRESUME x wld doubles).
 template <class R>
 hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_consts* dconsts, const double* destbl,
                                   hipStream_t stream) {
@@ -92,6 +95,7 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     int* const defer_count = args0.defer_count;
     FwdArgs args = args0;
     args.env_ready = 1;
+    args.resume = 0;
     args.defer_list = nullptr;
     args.defer_count = nullptr;
     const long long t1 = (long long)n * (args.mkx + 1), t2 = (long long)n * args.mkx;
@@ -100,7 +104,7 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     hipLaunchKernelGGL((setup_kernel<R, 2>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
     const bool large = n >= 2 * 1024 * 64;
-    const bool split = large && defer_list && defer_count && !args.colmap;
+    const bool split = large && defer_list && defer_count && args0.defer_state && !args.colmap;
     if (split) {
         hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, stream, defer_count);
         args.defer_list = defer_list;
@@ -112,9 +116,11 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     else
         hipLaunchKernelGGL(forward_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     if (split) {
-        // rerun the listed columns whole; they set up their own environment in their slots
+        // continue the listed columns at their second convection pass
         FwdArgs b = args0;
-        b.env_ready = 0;
+        b.env_ready = 1;
+        b.resume = 1;
+        b.slot0 = args0.col0;
         b.defer_list = nullptr;
         b.defer_count = nullptr;
         b.colmap = defer_list;

@@ -17,7 +17,9 @@ static int g_staged = 0;
 // 1: as the GPU pipeline does for large launches, stop columns that need a second
 // convection pass, list them and rerun them whole afterwards
 static int g_defer = 0;
+static int g_deferred = 0;   // columns the last deferring run continued in its second launch
 extern "C" void hostemu_set_defer(int on) { g_defer = on; }
+extern "C" int hostemu_last_deferred() { return g_deferred; }
 // backward-sweep window (entries) used by hostemu_adjoint; 0 = none
 static int g_window = 0;
 extern "C" void hostemu_set_window(int w) { g_window = w; }
@@ -35,10 +37,12 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
             for (int k = 1; k <= pver; ++k) Column<R>(a, T, i, i).setup_level2(k);
     }
     std::vector<int> deferred((size_t)ncol);
+    std::vector<double> state((size_t)NRESUME * (size_t)a.wld, -777.0);
     int ndefer = 0;
     if (g_defer) {
         a.defer_list = deferred.data();
         a.defer_count = &ndefer;
+        a.defer_state = state.data();
     }
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ncol; ++i) Column<R>(a, T, i, i).run();
@@ -46,8 +50,17 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
     a.defer_list = nullptr;
     a.defer_count = nullptr;
     if (g_defer) {
+        // continue the listed columns in their own slots, whose set-A environment the
+        // first pass (or the level-parallel set-up) left in place
+        a.env_ready = 1;
+        a.resume = 1;
+        a.slot0 = 0;
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int j = 0; j < ndefer; ++j) Column<R>(a, T, deferred[j], j).run();
+        for (int j = 0; j < ndefer; ++j) Column<R>(a, T, deferred[j], deferred[j]).run();
+        g_deferred = ndefer;
+        a.resume = 0;
+        a.env_ready = 0;
+        a.defer_state = nullptr;
     }
 }
 

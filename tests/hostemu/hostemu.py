@@ -102,6 +102,7 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
     L.hostemu_set_defer(0)
     if stats is not None:
         stats["recomputed"] = nfix.value
+        stats["deferred"] = L.hostemu_last_deferred() if defer else 0
     out["cush"] = cush
     return out, np.ascontiguousarray(diag.T), rc, bad.value
 
