@@ -63,6 +63,9 @@ def parse():
                          "(for the multi-million-column workloads; columns are independent)")
     ap.add_argument("--workspace-gib", type=float, default=None,
                     help="context workspace limit (default: the library's 24 GiB; the adjoint uses 160 GiB)")
+    ap.add_argument("--chunks", type=int, default=1,
+                    help="split the workload into this many physics chunks and time one uwshcu_forward_chunks_device "
+                         "call over them (value), beside one uwshcu_forward_device call per chunk")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer three times: the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -161,6 +164,35 @@ def main():
         extra["adjoint"] = {"fp64_forward_ms": fwd_ms, "forward_plus_adjoint_ms": float(np.mean(per_step_ms)),
                             "ratio_to_fp64_forward": fwd_ms / float(np.mean(per_step_ms)),
                             "param_gradient": grads["par_sum"].cpu().tolist()}
+    elif args.chunks > 1:
+        # the chunk arrays of a timestep: separate allocations, as CAM holds them
+        bounds = np.linspace(0, args.ncol, args.chunks + 1).astype(int)
+        calls = []
+        for c in range(args.chunks):
+            a, b = int(bounds[c]), int(bounds[c + 1])
+            sub = {k: v[..., a:b].contiguous() for k, v in inp.items()}
+            calls.append((sub, ctx.alloc_outputs(b - a, args.pver, args.ncnst), sub["cush"].clone(), None))
+
+        table = ctx.chunk_table(calls)
+        cush_all = torch.cat([sub["cush"] for sub, _, _, _ in calls])
+
+        def batched():
+            # cush is in/out: restore every chunk's input value with one multi-tensor copy
+            torch._foreach_copy_([cu for _, _, cu, _ in calls], list(cush_all.split([cu.numel() for _, _, cu, _ in calls])))
+            ctx.forward_chunks(table, dt=ch.dt, precision=prec, stream=stream)
+
+        def one_by_one():
+            for sub, o, cu, _ in calls:
+                cu.copy_(sub["cush"])
+                ctx.forward_into(sub, o, cu, dt=ch.dt, precision=prec, stream=stream)
+        loop_ms, _ = timed(one_by_one, args.steps, args.warmup, stream)
+        per_step_ms, wall = timed(batched, args.steps, args.warmup, stream)
+        ctx.check(stream)
+        fwd(prec)                 # the diagnostics below come from the whole chunk
+        extra["chunks"] = {"count": args.chunks, "columns_per_chunk": args.ncol // args.chunks,
+                           "one_call_per_chunk_ms": float(np.mean(loop_ms)),
+                           "batched_ms": float(np.mean(per_step_ms)),
+                           "speedup": float(np.mean(loop_ms)) / float(np.mean(per_step_ms))}
     else:
         per_step_ms, wall = timed(lambda: fwd(prec), args.steps, args.warmup, stream)
     rc, bad = ctx.status(stream)
@@ -191,6 +223,8 @@ def main():
     mode = "fp64 forward" if prec == _capi.FP64 else "mixed forward"
     if args.adjoint:
         mode = "fp64 forward + adjoint"
+    if args.chunks > 1:
+        mode += f" over {args.chunks} chunks in one call"
     res = {
         "metric": METRIC,
         "value": cols_per_s,
@@ -214,8 +248,9 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(mode, args),
             "algorithmic_bytes_per_launch": bpc * args.ncol, "algorithmic_bytes_per_column": bpc,
-            "timed": "the forward pipeline (two level-parallel set-up kernels + the column kernel, which is 97 % "
-                     "of it in rocprofv3), HIP events on its stream",
+            "timed": ("the forward pipeline (two level-parallel set-up kernels + the column kernel, which is 97 % "
+                      "of it in rocprofv3), HIP events on its stream") if args.chunks == 1 else
+                     "one batched call: table upload, gather, the forward pipeline and scatter, HIP events on its stream",
             "note": "latency-bound FP64 VALU kernel; HBM view of the algorithmic I/O",
         },
         "step_ms_min": float(np.min(per_step_ms)),
@@ -237,7 +272,7 @@ def measured_traffic(mode, args):
             ent = json.load(f).get(key)
     except OSError:
         return None
-    return ent["traffic_bytes"] if ent and "adjoint" not in mode else None
+    return ent["traffic_bytes"] if ent and "adjoint" not in mode and "chunks" not in mode else None
 
 
 def mixed_error(mix, f64, nfix, ncol):
@@ -259,6 +294,20 @@ def mixed_error(mix, f64, nfix, ncol):
             "reference_harness_rms_vs_fp64": parity.reference_rms_errors(mix, f64)}
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host the CPU baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_leg(args, ch, out, cush):
     """CPU baseline (the oracle on a bounded sample) and the GPU-vs-oracle error on it."""
     import numpy as np
@@ -277,6 +326,11 @@ def cpu_leg(args, ch, out, cush):
     t0 = time.perf_counter()
     ref, rdiag, rc, _ = oracle.run(sub, nthreads=nthr)    # host libm: the faithful reading of the reference
     dt_cpu = time.perf_counter() - t0
+    n1 = min(2048, n)                                     # one thread, on a smaller prefix
+    one = Chunk(n1, ch.pver, ch.ncnst, ch.dt, {k: np.ascontiguousarray(v[..., :n1]) for k, v in ch.fields.items()})
+    t1 = time.perf_counter()
+    oracle.run(one, nthreads=1)
+    dt_one = time.perf_counter() - t1
     pm, pmdiag, _, _ = oracle.run(sub, nthreads=nthr, portable=True)
     got = {k: v.cpu().numpy()[..., :n] for k, v in out.items() if k != "diag"}
     got["cush"] = cush.cpu().numpy()[:n]
@@ -289,7 +343,9 @@ def cpu_leg(args, ch, out, cush):
     return {
         "cpu_baseline": {"value": n / dt_cpu, "unit": "columns/s", "cores": nthr, "kind": "port",
                          "sample": f"first {n} columns of the same chunk, FP64 C oracle (host libm), "
-                                   f"OpenMP over columns, {nthr} threads"},
+                                   f"OpenMP over columns, {nthr} threads",
+                         "single_thread_value": n1 / dt_one, "single_thread_sample": f"first {n1} columns, 1 thread",
+                         "host_cpu": host_cpu()},
         "error_vs_oracle": {
             "bit_identical_to_portable_math_oracle": bool(identical),
             "host_libm_oracle_max_col_rel_err": float(worst_col.max()),

@@ -51,6 +51,16 @@ struct uwshcu_ctx {
     int* d_defer = nullptr;
     double* d_defer_state = nullptr;   // NRESUME x defer_cols doubles: their saved scalars
     size_t defer_cols = 0;
+    // chunk batching: the packed column range, its device tables and their pinned
+    // host staging (reused once the event says the previous upload finished)
+    double* d_packed = nullptr;
+    size_t packed_bytes = 0;
+    char* d_ctab = nullptr;
+    size_t ctab_bytes = 0;
+    char* h_ctab = nullptr;
+    size_t h_ctab_bytes = 0;
+    hipEvent_t ctab_event = nullptr;
+    bool ctab_pending = false;
     hipStream_t stream = nullptr;
     std::mutex mu;
 };
@@ -527,6 +537,10 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_counts) (void)hipFree(ctx->d_counts);
     if (ctx->d_defer) (void)hipFree(ctx->d_defer);
     if (ctx->d_defer_state) (void)hipFree(ctx->d_defer_state);
+    if (ctx->d_packed) (void)hipFree(ctx->d_packed);
+    if (ctx->d_ctab) (void)hipFree(ctx->d_ctab);
+    if (ctx->h_ctab) (void)hipHostFree(ctx->h_ctab);
+    if (ctx->ctab_event) (void)hipEventDestroy(ctx->ctab_event);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return UWSHCU_OK;
@@ -590,6 +604,148 @@ int uwshcu_forward_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* p
              umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
              qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
     return run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream);
+}
+
+int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int nchunk,
+                                 const uwshcu_chunk* chunks, int pver, int ncnst, double dt, void* stream) {
+    if (!ctx || nchunk < 1 || !chunks || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX) return UWSHCU_ERR_ARG;
+    long long n = 0;
+    for (int c = 0; c < nchunk; ++c) {
+        const uwshcu_chunk& k = chunks[c];
+        if (k.pcols < 1 || k.ncol < 0 || k.ncol > k.pcols) return UWSHCU_ERR_ARG;
+        if (k.ncol == 0) continue;
+        const void* req[] = {k.ps0, k.zs0, k.p0, k.z0, k.dp0, k.u0, k.v0, k.qv0, k.ql0, k.qi0, k.t0, k.s0, k.tke,
+                             k.pblh, k.dpdry0, k.cush, k.umf, k.slflx, k.qtflx, k.flxprc1, k.flxsnow1, k.wu,
+                             k.qvten, k.qlten, k.qiten, k.sten, k.uten, k.vten, k.qrten, k.qsten, k.evapc,
+                             k.cufrc, k.qcu, k.qlu, k.qiu, k.qc, k.precip, k.snow, k.cbmf, k.rliq, k.cnt, k.cnb};
+        for (const void* q : req)
+            if (!q) return UWSHCU_ERR_ARG;
+        if (ncnst > 0 && (!k.tr0 || !k.trten)) return UWSHCU_ERR_ARG;
+        n += k.ncol;
+    }
+    if (n == 0) return UWSHCU_OK;
+    if (n > 0x7fffffffLL) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    const hipStream_t s = (hipStream_t)stream;
+    const int L = pver, KS = pver + 1, TR = pver * ncnst;
+
+    // Fields in packed order.  Inputs (gathered): cush last, shared with the outputs.
+    auto in_ptrs = [](const uwshcu_chunk& k) {
+        return std::vector<double*>{const_cast<double*>(k.ps0), const_cast<double*>(k.zs0), const_cast<double*>(k.tke),
+                                    const_cast<double*>(k.p0),  const_cast<double*>(k.z0),  const_cast<double*>(k.dp0),
+                                    const_cast<double*>(k.u0),  const_cast<double*>(k.v0),  const_cast<double*>(k.qv0),
+                                    const_cast<double*>(k.ql0), const_cast<double*>(k.qi0), const_cast<double*>(k.t0),
+                                    const_cast<double*>(k.s0),  const_cast<double*>(k.dpdry0),
+                                    const_cast<double*>(k.tr0), const_cast<double*>(k.pblh), k.cush};
+    };
+    const std::vector<int> in_rows{KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, TR, 1, 1};
+    // Outputs (scattered): cush again, at the in-cush packed array.
+    auto out_ptrs = [](const uwshcu_chunk& k) {
+        return std::vector<double*>{k.umf,   k.slflx, k.qtflx, k.flxprc1, k.flxsnow1, k.wu,     k.qvten,
+                                    k.qlten, k.qiten, k.sten,  k.uten,    k.vten,     k.qrten,  k.qsten,
+                                    k.evapc, k.cufrc, k.qcu,   k.qlu,     k.qiu,      k.qc,     k.trten,
+                                    k.precip, k.snow, k.cbmf,  k.rliq,    k.cnt,      k.cnb,    k.cush};
+    };
+    const std::vector<int> out_rows{KS, KS, KS, KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, L, L, L, TR,
+                                    1,  1,  1,  1,  1,  1,  1};
+    const int nin = (int)in_rows.size(), nout = (int)out_rows.size();
+
+    // packed arrays: inputs, then the outputs other than cush
+    size_t rows_total = 0;
+    for (int r : in_rows) rows_total += (size_t)r;
+    for (int f = 0; f < nout - 1; ++f) rows_total += (size_t)out_rows[f];
+    const size_t pbytes = rows_total * (size_t)n * sizeof(double);
+    if (pbytes > ctx->packed_bytes) {
+        if (ctx->d_packed) {
+            if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+            (void)hipFree(ctx->d_packed);
+            ctx->d_packed = nullptr;
+            ctx->packed_bytes = 0;
+        }
+        if (!hip_ok(hipMalloc((void**)&ctx->d_packed, pbytes))) return UWSHCU_ERR_NOMEM;
+        ctx->packed_bytes = pbytes;
+    }
+    std::vector<double*> pin(nin), pout(nout);
+    {
+        double* q = ctx->d_packed;
+        for (int f = 0; f < nin; ++f) { pin[f] = q; q += (size_t)in_rows[f] * (size_t)n; }
+        for (int f = 0; f < nout - 1; ++f) { pout[f] = q; q += (size_t)out_rows[f] * (size_t)n; }
+        pout[nout - 1] = pin[nin - 1];
+    }
+
+    // device tables, built in pinned host memory and uploaded on the stream
+    auto al8 = [](size_t b) { return (b + 7) & ~(size_t)7; };
+    const size_t o_off = 0, o_pcols = al8(o_off + sizeof(long long) * ((size_t)nchunk + 1));
+    const size_t o_rin = al8(o_pcols + sizeof(int) * (size_t)nchunk), o_rout = al8(o_rin + sizeof(int) * nin);
+    const size_t o_pin = al8(o_rout + sizeof(int) * nout), o_pout = o_pin + sizeof(double*) * nin;
+    const size_t o_tin = o_pout + sizeof(double*) * nout, o_tout = o_tin + sizeof(double*) * (size_t)nin * nchunk;
+    const size_t tbytes = o_tout + sizeof(double*) * (size_t)nout * nchunk;
+    if (!ctx->ctab_event && !hip_ok(hipEventCreateWithFlags(&ctx->ctab_event, hipEventDisableTiming)))
+        return UWSHCU_ERR_HIP;
+    if (ctx->ctab_pending && !hip_ok(hipEventSynchronize(ctx->ctab_event))) return UWSHCU_ERR_HIP;
+    ctx->ctab_pending = false;
+    if (tbytes > ctx->h_ctab_bytes) {
+        if (ctx->h_ctab) (void)hipHostFree(ctx->h_ctab);
+        ctx->h_ctab = nullptr;
+        ctx->h_ctab_bytes = 0;
+        if (!hip_ok(hipHostMalloc((void**)&ctx->h_ctab, tbytes, hipHostMallocDefault))) return UWSHCU_ERR_NOMEM;
+        ctx->h_ctab_bytes = tbytes;
+    }
+    if (tbytes > ctx->ctab_bytes) {
+        if (ctx->d_ctab) {
+            if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
+            (void)hipFree(ctx->d_ctab);
+            ctx->d_ctab = nullptr;
+            ctx->ctab_bytes = 0;
+        }
+        if (!hip_ok(hipMalloc((void**)&ctx->d_ctab, tbytes))) return UWSHCU_ERR_NOMEM;
+        ctx->ctab_bytes = tbytes;
+    }
+    char* h = ctx->h_ctab;
+    long long* off = reinterpret_cast<long long*>(h + o_off);
+    int* pcols = reinterpret_cast<int*>(h + o_pcols);
+    off[0] = 0;
+    for (int c = 0; c < nchunk; ++c) {
+        off[c + 1] = off[c] + chunks[c].ncol;
+        pcols[c] = chunks[c].pcols;
+        const std::vector<double*> pi = in_ptrs(chunks[c]), po = out_ptrs(chunks[c]);
+        for (int f = 0; f < nin; ++f)
+            reinterpret_cast<double**>(h + o_tin)[(size_t)f * nchunk + c] = chunks[c].ncol ? pi[f] : nullptr;
+        for (int f = 0; f < nout; ++f)
+            reinterpret_cast<double**>(h + o_tout)[(size_t)f * nchunk + c] = chunks[c].ncol ? po[f] : nullptr;
+    }
+    std::memcpy(h + o_rin, in_rows.data(), sizeof(int) * nin);
+    std::memcpy(h + o_rout, out_rows.data(), sizeof(int) * nout);
+    std::memcpy(h + o_pin, pin.data(), sizeof(double*) * nin);
+    std::memcpy(h + o_pout, pout.data(), sizeof(double*) * nout);
+    if (!hip_ok(hipMemcpyAsync(ctx->d_ctab, h, tbytes, hipMemcpyHostToDevice, s))) return UWSHCU_ERR_HIP;
+    if (!hip_ok(hipEventRecord(ctx->ctab_event, s))) return UWSHCU_ERR_HIP;
+    ctx->ctab_pending = true;
+
+    char* d = ctx->d_ctab;
+    const uwshcu::ChunkTable tin{nchunk, nin, n, reinterpret_cast<const long long*>(d + o_off),
+                                 reinterpret_cast<const int*>(d + o_pcols), reinterpret_cast<const int*>(d + o_rin),
+                                 reinterpret_cast<double* const*>(d + o_pin), reinterpret_cast<double* const*>(d + o_tin)};
+    const uwshcu::ChunkTable tout{nchunk, nout, n, reinterpret_cast<const long long*>(d + o_off),
+                                  reinterpret_cast<const int*>(d + o_pcols), reinterpret_cast<const int*>(d + o_rout),
+                                  reinterpret_cast<double* const*>(d + o_pout),
+                                  reinterpret_cast<double* const*>(d + o_tout)};
+    if (!hip_ok(uwshcu::launch_chunks_gather(tin, s))) return UWSHCU_ERR_HIP;
+    Chunk ch{};
+    ch.ps0 = pin[0]; ch.zs0 = pin[1]; ch.tke = pin[2]; ch.p0 = pin[3]; ch.z0 = pin[4]; ch.dp0 = pin[5];
+    ch.u0 = pin[6]; ch.v0 = pin[7]; ch.qv0 = pin[8]; ch.ql0 = pin[9]; ch.qi0 = pin[10]; ch.t0 = pin[11];
+    ch.s0 = pin[12]; ch.dpdry0 = pin[13]; ch.tr0 = pin[14]; ch.pblh = pin[15]; ch.cush = pin[16];
+    ch.umf = pout[0]; ch.slflx = pout[1]; ch.qtflx = pout[2]; ch.flxprc1 = pout[3]; ch.flxsnow1 = pout[4];
+    ch.wu = pout[5]; ch.qvten = pout[6]; ch.qlten = pout[7]; ch.qiten = pout[8]; ch.sten = pout[9];
+    ch.uten = pout[10]; ch.vten = pout[11]; ch.qrten = pout[12]; ch.qsten = pout[13]; ch.evapc = pout[14];
+    ch.cufrc = pout[15]; ch.qcu = pout[16]; ch.qlu = pout[17]; ch.qiu = pout[18]; ch.qc = pout[19];
+    ch.trten = pout[20]; ch.precip = pout[21]; ch.snow = pout[22]; ch.cbmf = pout[23]; ch.rliq = pout[24];
+    ch.cnt = pout[25]; ch.cnb = pout[26];
+    const int rc = run_device(ctx, precision, par, (int)n, (int)n, pver, ncnst, dt, ch, nullptr, s);
+    if (rc) return rc;
+    if (!hip_ok(uwshcu::launch_chunks_scatter(tout, s))) return UWSHCU_ERR_HIP;
+    return UWSHCU_OK;
 }
 
 int uwshcu_adjoint_device(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,

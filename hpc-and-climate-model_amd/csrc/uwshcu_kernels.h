@@ -33,5 +33,21 @@ hipError_t launch_param_sum(const double* par, long long ldc, int ncol, double* 
 // Set the two status words to zero (a fill kernel, so that graph capture records no memset node).
 hipError_t launch_clear_status(int* err, hipStream_t stream);
 
+// Device-side description of a set of chunks and of their packed column range
+// (all pointers device pointers; see uwshcu_chunks.hip).
+struct ChunkTable {
+    int nchunk, nfield;
+    long long n;               // packed columns: off[nchunk]
+    const long long* off;      // [nchunk + 1] first packed column of each chunk
+    const int* pcols;          // [nchunk] leading dimension of each chunk's arrays
+    const int* rows;           // [nfield] rows of each field (levels, levels x constituents, or 1)
+    double* const* packed;     // [nfield] packed arrays, [rows][n]
+    double* const* ptr;        // [nfield][nchunk] the chunks' arrays (null: zero on gather, skip on scatter)
+};
+// packed <- chunks (every field of the table)
+hipError_t launch_chunks_gather(const ChunkTable& t, hipStream_t stream);
+// chunks <- packed (every field of the table with a non-null chunk array)
+hipError_t launch_chunks_scatter(const ChunkTable& t, hipStream_t stream);
+
 }  // namespace uwshcu
 #endif

@@ -62,6 +62,17 @@ class Gradients(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in GRAD_FIELDS]
 
 
+# uwshcu_chunk (uwshcu_types.h): one chunk of uwshcu_forward_chunks_device
+CHUNK_ARRAYS = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke", "pblh",
+                "dpdry0", "cush", "umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu", "qvten", "qlten", "qiten",
+                "sten", "uten", "vten", "trten", "qrten", "qsten", "evapc", "cufrc", "qcu", "qlu", "qiu", "qc",
+                "precip", "snow", "cbmf", "rliq", "cnt", "cnb")
+
+
+class ChunkDesc(C.Structure):
+    _fields_ = [("pcols", C.c_int), ("ncol", C.c_int)] + [(n, C.c_void_p) for n in CHUNK_ARRAYS]
+
+
 # Argument names of the forward entry points, after (ctx, precision, par, pcols, ncol,
 # pver, ncnst, dt), in the order of compute_uwshcu_inv (uwshcu.F90:251-265).
 INPUT_NAMES = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke",
@@ -76,7 +87,8 @@ SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params
            "uwshcu_ctx_destroy", "uwshcu_ctx_set_workspace_limit", "uwshcu_workspace_bytes_per_column",
            "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
-           "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity")
+           "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity",
+           "uwshcu_forward_chunks_device")
 
 _lib = None
 
@@ -122,6 +134,8 @@ def lib():
     L.uwshcu_adjoint_device.argtypes = [vp, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp] + \
         [vp] * len(ARRAY_ARGS) + [vp, C.POINTER(Cotangents), C.POINTER(Gradients), vp]
     L.uwshcu_ctx_set_tape_capacity.argtypes = [vp, C.c_longlong]
+    L.uwshcu_forward_chunks_device.argtypes = [vp, C.c_int, C.POINTER(Params), C.c_int, C.POINTER(ChunkDesc), C.c_int,
+                                               C.c_int, dp, vp]
     _lib = L
     return L
 

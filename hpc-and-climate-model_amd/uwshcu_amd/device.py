@@ -98,6 +98,45 @@ class Context:
         if rc:
             raise UwshcuError(rc, what="uwshcu_forward_device")
 
+    @staticmethod
+    def chunk_table(chunks: list):
+        """The descriptor table of uwshcu_forward_chunks_device for ``chunks``, a list of
+        ``(inp, out, cush, ncol)`` (``ncol`` None = all of the chunk's pcols), each laid
+        out as for ``forward_into``.  Build it once and pass it to ``forward_chunks`` for
+        every step that reuses the same arrays."""
+        if not chunks:
+            raise ValueError("no chunks")
+        pver = chunks[0][0]["p0"].shape[0]
+        tr = chunks[0][0]["tr0"]
+        ncnst = tr.shape[0] if tr.dim() == 3 else 0
+        desc = (_capi.ChunkDesc * len(chunks))()
+        for j, (inp, out, cush, ncol) in enumerate(chunks):
+            pv, pcols = inp["p0"].shape
+            if pv != pver:
+                raise ValueError("all chunks must have the same number of levels")
+            d = desc[j]
+            d.pcols, d.ncol = pcols, pcols if ncol is None else ncol
+            for k in _capi.CHUNK_ARRAYS:
+                t = cush if k == "cush" else (inp[k] if k in inp else out[k])
+                if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                    raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+                setattr(d, k, t.data_ptr())
+        return (desc, len(chunks), pver, ncnst)
+
+    def forward_chunks(self, chunks, *, dt: float = 1800.0, precision: int = FP64,
+                       params: _capi.Params | None = None, stream=None) -> None:
+        """One launch over several chunks (a list as for ``chunk_table``, or its result).
+        Writes every chunk's ``out`` and ``cush`` in place (asynchronous)."""
+        desc, n, pver, ncnst = chunks if isinstance(chunks, tuple) else self.chunk_table(chunks)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        par = params or _capi.default_params()
+        rc = _capi.lib().uwshcu_forward_chunks_device(self._h, precision, C.byref(par), n, desc, pver, ncnst,
+                                                       C.c_double(dt), C.c_void_p(sp))
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_forward_chunks_device")
+
     def status(self, stream=None) -> tuple[int, int]:
         if stream is None:
             stream = torch.cuda.current_stream(self.device)

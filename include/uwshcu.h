@@ -92,6 +92,20 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
     double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
 
+/* Forward physics over many chunks in one launch.  The reference calls the scheme
+ * once per physics chunk (physpkg.F90:782-803 -> convect_shallow.F90:605-619, a few
+ * dozen to a thousand columns each); this entry point takes a timestep's chunks
+ * (`chunks`, a host array of nchunk descriptors holding device pointers), gathers
+ * their columns into one packed range in context-owned device memory, runs the
+ * column pipeline once over it, and scatters the outputs (and cush) back.  Results
+ * equal nchunk calls of uwshcu_forward_device bit for bit.  Asynchronous on `stream`
+ * like uwshcu_forward_device; uwshcu_status then reports a fatal condition's column
+ * as its position in the packed range (the chunks' ncol summed in order).  The packed
+ * range holds about 11 KB per column at 35 levels and 3 constituents, beside the
+ * workspace. */
+int uwshcu_forward_chunks_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *par,
+    int nchunk, const uwshcu_chunk *chunks, int pver, int ncnst, double dt, void *stream);
+
 /* Adjoint (vector-Jacobian product) of the FP64 forward on device arrays.  No
  * counterpart in the reference, which has no derivative code.  Re-runs the FP64
  * forward of every column while recording it on a per-column tape (the branch

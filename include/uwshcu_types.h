@@ -61,6 +61,19 @@ typedef struct uwshcu_gradients {
     double *t0, *qv0, *u0, *v0, *tke, *pblh, *par_col, *par_sum;
 } uwshcu_gradients;
 
+// One physics chunk for uwshcu_forward_chunks_device: the device arrays of one
+// bxcompute_uwshcu_inv_ call (Fortran (pcols, pver[+1]) layout, CAM level order,
+// tr0/trten (pcols, pver, ncnst)).  Only the first ncol columns are read or written.
+// tr0/trten may be NULL when ncnst is 0; every other array is required when ncol > 0.
+typedef struct uwshcu_chunk {
+    int pcols, ncol;
+    const double *ps0, *zs0, *p0, *z0, *dp0, *u0, *v0, *qv0, *ql0, *qi0, *t0, *s0, *tr0, *tke, *pblh, *dpdry0;
+    double *cush;
+    double *umf, *slflx, *qtflx, *flxprc1, *flxsnow1, *wu;
+    double *qvten, *qlten, *qiten, *sten, *uten, *vten, *trten, *qrten, *qsten, *evapc, *cufrc, *qcu, *qlu, *qiu, *qc;
+    double *precip, *snow, *cbmf, *rliq, *cnt, *cnb;
+} uwshcu_chunk;
+
 // Return codes (negative values are conditions the reference treats as fatal).
 enum {
     UWSHCU_OK                 = 0,

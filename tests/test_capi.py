@@ -35,6 +35,9 @@ def test_struct_layout_matches_header():
     # offsets the HIP side relies on: the ctypes mirror must match the C struct
     assert C.sizeof(_capi.Consts) == 8 * 19 + 4 * 4 + 8 * 5 + 8 * 33 + 4 * 33 + 4
     assert C.sizeof(_capi.Params) == 24
+    # uwshcu_chunk: two ints, then 44 array pointers
+    assert C.sizeof(_capi.ChunkDesc) == 8 + 8 * 44
+    assert _capi.ChunkDesc.ps0.offset == 8 and _capi.ChunkDesc.cnb.offset == 8 + 8 * 43
 
 
 def test_default_constants_equal_the_oracle():
@@ -71,6 +74,9 @@ def test_device_entry_points_reject_bad_arguments_without_a_gpu():
     rc = lib.uwshcu_forward_device(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
     assert rc == -10
     rc = lib.uwshcu_forward_host(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
+    assert rc == -10
+    desc = (_capi.ChunkDesc * 2)()
+    rc = lib.uwshcu_forward_chunks_device(None, 0, C.byref(par), 2, desc, 30, 3, C.c_double(1800.0), None)
     assert rc == -10
 
 
