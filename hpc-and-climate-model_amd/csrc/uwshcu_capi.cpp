@@ -748,6 +748,20 @@ int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_pa
     return UWSHCU_OK;
 }
 
+int uwshcu_merge_shallow_device(int pcols, int ncol, int pver, double latvap, const uwshcu_merge_fields* f,
+                                void* stream) {
+    if (pcols < 1 || ncol < 0 || ncol > pcols || pver < 1 || !f) return UWSHCU_ERR_ARG;
+    const void* req[] = {f->umf,   f->slflx, f->qtflx, f->qrten, f->qsten,   f->qcu,    f->qc2,   f->rliq2,
+                         f->cnt2,  f->cnb2,  f->cbmf,  f->pmid,  f->rprddp,  f->cmfmc,  f->cnt,   f->cnb,
+                         f->rliq,  f->qc,    f->icwmr, f->rprdsh, f->rprdtot, f->cmfsl, f->cmflq, f->freqsh,
+                         f->pcnt,  f->pcnb,  f->massflx_sh};
+    for (const void* q : req)
+        if (!q) return UWSHCU_ERR_ARG;
+    if (!hip_ok(uwshcu::launch_merge_shallow(pcols, ncol, pver, latvap, *f, (hipStream_t)stream)))
+        return UWSHCU_ERR_HIP;
+    return UWSHCU_OK;
+}
+
 int uwshcu_adjoint_device(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,
                           double dt, const double* ps0, const double* zs0, const double* p0, const double* z0,
                           const double* dp0, const double* u0, const double* v0, const double* qv0,

@@ -48,6 +48,9 @@ struct ChunkTable {
 hipError_t launch_chunks_gather(const ChunkTable& t, hipStream_t stream);
 // chunks <- packed (every field of the table with a non-null chunk array)
 hipError_t launch_chunks_scatter(const ChunkTable& t, hipStream_t stream);
+// The driver-side merge of uwshcu_merge_shallow_device (uwshcu_merge.hip).
+hipError_t launch_merge_shallow(int pcols, int ncol, int pver, double latvap, const uwshcu_merge_fields& f,
+                                hipStream_t stream);
 
 }  // namespace uwshcu
 #endif

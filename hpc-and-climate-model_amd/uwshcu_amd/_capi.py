@@ -73,6 +73,16 @@ class ChunkDesc(C.Structure):
     _fields_ = [("pcols", C.c_int), ("ncol", C.c_int)] + [(n, C.c_void_p) for n in CHUNK_ARRAYS]
 
 
+# uwshcu_merge_fields (uwshcu_types.h), in declaration order
+MERGE_FIELDS = ("umf", "slflx", "qtflx", "qrten", "qsten", "qcu", "qc2", "rliq2", "cnt2", "cnb2", "cbmf", "pmid",
+                "rprddp", "cmfmc", "cnt", "cnb", "rliq", "qc", "icwmr", "rprdsh", "rprdtot", "cmfsl", "cmflq",
+                "freqsh", "pcnt", "pcnb", "massflx_sh")
+
+
+class MergeFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in MERGE_FIELDS]
+
+
 # Argument names of the forward entry points, after (ctx, precision, par, pcols, ncol,
 # pver, ncnst, dt), in the order of compute_uwshcu_inv (uwshcu.F90:251-265).
 INPUT_NAMES = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke",
@@ -88,7 +98,7 @@ SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params
            "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
            "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity",
-           "uwshcu_forward_chunks_device")
+           "uwshcu_forward_chunks_device", "uwshcu_merge_shallow_device")
 
 _lib = None
 
@@ -136,6 +146,7 @@ def lib():
     L.uwshcu_ctx_set_tape_capacity.argtypes = [vp, C.c_longlong]
     L.uwshcu_forward_chunks_device.argtypes = [vp, C.c_int, C.POINTER(Params), C.c_int, C.POINTER(ChunkDesc), C.c_int,
                                                C.c_int, dp, vp]
+    L.uwshcu_merge_shallow_device.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.POINTER(MergeFields), vp]
     _lib = L
     return L
 

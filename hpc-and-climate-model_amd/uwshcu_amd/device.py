@@ -226,6 +226,27 @@ class Context:
         return out
 
 
+def merge_shallow(fields: dict, *, ncol: int | None = None, latvap: float | None = None, stream=None) -> None:
+    """convect_shallow_tend's merge after the scheme (uwshcu_merge_shallow_device) on
+    device tensors laid out [levels, pcols]; updates the in/out fields and writes the
+    outputs in place (asynchronous).  ``latvap`` defaults to the library's xlv."""
+    pver, pcols = fields["pmid"].shape
+    ncol = pcols if ncol is None else ncol
+    latvap = _capi.default_consts().xlv if latvap is None else latvap
+    f = _capi.MergeFields()
+    for k in _capi.MERGE_FIELDS:
+        t = fields[k]
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+        setattr(f, k, t.data_ptr())
+    if stream is None:
+        stream = torch.cuda.current_stream(fields["pmid"].device)
+    sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+    rc = _capi.lib().uwshcu_merge_shallow_device(pcols, ncol, pver, C.c_double(latvap), C.byref(f), C.c_void_p(sp))
+    if rc:
+        raise UwshcuError(rc, what="uwshcu_merge_shallow_device")
+
+
 def chunk_to_device(chunk, device="cuda") -> dict:
     """Copy a generator chunk (numpy fields) to device tensors."""
     return {k: torch.from_numpy(v).to(device) for k, v in chunk.fields.items()}

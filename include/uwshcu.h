@@ -106,6 +106,20 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
 int uwshcu_forward_chunks_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *par,
     int nchunk, const uwshcu_chunk *chunks, int pver, int ncnst, double dt, void *stream);
 
+/* The driver-side merge of the scheme's outputs with the deep-convection fields that
+ * convect_shallow_tend performs after the call (convect_shallow.F90:616-660 for the UW
+ * branch, 790-836 for all schemes), on device arrays, for columns i < ncol:
+ *   icwmr = qcu; rprdsh = qrten + qsten; cmfsl = slflx; cmflq = qtflx * latvap;
+ *   freqsh = 1 if max(cmfmc2(i, 1:pver)) <= 0 else 0; cmfmc(i, 1:pver) += cmfmc2(i, 1:pver);
+ *   cnt = min(cnt, cnt2); cnb = max(cnb, cnb2); pcnt = pmid(i, int(cnt)); pcnb = pmid(i, int(cnb));
+ *   rprdtot = rprdsh + rprddp; qc += qc2; rliq += rliq2;
+ * and massflx_sh(1:pcols) = cbmf(1:pcols).  Level indices outside 1..pver are clamped
+ * to that range for pcnt/pcnb (the driver would read out of bounds).  The driver's
+ * copy ptend%q(:, :, 4:pcnst) = trten(:, :, 4:pcnst) is a plain device copy of this
+ * library's trten and is left to the caller.  Asynchronous on `stream`. */
+int uwshcu_merge_shallow_device(int pcols, int ncol, int pver, double latvap, const uwshcu_merge_fields *f,
+    void *stream);
+
 /* Adjoint (vector-Jacobian product) of the FP64 forward on device arrays.  No
  * counterpart in the reference, which has no derivative code.  Re-runs the FP64
  * forward of every column while recording it on a per-column tape (the branch

@@ -74,6 +74,29 @@ typedef struct uwshcu_chunk {
     double *precip, *snow, *cbmf, *rliq, *cnt, *cnb;
 } uwshcu_chunk;
 
+// Fields of the driver-side merge that follows the scheme in convect_shallow_tend
+// (convect_shallow.F90:616-660 and 790-836): device pointers in the driver's
+// (pcols, ...) layout, CAM level order, all required.
+typedef struct uwshcu_merge_fields {
+    // in: this library's outputs for the chunk
+    const double *umf;                    // cmfmc2 (pcols, pver+1)
+    const double *slflx, *qtflx;          // (pcols, pver+1)
+    const double *qrten, *qsten, *qcu;    // rprdsh, cmfdqs, iccmr_UW (pcols, pver)
+    const double *qc2;                    // qc (pcols, pver)
+    const double *rliq2, *cnt2, *cnb2, *cbmf;   // (pcols)
+    // in: physics state and deep convection
+    const double *pmid, *rprddp;          // (pcols, pver)
+    // in/out: the deep-convection fields, merged in place
+    double *cmfmc;                        // (pcols, pver+1): rows 1..pver
+    double *cnt, *cnb, *rliq;             // (pcols)
+    double *qc;                           // (pcols, pver)
+    // out
+    double *icwmr, *rprdsh, *rprdtot;     // (pcols, pver)
+    double *cmfsl, *cmflq;                // (pcols, pver+1)
+    double *freqsh, *pcnt, *pcnb;         // (pcols)
+    double *massflx_sh;                   // (pcols): all pcols entries, as the driver copies them
+} uwshcu_merge_fields;
+
 // Return codes (negative values are conditions the reference treats as fatal).
 enum {
     UWSHCU_OK                 = 0,
