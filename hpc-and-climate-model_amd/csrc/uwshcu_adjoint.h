@@ -54,6 +54,14 @@ struct AdGradients {
 // carrying what farther nodes already added in adj; later contributions go to the
 // window.  Every adjoint still receives its contributions in decreasing node order,
 // so the result is the same, bit for bit, as without the window.
+//
+// The windowed sweep works kSweepAhead nodes at a time.  It first issues, together,
+// the loads of the batch's tape nodes and of the adjoints that enter the window during
+// the batch (adj of node q - W for each node q), so that their latencies overlap
+// instead of adding up node by node.  A far contribution the batch itself makes to one
+// of those entering adjoints is added to the early-loaded copy too, before the copy
+// enters the window, so every adjoint still receives its contributions in the same
+// order and the result stays bit for bit the same.
 template <class Win>
 UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes, size_t wld, int slot, int mkx,
                                  int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g,
@@ -83,21 +91,42 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adj, const int* nodes
     }
     const int nin = ad_inputs(mkx);
     if (W > 0) {
+        constexpr int kSweepAhead = 8;
         for (int q = t.n - 1; q >= 0 && q >= t.n - W; --q) win(q % W) = A(q);
-        for (int q = t.n - 1; q >= nin; --q) {
-            const double w = win(q % W);
-            if (w != 0.0) {
-                const TapeNode nd = t.nodes[tape_slot(t, q)];
-                if (nd.a >= 0) {
-                    if (q - nd.a < W) win(nd.a % W) += w * nd.da;
-                    else A(nd.a) += w * nd.da;
-                }
-                if (nd.b >= 0) {
-                    if (q - nd.b < W) win(nd.b % W) += w * nd.db;
-                    else A(nd.b) += w * nd.db;
-                }
+        for (int q0 = t.n - 1; q0 >= nin; q0 -= kSweepAhead) {
+            TapeNode buf[kSweepAhead];
+            double enter[kSweepAhead];   // enter[u]: adj of node q0 - u - W, entering at step u
+#pragma unroll
+            for (int u = 0; u < kSweepAhead; ++u) {
+                if (q0 - u >= nin) buf[u] = t.nodes[tape_slot(t, q0 - u)];
+                enter[u] = (q0 - u >= nin && q0 - u - W >= 0) ? A(q0 - u - W) : 0.0;
             }
-            if (q - W >= 0) win(q % W) = A(q - W);      // node q - W enters the window
+            // a far contribution c to node x made at step u: x enters at step q0 - W - x
+            auto far = [&](int x, double c, int u) {
+                A(x) += c;
+                const int r = q0 - W - x;
+#pragma unroll
+                for (int v = 0; v < kSweepAhead; ++v)
+                    if (v == r && v >= u) enter[v] += c;
+            };
+#pragma unroll
+            for (int u = 0; u < kSweepAhead; ++u) {
+                const int q = q0 - u;
+                if (q < nin) break;
+                const double w = win(q % W);
+                if (w != 0.0) {
+                    const TapeNode& nd = buf[u];
+                    if (nd.a >= 0) {
+                        if (q - nd.a < W) win(nd.a % W) += w * nd.da;
+                        else far(nd.a, w * nd.da, u);
+                    }
+                    if (nd.b >= 0) {
+                        if (q - nd.b < W) win(nd.b % W) += w * nd.db;
+                        else far(nd.b, w * nd.db, u);
+                    }
+                }
+                if (q - W >= 0) win(q % W) = enter[u];      // node q - W enters the window
+            }
         }
         for (int q = nin - 1; q >= 0 && q >= nin - W; --q) A(q) = win(q % W);
     } else {
