@@ -105,7 +105,7 @@ def timed(step, steps, warmup, stream):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(the column physics shards by columns; see the design notes)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:

@@ -12,4 +12,4 @@ for p in (os.path.join(ROOT, "hpc-and-climate-model_amd"), os.path.join(ROOT, "o
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
