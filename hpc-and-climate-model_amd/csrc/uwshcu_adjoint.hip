@@ -17,8 +17,7 @@ __global__ __launch_bounds__(64) void adjoint_count_kernel(FwdArgs args, int n, 
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     // the tape header lives in LDS: every recorded operation updates it
-    __shared__ TapeCtx s_tape[64];
-    TapeCtx& tape = s_tape[threadIdx.x];
+    TapeCtx& tape = *ad_tape();
     ad_begin(tape, nullptr, 0x7fffffff, ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
@@ -35,8 +34,7 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    __shared__ TapeCtx s_tape[64];
-    TapeCtx& tape = s_tape[threadIdx.x];
+    TapeCtx& tape = *ad_tape();
     ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);

@@ -145,7 +145,9 @@ struct Column {
           ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<D*>(args.ws)),
           ws(reinterpret_cast<R*>(reinterpret_cast<D*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
                                                                     (size_t)args.wld)),
-          tape(own_tape ? own_tape : args.ad.tapes ? args.ad.tapes + slot : nullptr) {}
+          tape(own_tape ? own_tape : args.ad.tapes ? args.ad.tapes + slot : nullptr) {
+        if constexpr (kAD) ad_bind_tape(tape);
+    }
 
     // saved scalar f of a deferred column (FwdArgs::defer_state)
     UWSHCU_HD inline double& rstate(int f) const {
@@ -184,7 +186,7 @@ struct Column {
         if constexpr (kAD) {
             const int base = p == a.t0 ? ADI_T0 : p == a.qv0 ? ADI_QV0 : p == a.u0 ? ADI_U0 : p == a.v0 ? ADI_V0
                            : p == a.tke ? ADI_TKE : -1;
-            if (base >= 0) return D(p[row(k)], base * KS + k, tape);
+            if (base >= 0) return D(p[row(k)], base * KS + k);
         }
         return D(p[row(k)]);
     }
@@ -210,7 +212,7 @@ struct Column {
     // tunables: j = 0 rpen, 1 rle, 2 rkm
     UWSHCU_HD inline D param(int j) const {
         const double v = j == 0 ? a.par.rpen : j == 1 ? a.par.rle : a.par.rkm;
-        if constexpr (kAD) return D(v, ADI_PARAMS * KS + j, tape);
+        if constexpr (kAD) return D(v, ADI_PARAMS * KS + j);
         else return v;
     }
     UWSHCU_HD inline D& E(int arr, int k) const {
