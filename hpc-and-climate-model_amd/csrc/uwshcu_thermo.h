@@ -26,6 +26,14 @@ UWSHCU_HD inline void record_error(int* err, int code, int col) {
     if (__atomic_compare_exchange_n(&err[0], &expected, code, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) err[1] = col;
 }
 
+// The FP64 saturation adjustment's internal state at its end, for the adjoint's
+// implicit-function derivative: the ice fraction, effective latent heat, the last two
+// Newton iterates and the saturation values at the last.
+struct SatState {
+    bool sat;                       // false: unsaturated, no Newton steps taken
+    double tc, nu, leff, temps, tprev, es, qs;
+};
+
 template <class D>
 struct ThermoT {
     const uwshcu_consts* c;   // kernel-argument copy
@@ -133,7 +141,8 @@ struct ThermoT {
     //    and reads the table again only when the index moves, which after the first
     //    steps it rarely does.
     UWSHCU_HD __attribute__((always_inline)) int conden_d(double p, double ex, double thl, double qt, double& th,
-                                                         double& qv, double& ql, double& qi) const {
+                                                         double& qv, double& ql, double& qi,
+                                                         SatState* st = nullptr) const {
         const double omeps = 1.0 - c->epsqs;
         int ic = -1;
         double tlo = 0.0, thi = 0.0;
@@ -162,11 +171,14 @@ struct ThermoT {
             ql = 0.0;
             qi = 0.0;
             th = div_fast(tc, ex);
+            if (st) st->sat = false;
             return 0;
         }
         const double rleff = div_fast(1.0, leff), rr = div_fast(1.0, c->r);
         const double cpl = div_rcp(c->cp, leff, rleff);
+        double tprev = temps;
         for (int it = 0; it < 10; ++it) {
+            tprev = temps;
             const double rt = div_fast(1.0, temps);
             const double num = div_rcp((tc - temps) * c->cp, leff, rleff) + qt - rvls;
             const double d3 = div_rcp(div_rcp(div_rcp(c->ep2 * leff * rvls, c->r, rr), temps, rt), temps, rt);
@@ -179,6 +191,7 @@ struct ThermoT {
         ql = qc * (1.0 - nu);
         qi = nu * qc;
         th = div_fast(temps, ex);
+        if (st) *st = SatState{true, tc, nu, leff, temps, tprev, es, qs};
         return (fabs((temps - div_fast(leff, c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
     }
 
@@ -194,13 +207,11 @@ struct ThermoT {
     UWSHCU_HD bool conden_ift(const D& p, const D& thl, const D& qt, D& th, D& qv, D& ql, D& qi, int& id) const {
         const double pv = val(p), thlv = val(thl), qtv = val(qt);
         const double ex = exnf_d(pv);
-        const double tc = thlv * ex;
-        const double nu = fmax(fmin((268.0 - tc) / 20.0, 1.0), 0.0);
-        const double leff = (1.0 - nu) * c->xlv + nu * c->xls;
-        double temps = tc, es, qs;
-        qsat_d(temps, pv, es, qs);
-        double rvls = qs;
-        if (qs >= qtv) {
+        // the values: the FP64 routine itself (the forward kernels' bits)
+        SatState st;
+        double thv, qvv, qlv, qiv;
+        const int idv = ThermoT<double>{c, estbl}.conden_d(pv, ex, thlv, qtv, thv, qvv, qlv, qiv, &st);
+        if (!st.sat) {
             // unsaturated: qv = qt, no condensate, th = (thl ex)/ex
             const D exa = exnf(p);
             const D tca = thl * exa;
@@ -211,18 +222,10 @@ struct ThermoT {
             id = 0;
             return true;
         }
-        double tprev = temps;
-        for (int it = 0; it < 10; ++it) {
-            tprev = temps;
-            temps = temps + ((tc - temps) * c->cp / leff + qtv - rvls) /
-                            (c->cp / leff + c->ep2 * leff * rvls / c->r / temps / temps);
-            qsat_d(temps, pv, es, qs);
-            rvls = qs;
-        }
-        if (fabs(temps - tprev) > 1.e-12 * temps) return false;
+        const double tc = st.tc, nu = st.nu, leff = st.leff, temps = st.temps, es = st.es, qs = st.qs;
+        if (fabs(temps - st.tprev) > 1.e-12 * temps) return false;
         const double qc = fmax(qtv - qs, 0.0);
-        const double qvv = qtv - qc, qlv = qc * (1.0 - nu), qiv = nu * qc, thv = temps / ex;
-        id = (fabs((temps - (leff / c->cp) * qc) - tc) >= 1.0) ? 1 : 0;
+        id = idv;
         // partial derivatives
         const double ex_p = c->rovcp * ex / pv;
         const double tc_thl = ex, tc_p = thlv * ex_p;
