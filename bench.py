@@ -243,6 +243,7 @@ def main():
                         f"{args.ncnst} constituents, dt={ch.dt:g} s",
             "ncol": args.ncol, "pver": args.pver, "ncnst": args.ncnst,
             "convecting_fraction": conv_frac, "inversion_frac": args.inversion_frac, "seed": args.seed,
+            "workspace_gib": ws_gib if ws_gib else 24.0,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
