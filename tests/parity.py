@@ -23,6 +23,8 @@ def column_rel_err(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     floor = max(gmax * 1e-12, 1e-300)
     if a.ndim == 1:
         return np.abs(a - b) / np.maximum(np.abs(b), floor)
+    if a.shape[0] == 0:                     # no constituents: no error per column
+        return np.zeros(a.shape[-1])
     num = np.abs(a - b).max(axis=0)
     den = np.maximum(np.abs(b).max(axis=0), floor)
     return num / den

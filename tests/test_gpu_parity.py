@@ -38,6 +38,7 @@ def _gpu_run(chunk, precision=0, params=None):
     torch.cuda.synchronize()
     res = {k: v.cpu().numpy() for k, v in out.items()}
     res["diag"] = np.ascontiguousarray(res["diag"].T)
+    res["trten"] = res["trten"][:chunk.ncnst]     # allocated with at least one constituent
     return res
 
 
@@ -142,6 +143,40 @@ def test_launch_large_enough_for_the_two_wavefront_kernel():
     """From 131072 columns on, the column kernel is the build that lets two wavefronts
     share a SIMD; its results must be the same bits."""
     _check(make_chunk(163840, 35, 3, seed=9), "163840x35x3")
+
+
+@pytest.mark.parametrize("ncol,pver,ncnst", [(1, 30, 3), (100, 30, 0), (200, 12, 2), (64, 5, 1)])
+def test_edge_shapes(ncol, pver, ncnst):
+    """One column, no constituents, few levels (5 levels: no column convects)."""
+    _check(make_chunk(ncol, pver, ncnst, seed=3), f"{ncol}x{pver}x{ncnst}")
+
+
+def test_zero_columns_is_a_no_op():
+    import torch
+    from uwshcu_amd import device
+    ch = make_chunk(16, 30, 3, seed=3)
+    ctx = device.default_context()
+    inp = device.chunk_to_device(ch)
+    out = ctx.alloc_outputs(16, 30, 3)
+    for v in out.values():
+        v.fill_(7.0)
+    cush = inp["cush"].clone()
+    ctx.forward_into(inp, out, cush, ncol=0, dt=ch.dt)
+    ctx.check()
+    assert all(bool((v == 7.0).all()) for v in out.values())
+    assert torch.equal(cush, inp["cush"])
+
+
+def test_other_tunables_on_the_gpu():
+    """rpen, rle, rkm other than the defaults (the adjoint's parameters)."""
+    from uwshcu_amd import _capi, device
+    ch = make_chunk(2048, 35, 3, seed=12)
+    c = oracle.default_consts(rpen=5.0, rle=0.15, rkm=12.0)
+    pm, pmdiag, rc, _ = oracle.run(ch, consts=c, nthreads=8, portable=True)
+    got = _gpu_run(ch, params=_capi.default_params(rpen=5.0, rle=0.15, rkm=12.0))
+    np.testing.assert_array_equal(got["diag"], pmdiag)
+    for k in pm:
+        np.testing.assert_array_equal(got[k], pm[k], err_msg=k)
 
 
 def test_convecting_fraction_sweep():
