@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <vector>
 
@@ -631,25 +632,25 @@ int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_pa
     const int L = pver, KS = pver + 1, TR = pver * ncnst;
 
     // Fields in packed order.  Inputs (gathered): cush last, shared with the outputs.
+    constexpr int nin = 17, nout = 28;
     auto in_ptrs = [](const uwshcu_chunk& k) {
-        return std::vector<double*>{const_cast<double*>(k.ps0), const_cast<double*>(k.zs0), const_cast<double*>(k.tke),
+        return std::array<double*, nin>{const_cast<double*>(k.ps0), const_cast<double*>(k.zs0), const_cast<double*>(k.tke),
                                     const_cast<double*>(k.p0),  const_cast<double*>(k.z0),  const_cast<double*>(k.dp0),
                                     const_cast<double*>(k.u0),  const_cast<double*>(k.v0),  const_cast<double*>(k.qv0),
                                     const_cast<double*>(k.ql0), const_cast<double*>(k.qi0), const_cast<double*>(k.t0),
                                     const_cast<double*>(k.s0),  const_cast<double*>(k.dpdry0),
                                     const_cast<double*>(k.tr0), const_cast<double*>(k.pblh), k.cush};
     };
-    const std::vector<int> in_rows{KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, TR, 1, 1};
+    const std::array<int, nin> in_rows{KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, TR, 1, 1};
     // Outputs (scattered): cush again, at the in-cush packed array.
     auto out_ptrs = [](const uwshcu_chunk& k) {
-        return std::vector<double*>{k.umf,   k.slflx, k.qtflx, k.flxprc1, k.flxsnow1, k.wu,     k.qvten,
+        return std::array<double*, nout>{k.umf,   k.slflx, k.qtflx, k.flxprc1, k.flxsnow1, k.wu,     k.qvten,
                                     k.qlten, k.qiten, k.sten,  k.uten,    k.vten,     k.qrten,  k.qsten,
                                     k.evapc, k.cufrc, k.qcu,   k.qlu,     k.qiu,      k.qc,     k.trten,
                                     k.precip, k.snow, k.cbmf,  k.rliq,    k.cnt,      k.cnb,    k.cush};
     };
-    const std::vector<int> out_rows{KS, KS, KS, KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, L, L, L, TR,
-                                    1,  1,  1,  1,  1,  1,  1};
-    const int nin = (int)in_rows.size(), nout = (int)out_rows.size();
+    const std::array<int, nout> out_rows{KS, KS, KS, KS, KS, KS, L, L, L, L, L, L, L, L, L, L, L, L, L, L, TR,
+                                         1,  1,  1,  1,  1,  1,  1};
 
     // packed arrays: inputs, then the outputs other than cush
     size_t rows_total = 0;
@@ -666,7 +667,8 @@ int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_pa
         if (!hip_ok(hipMalloc((void**)&ctx->d_packed, pbytes))) return UWSHCU_ERR_NOMEM;
         ctx->packed_bytes = pbytes;
     }
-    std::vector<double*> pin(nin), pout(nout);
+    std::array<double*, nin> pin{};
+    std::array<double*, nout> pout{};
     {
         double* q = ctx->d_packed;
         for (int f = 0; f < nin; ++f) { pin[f] = q; q += (size_t)in_rows[f] * (size_t)n; }
@@ -709,7 +711,8 @@ int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_pa
     for (int c = 0; c < nchunk; ++c) {
         off[c + 1] = off[c] + chunks[c].ncol;
         pcols[c] = chunks[c].pcols;
-        const std::vector<double*> pi = in_ptrs(chunks[c]), po = out_ptrs(chunks[c]);
+        const std::array<double*, nin> pi = in_ptrs(chunks[c]);
+        const std::array<double*, nout> po = out_ptrs(chunks[c]);
         for (int f = 0; f < nin; ++f)
             reinterpret_cast<double**>(h + o_tin)[(size_t)f * nchunk + c] = chunks[c].ncol ? pi[f] : nullptr;
         for (int f = 0; f < nout; ++f)
