@@ -153,12 +153,19 @@ struct Chunk {
     double *qrten, *qsten, *precip, *snow, *evapc, *cufrc, *qcu, *qlu, *qiu, *cbmf, *qc, *rliq, *cnt, *cnb, *wu;
 };
 
+// The modified scheme's options and extra inputs (device pointers).
+struct ZwInputs {
+    const uwshcu_zw_params* par;
+    const double *omega, *tpert, *qpert;
+};
+
 int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,
-               double dt, const Chunk& ch, int* diag, hipStream_t stream) {
+               double dt, const Chunk& ch, int* diag, hipStream_t stream, const ZwInputs* zw = nullptr) {
     if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
         return UWSHCU_ERR_ARG;
     if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED && precision != UWSHCU_MIXED_UNCHECKED)
         return UWSHCU_ERR_ARG;
+    if (zw && (precision != UWSHCU_FP64 || !zw->par || !zw->omega || !zw->tpert || !zw->qpert)) return UWSHCU_ERR_ARG;
     if (ncol == 0) return UWSHCU_OK;
     if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
     const bool fp32 = precision != UWSHCU_FP64;
@@ -212,6 +219,12 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     a.diag = diag;
     a.err = ctx->d_err;
     a.ws = ctx->d_ws;
+    if (zw) {
+        a.zw = *zw->par;
+        a.omega = zw->omega;
+        a.tpert = zw->tpert;
+        a.qpert = zw->qpert;
+    }
     if (ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
         a.defer_list = ctx->d_defer + 1;
         a.defer_count = ctx->d_defer;
@@ -228,8 +241,9 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     for (long long c0 = 0; c0 < ncol; c0 += nb) {
         const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
         a.col0 = (int)c0;
-        hipError_t e = fp32 ? uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream)
-                            : uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream);
+        hipError_t e = zw ? uwshcu::launch_forward_zw(a, n, ctx->d_consts, ctx->d_estbl, stream)
+                       : fp32 ? uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream)
+                              : uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream);
         if (!hip_ok(e)) return UWSHCU_ERR_HIP;
     }
     if (!recompute) return UWSHCU_OK;
@@ -487,6 +501,27 @@ int uwshcu_default_params(uwshcu_params* p) {
     return UWSHCU_OK;
 }
 
+int uwshcu_zw_default_params(uwshcu_params* p, uwshcu_zw_params* z) {
+    if (!p && !z) return UWSHCU_ERR_ARG;
+    if (p) {
+        p->rpen = 15.0;   // uwshcu_rpen (zwshcu.F90:44)
+        p->rle = 0.1;
+        p->rkm = 10.0;    // zmh_rkm (zwshcu.F90:43, 1251)
+    }
+    if (z) {
+        *z = uwshcu_zw_params{};
+        z->lf = 1;
+        z->fwcrit = 1.0;
+        z->ent0 = (double)0.015f;   // default-real literals in the reference
+        z->ufrcd = 2.0;
+        z->ufrc0 = 2.0;
+        z->ufrcz = 2.0;
+        z->tpertmax = 2.0;
+        z->qpertmax = (double)1.0e-3f;
+    }
+    return UWSHCU_OK;
+}
+
 size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision) {
     const size_t elem = (precision == UWSHCU_FP64) ? sizeof(double) : sizeof(float);
     return uwshcu::ws_bytes_per_column(pver, ncnst, elem);
@@ -605,6 +640,39 @@ int uwshcu_forward_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* p
              umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
              qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
     return run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream);
+}
+
+int uwshcu_zw_forward_device(uwshcu_ctx* ctx, const uwshcu_params* par, const uwshcu_zw_params* zw, int pcols,
+                             int ncol, int pver, int ncnst, double dt, const double* ps0, const double* zs0,
+                             const double* p0, const double* z0, const double* dp0, const double* u0,
+                             const double* v0, const double* qv0, const double* ql0, const double* qi0,
+                             const double* t0, const double* s0, const double* tr0, const double* tke,
+                             const double* cldfrc, const double* concldfrc, const double* pblh, double* cush,
+                             const double* omega, const double* tpert, const double* qpert, double* umf,
+                             double* slflx, double* qtflx, double* flxprc1, double* flxsnow1, double* qvten,
+                             double* qlten, double* qiten, double* sten, double* uten, double* vten, double* trten,
+                             double* qrten, double* qsten, double* precip, double* snow, double* evapc,
+                             double* cufrc, double* qcu, double* qlu, double* qiu, double* cbmf, double* qc,
+                             double* rliq, double* cnt, double* cnb, const double* dpdry0, double* wu, int* diag,
+                             void* stream) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!ctx || !zw) return UWSHCU_ERR_ARG;
+    const void* req[] = {ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tke, pblh, cush, omega, tpert,
+                         qpert, umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, qrten,
+                         qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu};
+    for (const void* p : req)
+        if (!p) return UWSHCU_ERR_ARG;
+    if (ncnst > 0 && (!tr0 || !trten)) return UWSHCU_ERR_ARG;
+    uwshcu_params pz;
+    uwshcu_zw_default_params(&pz, nullptr);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Chunk ch{ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tr0, tke, pblh, dpdry0, cush,
+             umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
+             qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
+    const ZwInputs zin{zw, omega, tpert, qpert};
+    return run_device(ctx, UWSHCU_FP64, par ? par : &pz, pcols, ncol, pver, ncnst, dt, ch, diag,
+                      (hipStream_t)stream, &zin);
 }
 
 int uwshcu_forward_chunks_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int nchunk,

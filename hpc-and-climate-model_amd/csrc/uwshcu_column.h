@@ -101,7 +101,21 @@ struct FwdArgs {
     int resume;
     int slot0;
     AdArgs ad;               // adjoint kernel only
+    // the modified scheme (Column<..., true>): its options, layer omega (caller layout)
+    // and the per-column perturbations tpert and qpert
+    uwshcu_zw_params zw;
+    const double *omega, *tpert, *qpert;
 };
+
+// zmh_ramp (mzfunctions_mod.F90:222-248): 0 below x0, 1 above x0 + xint, linear in
+// between; the floor on xint is the single-precision literal 1.e-20 there.
+UWSHCU_HD inline double zmh_ramp(double x, double x0, double xint) {
+    double x2;
+    if (x < x0) x2 = 0.0;
+    else if (x > x0 + xint) x2 = 1.0;
+    else x2 = (x - x0) / fmax(xint, (double)1.e-20f);
+    return fmin(x2, 1.0);
+}
 
 // Reverse-mode state of one launch (adjoint kernel only).  Output fields are held as
 // tape values while the column runs: field f, level k of workspace slot j lives at
@@ -128,8 +142,12 @@ enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
 
 // R: bulk (plume) arithmetic type; D: the FP64 type (double, or AD in the adjoint).
-template <class R, class D = double>
+// ZW: the modified scheme compute_zwshcu (zwshcu.F90:630-4969), FP64 only; its
+// differences from compute_uwshcu are marked where they apply.
+template <class R, class D = double, bool ZW = false>
 struct Column {
+    static_assert(!ZW || (std::is_same<R, double>::value && std::is_same<D, double>::value),
+                  "the modified scheme runs in FP64");
     static constexpr bool kAD = is_ad<D>::value;
     const FwdArgs& a;
     const ThermoT<D>& T;
@@ -183,6 +201,15 @@ struct Column {
     }
     UWSHCU_HD inline D in(const double* p, int k) const {
         UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
+        if constexpr (ZW) {
+            // compute_zwshcu_inv's PBL perturbation (zwshcu.F90:547-566): tpert / qpert,
+            // capped, added to t0 / qv0 in every layer whose bottom interface is not
+            // above pblh (s0 gets cp*tpert there too; the column physics never reads s0)
+            if (a.zw.pertt && (p == a.t0 || p == a.qv0) && !(a.zs0[row(k - 1)] > a.pblh[i])) {
+                if (p == a.t0) return p[row(k)] + fmin(a.tpert[i], a.zw.tpertmax);
+                return p[row(k)] + fmin(a.qpert[i], a.zw.qpertmax);
+            }
+        }
         if constexpr (kAD) {
             const int base = p == a.t0 ? ADI_T0 : p == a.qv0 ? ADI_QV0 : p == a.u0 ? ADI_U0 : p == a.v0 ? ADI_V0
                            : p == a.tke ? ADI_TKE : -1;
@@ -429,8 +456,8 @@ struct Column {
     }
 };
 
-template <class R, class D>
-UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
+template <class R, class D, bool ZW>
+UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const {
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
@@ -688,14 +715,33 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
             thv0rel = R(thv0lcl);
         }
 
-        // ---- S9 cloud-base closure (uwshcu.F90:2092-2184), FP64 ----
+        // ---- S9 cloud-base closure (uwshcu.F90:2092-2184; zwshcu.F90:2275-2384), FP64 ----
         R wtw;
         {
-            const D wcrit = sqrt(2.0 * cin * 1.0);
-            const D sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
+            D wcrit = sqrt(2.0 * cin * 1.0);
+            D sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
+            D rho0inv = 0.0;
+            if constexpr (ZW) {
+                rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
+                if (a.zw.pertw) {
+                    // zwshcu.F90:2282-2285: a warm perturbation widens the w' distribution
+                    const double tp = fmin(a.tpert[i], a.zw.tpertmax);
+                    tkeavg = tkeavg + zmh_ramp(tp, 0.0, 2.0) * 2.0;
+                    sigmaw = sqrt(1.0 * tkeavg + 5.e-4) + zmh_ramp(tp, 0.0, 1.0) * 0.1;
+                }
+                if (a.zw.lf) {
+                    // zwshcu.F90:2296-2300: ascent at the PBL-top layer lowers wcrit
+                    UW_CHECK(kinv - 1 >= 1 && kinv - 1 <= mkx);
+                    const D tmp1 = fmin(fmax(-a.omega[row(kinv - 1)], 0.0), 3.0);
+                    wcrit = wcrit - tmp1 / rho0inv / 9.8;
+                    wcrit = fmax(wcrit, 0.0);
+                }
+                wcrit = wcrit * a.zw.fwcrit;   // zwshcu.F90:2303
+            }
             D mu = wcrit / sigmaw / 1.4142;
             if (mu >= 3.0) { exit_code = EX_MU3; goto L333; }
-            const D rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
+            if constexpr (!ZW)
+                rho0inv = in(a.ps0, kinv - 1) / (C.r * E(A_THV0TOP, kinv - 1) * E(A_EXNS0, kinv - 1));
             D cbmf_d = (rho0inv * sigmaw / 2.5066) * uexp(-(mu * mu));
             const D cbmflimit = 0.9 * in(a.dp0, kinv - 1) / C.g / dtd;
             D mumin0 = 0.0;
@@ -871,6 +917,15 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
                     W(W_REI, k) = reik;
                     ferk = reik * ee2;
                     fdrk = reik * ud2;
+                    if constexpr (ZW) {
+                        if (a.zw.ent) {
+                            // zwshcu.F90:2818-2844: entrainment from the updraft buoyancy
+                            const R bog = rbuoy * (W(W_THVU, km1) / thvebot - R(1)) * g;
+                            ferk = R(a.zw.ent0) * bog / fmax(wue * wue, R(1.0e-4));
+                            const R fer_min = bog > R(1.e-4) ? R(1.e-4) : R(0);
+                            ferk = fmin(fmax(ferk, fer_min), R(3.0e-2));
+                        }
+                    }
                     W(W_FER, k) = ferk;
                     W(W_FDR, k) = fdrk;
                     umfk = umf_km1 * uexp(dpe * (ferk - fdrk));
@@ -1507,6 +1562,19 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
                 R qcuk = R(0.5) * (qcubelow + qlr + qir);
                 R qluk = R(0.5) * (qlubelow + qlr);
                 R qiuk = R(0.5) * (qiubelow + qir);
+                R tmp2 = R(0), tmp3 = R(1);
+                if constexpr (ZW) {
+                    if (a.zw.cloud_sh) {
+                        // zwshcu.F90:4412-4429: cumulus fraction widened by detrainment
+                        tmp3 = R(0.5) * R(a.zw.ufrc0);
+                        R tmp1 = (R(in(a.z0, k)) - R(in(a.z0, krel))) / (R(in(a.z0, kpen)) - R(in(a.z0, krel)) + R(10));
+                        tmp1 = R(1) + R(a.zw.ufrcz) * tmp1;
+                        tmp3 = tmp3 * tmp1;
+                        tmp2 = W(W_FDR, k) * R(out(a.umf, k)) * R(in(a.zs0, k) - in(a.zs0, k - 1)) * dt /
+                               R(in(a.dp0, k)) * R(9.8) * R(a.zw.ufrcd);
+                        tmp2 = fmin(R(1) - rmaxfrac, tmp2);
+                    }
+                }
                 R cufrck = (W(W_UFRC, k - 1) + W(W_UFRC, k));
                 if (k == krel) {
                     cufrck = (ufrclcl + W(W_UFRC, k)) * (prel - R(in(a.ps0, k))) / R(in(a.ps0, k - 1) - in(a.ps0, k));
@@ -1521,6 +1589,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D>::run_body() const {
                 out(a.qcu, k) = D(qcuk);
                 out(a.qlu, k) = D(qluk);
                 out(a.qiu, k) = D(qiuk);
+                if constexpr (ZW) cufrck = tmp3 * cufrck + tmp2;   // zwshcu.F90:4449
                 out(a.cufrc, k) = D(cufrck);
                 qcubelow = qlr + qir;
                 qlubelow = qlr;

@@ -10,6 +10,9 @@ namespace uwshcu {
 // Launch the FP64 forward kernel over columns [args.col0, args.col0 + n) of the chunk.
 hipError_t launch_forward_fp64(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                hipStream_t stream);
+// The modified scheme compute_zwshcu (FP64; args.zw, args.omega, args.tpert, args.qpert).
+hipError_t launch_forward_zw(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                             hipStream_t stream);
 // Same for the mixed-precision kernel (FP32 bulk, FP64 saturation/CIN/fix-ups).
 hipError_t launch_forward_mixed(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                 hipStream_t stream);

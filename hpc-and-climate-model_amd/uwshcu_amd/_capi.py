@@ -83,6 +83,12 @@ class MergeFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in MERGE_FIELDS]
 
 
+class ZwParams(C.Structure):
+    """Options of the modified scheme compute_zwshcu_inv (uwshcu_zw_params)."""
+    _fields_ = [(n, C.c_int) for n in ("lf", "ent", "cloud_sh", "pertt", "pertw", "pad_")] + [
+        (n, C.c_double) for n in ("fwcrit", "ent0", "ufrcd", "ufrc0", "ufrcz", "tpertmax", "qpertmax")]
+
+
 # Argument names of the forward entry points, after (ctx, precision, par, pcols, ncol,
 # pver, ncnst, dt), in the order of compute_uwshcu_inv (uwshcu.F90:251-265).
 INPUT_NAMES = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke",
@@ -92,13 +98,18 @@ OUTPUT_NAMES = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten"
                 "cbmf", "qc", "rliq", "cnt", "cnb")
 # full ordered list of the array arguments
 ARRAY_ARGS = INPUT_NAMES + ("cush",) + OUTPUT_NAMES + ("dpdry0", "wu")
+# uwshcu_zw_forward_device: after (ctx, par, zw, pcols, ncol, pver, ncnst, dt), in the
+# order of compute_zwshcu_inv (zwshcu.F90:336-350) without its unused inputs
+ZW_INPUTS = ("omega", "tpert", "qpert")
+ZW_ARRAY_ARGS = INPUT_NAMES + ("cush",) + ZW_INPUTS + OUTPUT_NAMES + ("dpdry0", "wu")
 
 SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params", "uwshcu_ctx_create",
            "uwshcu_ctx_destroy", "uwshcu_ctx_set_workspace_limit", "uwshcu_workspace_bytes_per_column",
            "uwshcu_forward_device", "uwshcu_status", "uwshcu_forward_host", "bxcompute_uwshcu_inv_",
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
            "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity",
-           "uwshcu_forward_chunks_device", "uwshcu_merge_shallow_device")
+           "uwshcu_forward_chunks_device", "uwshcu_merge_shallow_device", "uwshcu_zw_default_params",
+           "uwshcu_zw_forward_device")
 
 _lib = None
 
@@ -135,6 +146,9 @@ def lib():
     L.uwshcu_workspace_bytes_per_column.restype = C.c_size_t
     head = [vp, C.c_int, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp]
     L.uwshcu_forward_device.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, vp]
+    L.uwshcu_zw_default_params.argtypes = [C.POINTER(Params), C.POINTER(ZwParams)]
+    L.uwshcu_zw_forward_device.argtypes = [vp, C.POINTER(Params), C.POINTER(ZwParams), C.c_int, C.c_int, C.c_int,
+                                           C.c_int, dp] + [vp] * len(ZW_ARRAY_ARGS) + [vp, vp]
     L.uwshcu_forward_host.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, ip]
     L.uwshcu_status.argtypes = [vp, vp, ip]
     L.uwshcu_set_dropin_params.argtypes = [C.POINTER(Params)]
@@ -168,6 +182,18 @@ def build_estbl(c: Consts | None = None):
 
 def default_params(rpen=10.0, rle=0.1, rkm=14.0) -> Params:
     return Params(rpen, rle, rkm)
+
+
+def zw_default_params(**kw) -> tuple:
+    """(Params, ZwParams) with the modified scheme's defaults (uwshcu_zw_default_params);
+    keyword arguments override ZwParams fields."""
+    p, z = Params(), ZwParams()
+    rc = lib().uwshcu_zw_default_params(C.byref(p), C.byref(z))
+    if rc:
+        raise UwshcuError(rc, what="uwshcu_zw_default_params")
+    for k, v in kw.items():
+        setattr(z, k, v)
+    return p, z
 
 
 def workspace_bytes_per_column(pver, ncnst, precision=FP64) -> int:

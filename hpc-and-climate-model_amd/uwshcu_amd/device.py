@@ -98,6 +98,52 @@ class Context:
         if rc:
             raise UwshcuError(rc, what="uwshcu_forward_device")
 
+    def zw_forward_into(self, inp: dict, out: dict, cush: torch.Tensor, zw: _capi.ZwParams, *,
+                        ncol: int | None = None, dt: float = 1800.0, params: _capi.Params | None = None,
+                        stream=None) -> None:
+        """The modified scheme compute_zwshcu_inv (uwshcu_zw_forward_device, FP64):
+        ``inp`` also holds ``omega`` [pver, pcols], ``tpert`` and ``qpert`` [pcols].
+        Writes ``out`` and ``cush`` in place (asynchronous)."""
+        pver, pcols = inp["p0"].shape
+        ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
+        ncol = pcols if ncol is None else ncol
+        par = params or _capi.zw_default_params()[0]
+        ptrs = {}
+        for k in _capi.INPUT_NAMES + _capi.ZW_INPUTS + ("dpdry0",):
+            t = inp[k]
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+            ptrs[k] = t.data_ptr()
+        if inp["omega"].shape != inp["p0"].shape or inp["tpert"].shape != (pcols,) or inp["qpert"].shape != (pcols,):
+            raise ValueError("omega must be [pver, pcols]; tpert and qpert [pcols]")
+        ptrs["cush"] = cush.data_ptr()
+        for k in _capi.OUTPUT_NAMES + ("wu",):
+            ptrs[k] = out[k].data_ptr()
+        diag = out.get("diag")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        args = [self._h, C.byref(par), C.byref(zw), pcols, ncol, pver, ncnst, C.c_double(dt)]
+        args += [C.c_void_p(ptrs[k]) for k in _capi.ZW_ARRAY_ARGS]
+        args += [C.c_void_p(diag.data_ptr() if diag is not None else 0), C.c_void_p(sp)]
+        rc = _capi.lib().uwshcu_zw_forward_device(*args)
+        if rc:
+            raise UwshcuError(rc, what="uwshcu_zw_forward_device")
+
+    def compute_zwshcu_inv(self, inp: dict, zw: _capi.ZwParams | None = None, *, dt: float = 1800.0,
+                           params: _capi.Params | None = None, diag: bool = False,
+                           ncol: int | None = None) -> dict:
+        """Run the modified scheme on the chunk and return the outputs (synchronous)."""
+        pver, pcols = inp["p0"].shape
+        ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
+        out = self.alloc_outputs(pcols, pver, ncnst, diag)
+        cush = inp["cush"].clone()
+        self.zw_forward_into(inp, out, cush, zw if zw is not None else _capi.zw_default_params()[1], ncol=ncol,
+                             dt=dt, params=params)
+        self.check()
+        out["cush"] = cush
+        return out
+
     @staticmethod
     def chunk_table(chunks: list):
         """The descriptor table of uwshcu_forward_chunks_device for ``chunks``, a list of

@@ -190,6 +190,23 @@ def make_chunk(ncol: int, pver: int = 30, ncnst: int = 3, seed: int = 0, dt: flo
     return Chunk(ncol, pver, ncnst, dt, f)
 
 
+def add_zw_inputs(ch: Chunk, seed: int = 1) -> Chunk:
+    """Add the extra inputs of the modified scheme (compute_zwshcu_inv): layer
+    pressure velocity ``omega`` [Pa/s] (CAM level order), and the PBL perturbations
+    ``tpert`` [K] and ``qpert`` [kg/kg].
+
+    omega is a per-column large-scale value U(-1.5, 0.5) (negative = ascent) plus
+    N(0, 0.3) noise per layer; tpert ~ U(-0.5, 3) and qpert ~ U(-2e-4, 2e-3), so that
+    some columns exceed the scheme's default caps (2 K and 1e-3)."""
+    rng = np.random.default_rng(np.random.PCG64(seed + 7919))
+    n, nl = ch.ncol, ch.pver
+    mean = rng.uniform(-1.5, 0.5, n)
+    ch.fields["omega"] = np.ascontiguousarray(mean[None, :] + rng.normal(0.0, 0.3, (nl, n)))
+    ch.fields["tpert"] = rng.uniform(-0.5, 3.0, n)
+    ch.fields["qpert"] = rng.uniform(-2.0e-4, 2.0e-3, n)
+    return ch
+
+
 def algorithmic_bytes_per_column(pver: int, ncnst: int) -> int:
     """FP64 bytes one column must move: inputs the physics reads plus the full output set.
 

@@ -53,6 +53,13 @@ int uwshcu_build_estbl(const uwshcu_consts *c, double *estbl);
 /* Default tunables: rpen = 10 (namelist uwshcu_rpen), rle = 0.1, rkm = 14. */
 int uwshcu_default_params(uwshcu_params *p);
 
+/* Defaults of the modified scheme compute_zwshcu_inv (zwshcu.F90:30-57, 1251):
+ * p (nullable) gets rpen = 15 (uwshcu_rpen), rle = 0.1, rkm = 10 (zmh_rkm); z
+ * (nullable) gets lf = 1, every other flag 0, fwcrit = 1, ent0 = 0.015f,
+ * ufrcd = ufrc0 = ufrcz = 2, tpertmax = 2, qpertmax = 1.0e-3f (the f values are
+ * single-precision literals in the reference, taken as the nearest floats). */
+int uwshcu_zw_default_params(uwshcu_params *p, uwshcu_zw_params *z);
+
 /* Create a context on a GPU (device < 0: the current device).  c == NULL uses
  * uwshcu_default_consts; estbl == NULL builds the table from c.  The context owns
  * the device copies of the constants and the table, a status word and a
@@ -86,6 +93,30 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
     const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
     const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
     const double *concldfrc, const double *pblh, double *cush,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
+
+/* The modified scheme compute_zwshcu_inv (zwshcu.F90:336-628; caller
+ * convect_shallow.F90:717-733) on device arrays, FP64: the arguments of
+ * uwshcu_forward_device without the precision, plus the options `zw`
+ * (uwshcu_types.h), omega (pcols, pver: layer pressure velocity, Pa/s, CAM level
+ * order) and tpert, qpert (pcols).  par == NULL uses uwshcu_zw_default_params.
+ * The reference's remaining inputs (gradt, gradq, landfrac, lat, lon, massflx_sh)
+ * feed only its frontal/terrain/breeze increments of tpert and qpert (which need a
+ * solar zenith angle and the sub-grid orography of the host model,
+ * zwshcu.F90:511-545) and its prognostic closure: the caller passes tpert and
+ * qpert with any such increments already added.  Asynchronous like
+ * uwshcu_forward_device; uwshcu_status reports fatal conditions. */
+int uwshcu_zw_forward_device(uwshcu_ctx *ctx, const uwshcu_params *par, const uwshcu_zw_params *zw,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    const double *omega, const double *tpert, const double *qpert,
     double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
     double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
     double *qrten, double *qsten, double *precip, double *snow, double *evapc,

@@ -32,6 +32,24 @@ typedef struct uwshcu_params {
     double rpen, rle, rkm;
 } uwshcu_params;
 
+// Options of the modified scheme compute_zwshcu_inv (zwshcu.F90:30-57, namelist
+// zwshcu_nl zwshcu.F90:100-107).  Flags are 0 or 1; the reference defaults are in
+// brackets.  cloud_sh is cloud_fraction's zmh_cloud_sh (cloud_fraction.F90:52), which
+// the ZMH_WXC and WXC macrophysics switch on (cloud_fraction.F90:204-207).
+typedef struct uwshcu_zw_params {
+    int lf;          // zmh_shallow_lf [1]: large-scale ascent lowers the critical updraft speed
+    int ent;         // zmh_shallow_ent [0]: buoyancy-based lateral entrainment rate
+    int cloud_sh;    // zmh_cloud_sh [0]: cumulus fraction widened by detrainment
+    int pertt;       // zmh_shallow_pertt [0]: tpert / qpert added to the PBL layers
+    int pertw;       // zmh_shallow_pertw [0]: tpert widens the updraft-speed distribution
+    int pad_;
+    double fwcrit;   // zmh_shallow_fwcrit [1.0]: factor on the critical updraft speed
+    double ent0;     // zmh_ent0 [0.015 as a single-precision literal]
+    double ufrcd, ufrc0, ufrcz;   // zmh_ufrcd, zmh_ufrc0, zmh_ufrcz [2.0 each]
+    double tpertmax; // zmh_shallow_tpertmax [2.0]: cap on tpert
+    double qpertmax; // zmh_shallow_qpertmax [1.0e-3 as a single-precision literal]: cap on qpert
+} uwshcu_zw_params;
+
 // Precision modes.
 enum {
     UWSHCU_FP64  = 0,   // all arithmetic in FP64

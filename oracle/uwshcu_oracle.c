@@ -5,6 +5,9 @@
  * Fortran computes it:
  *   compute_uwshcu_inv            uwshcu.F90:251-476
  *   compute_uwshcu (column loop)  uwshcu.F90:478-4664
+ *   compute_zwshcu_inv / compute_zwshcu, the modified scheme:
+ *                                 zwshcu.F90:336-628, 630-4969 (its differences
+ *                                 from compute_uwshcu are marked where they apply)
  *   helpers                       uwshcu.F90:4672-5454
  *   estblf/gestbl/vqsatd/fqsatd   wv_saturation.F90:64-204,482-609,734-751
  *   gffgch                        gffgch.F90:1-127, parameters esinti.F90:39-51
@@ -544,6 +547,8 @@ typedef struct colws {
     double qv0[KMAX], ql0[KMAX], qi0[KMAX], t0[KMAX], s0[KMAX], qw0[KMAX];
     double tr0[NT][KMAX];
     double pblh, cush;
+    /* compute_zwshcu only: omega (layers) and the capped tpert */
+    double omega[KMAX], tpert;
     /* environment */
     double qt0[KMAX], thl0[KMAX], thvl0[KMAX], ssqt0[KMAX], ssthl0[KMAX], ssu0[KMAX], ssv0[KMAX];
     double thv0bot[KMAX], thv0top[KMAX], thvl0bot[KMAX], thvl0top[KMAX], exn0[KMAX], exns0[KMAX];
@@ -590,8 +595,20 @@ enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4,
 
 #define THV(thj, qvj, qlj, qij) ((thj) * (1.0 + c->zvir * (qvj) - (qlj) - (qij)))
 
-/* returns 0 or -3 (mu > mumin2) */
-static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
+/* zmh_ramp (mzfunctions_mod.F90:222-248): 0 below x0, 1 above x0 + xint, linear
+ * in between.  The floor on xint is the single-precision literal 1.e-20. */
+static double zmh_ramp(double x, double x0, double xint)
+{
+    double x2;
+    if (x < x0) x2 = 0.0;
+    else if (x > x0 + xint) x2 = 1.0;
+    else x2 = (x - x0) / dmax(xint, (double)1.e-20f);
+    return dmin(x2, 1.0);
+}
+
+/* One column of compute_uwshcu, or of compute_zwshcu when zw is not NULL.
+ * Returns 0 or -3 (mu > mumin2). */
+static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, const orc_zw *zw)
 {
     /* uwshcu.F90:1040-1068 parameters */
     const double rkfre = 1.0, rmaxfrac = 0.10, mumin1 = 0.906, rbuoy = 1.0, rdrag = 1.0;
@@ -637,7 +654,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
     double thl0lcl, qt0lcl, thv0lcl = 0, thv0rel = 0, rho0inv, aquad, bquad, cquad, xc1, xc2;
     double excessu, excess0, xsat, xs1, xs2, bogbot = 0, bogtop = 0, delbog, drage = 0, expfac;
     double qcubelow, qlubelow = 0, qiubelow = 0, rainflx, snowflx, es, qs, gam, qsat_arg;
-    double xsrc, xmean, xtop, xbot, tmp1, tmp2;
+    double xsrc, xmean, xtop, xbot, tmp1, tmp2, tmp3;
     double thlten_sub, qtten_sub, qlten_sub, qiten_sub, nlten_sub, niten_sub, thl_prog, qt_prog;
     double qc_lm, qc_im, nc_lm, nc_im, ql_emf_kbup, qi_emf_kbup, nl_emf_kbup, ni_emf_kbup;
     double snowmlt, subsat, evprain, evpsnow, evplimit, evplimit_rain, evplimit_snow, evpint_rain, evpint_snow;
@@ -876,12 +893,32 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
             thv0rel = thv0lcl;
         }
 
-        /* ---- S9 cloud-base closure (uwshcu.F90:2092-2184) ---- */
-        wcrit = sqrt(2.0 * cin * rbuoy);
-        sigmaw = sqrt(rkfre * tkeavg + epsvarw);
-        mu = wcrit / sigmaw / 1.4142;
-        if (mu >= 3.0) { exit_code = EX_MU3; id_exit = 1; goto L333; }
-        rho0inv = ps0[kinv - 1] / (r * thv0top[kinv - 1] * exns0[kinv - 1]);
+        /* ---- S9 cloud-base closure (uwshcu.F90:2092-2184; zwshcu.F90:2275-2384) ---- */
+        if (!zw) {
+            wcrit = sqrt(2.0 * cin * rbuoy);
+            sigmaw = sqrt(rkfre * tkeavg + epsvarw);
+            mu = wcrit / sigmaw / 1.4142;
+            if (mu >= 3.0) { exit_code = EX_MU3; id_exit = 1; goto L333; }
+            rho0inv = ps0[kinv - 1] / (r * thv0top[kinv - 1] * exns0[kinv - 1]);
+        } else {
+            rho0inv = ps0[kinv - 1] / (r * thv0top[kinv - 1] * exns0[kinv - 1]);
+            sigmaw = sqrt(rkfre * tkeavg + epsvarw);
+            if (zw->pertw) {
+                /* zwshcu.F90:2282-2285 */
+                tkeavg = tkeavg + zmh_ramp(W->tpert, 0.0, 2.0) * 2.0;
+                sigmaw = sqrt(rkfre * tkeavg + epsvarw) + zmh_ramp(W->tpert, 0.0, 1.0) * 0.1;
+            }
+            wcrit = sqrt(2.0 * cin * rbuoy);
+            if (zw->lf) {
+                /* zwshcu.F90:2296-2300: ascent at the PBL-top layer lowers wcrit */
+                tmp1 = dmin(dmax(-W->omega[kinv - 1], 0.0), 3.0);
+                wcrit = wcrit - tmp1 / rho0inv / 9.8;
+                wcrit = dmax(wcrit, 0.0);
+            }
+            wcrit = wcrit * zw->fwcrit;   /* zwshcu.F90:2303 */
+            mu = wcrit / sigmaw / 1.4142;
+            if (mu >= 3.0) { exit_code = EX_MU3; id_exit = 1; goto L333; }
+        }
         cbmf = (rho0inv * sigmaw / 2.5066) * XEXP(-(mu * mu));
         {
             double cbmflimit = 0.9 * dp0[kinv - 1] / g / dt;
@@ -1033,6 +1070,15 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                         rei[k] = dmin(rei[k], 0.9 * XLOG(dp0[k] / g / dt / umf[km1] + 1.0) / dpe / (2.0 * xc - 1.0));
                     fer[k] = rei[k] * ee2;
                     fdr[k] = rei[k] * ud2;
+                    if (zw && zw->ent) {
+                        /* zwshcu.F90:2818-2844: entrainment from the updraft buoyancy */
+                        double fer_min;
+                        bogbot = rbuoy * (thvu[km1] / thvebot - 1.0) * g;
+                        fer[k] = zw->ent0 * bogbot / dmax(wue * wue, 1.0e-4);
+                        fer_min = 0.0;
+                        if (bogbot > 1.e-4) fer_min = 1.e-4;
+                        fer[k] = dmin(dmax(fer[k], fer_min), 3.0e-2);
+                    }
                     umf[k] = umf[km1] * XEXP(dpe * (fer[k] - fdr[k]));
                     emf[k] = 0.0;
                     if (fer[k] * dpe < 1.e-4) {
@@ -1546,6 +1592,17 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
             qcu[k] = 0.5 * (qcubelow + qlj + qij);
             qlu[k] = 0.5 * (qlubelow + qlj);
             qiu[k] = 0.5 * (qiubelow + qij);
+            /* zwshcu.F90:4412-4429: widened cumulus fraction (zmh_cloud_sh) */
+            tmp2 = 0.0;
+            tmp3 = 1.0;
+            if (zw && zw->cloud_sh) {
+                tmp3 = 0.5 * zw->ufrc0;
+                tmp1 = (z0[k] - z0[krel]) / (z0[kpen] - z0[krel] + 10.0);
+                tmp1 = 1.0 + zw->ufrcz * tmp1;
+                tmp3 = tmp3 * tmp1;
+                tmp2 = fdr[k] * umf[k] * (zs0[k] - zs0[k - 1]) * dt / dp0[k] * 9.8 * zw->ufrcd;
+                tmp2 = dmin(1.0 - rmaxfrac, tmp2);
+            }
             cufrc[k] = (ufrc[k - 1] + ufrc[k]);
             if (k == krel) {
                 cufrc[k] = (ufrclcl + ufrc[k]) * (prel - ps0[k]) / (ps0[k - 1] - ps0[k]);
@@ -1557,6 +1614,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt)
                     qiu[k] = 0.5 * (qiubelow + criqc * qij / (qlj + qij));
                 }
             }
+            if (zw) cufrc[k] = tmp3 * cufrc[k] + tmp2;   /* zwshcu.F90:4449 */
             qcubelow = qlj + qij;
             qlubelow = qlj;
             qiubelow = qij;
@@ -1667,12 +1725,17 @@ Ldone:
 /* compute_uwshcu_inv (uwshcu.F90:251-476)                                    */
 /* ------------------------------------------------------------------------- */
 
-int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int ncnst, double dt,
+/* compute_uwshcu_inv, or compute_zwshcu_inv (zwshcu.F90:336-628) when zw is not
+ * NULL.  The zwshcu wrapper caps tpert and qpert (zwshcu.F90:547-551) and, with
+ * pertt, adds them to t0, qv0 and s0 (cp * tpert) in every UW layer whose bottom
+ * interface height zs0(k-1) is not above pblh (zwshcu.F90:556-566), before the
+ * column physics and its wet-bulb computation see the inputs. */
+static int inv_common(const orc_consts *c, const orc_zw *zw, int mix, int mkx, int iend, int ncnst, double dt,
     const double *ps0_inv, const double *zs0_inv, const double *p0_inv, const double *z0_inv,
     const double *dp0_inv, const double *u0_inv, const double *v0_inv, const double *qv0_inv,
     const double *ql0_inv, const double *qi0_inv, const double *t0_inv, const double *s0_inv,
-    const double *tr0_inv, const double *tke_inv, const double *cldfrct_inv,
-    const double *concldfrct_inv, const double *pblh, double *cush,
+    const double *tr0_inv, const double *tke_inv, const double *pblh, double *cush,
+    const double *omega_inv, const double *tpert, const double *qpert,
     double *umf_inv, double *slflx_inv, double *qtflx_inv, double *flxprc1_inv, double *flxsnow1_inv,
     double *qvten_inv, double *qlten_inv, double *qiten_inv, double *sten_inv, double *uten_inv,
     double *vten_inv, double *trten_inv, double *qrten_inv, double *qsten_inv, double *precip,
@@ -1680,7 +1743,6 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
     double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
     const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col)
 {
-    (void)cldfrct_inv; (void)concldfrct_inv;
     int ntr = ncnst;
     if (c->ixnumliq > ntr) ntr = c->ixnumliq;
     if (c->ixnumice > ntr) ntr = c->ixnumice;
@@ -1688,15 +1750,25 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
         if (bad_col) *bad_col = 0;
         return -4;
     }
+    if (zw && (!omega_inv || !tpert || !qpert)) {
+        if (bad_col) *bad_col = 0;
+        return -4;
+    }
     colws *W = (colws *)calloc(1, sizeof(colws));
     int rc = 0;
 #define MID(a, i, kk) (a)[(size_t)((mkx + 1 - (kk)) - 1) * mix + (i)]
 #define INT(a, i, kk) (a)[(size_t)((mkx + 1 - (kk)) - 1) * mix + (i)]
+    /* the zwshcu input perturbation of column i, UW layer k: 1 where it applies */
+#define PERT(i, kk) (zw && zw->pertt && !(zs0_inv[(size_t)(mkx - ((kk) - 1)) * mix + (i)] > pblh[i]))
+#define TPERT(i) dmin(tpert[i], zw->tpertmax)
+#define QPERT(i) dmin(qpert[i], zw->qpertmax)
     /* findsp over the whole chunk first (uwshcu.F90:1216), in UW level order */
     for (int i = 0; i < iend && rc == 0; ++i) {
         for (int k = 1; k <= mkx; ++k) {
             double tsp, qsp, enerr;
-            int doit = orc_findsp_point(c, MID(qv0_inv, i, k), MID(t0_inv, i, k), MID(p0_inv, i, k), &tsp, &qsp, &enerr);
+            double qvk = MID(qv0_inv, i, k), tk = MID(t0_inv, i, k);
+            if (PERT(i, k)) { qvk = qvk + QPERT(i); tk = tk + TPERT(i); }
+            int doit = orc_findsp_point(c, qvk, tk, MID(p0_inv, i, k), &tsp, &qsp, &enerr);
             if (doit == 0) { rc = -1; if (bad_col) *bad_col = i; break; }
             if (doit == 2 && enerr > 1.e-4) { rc = -2; if (bad_col) *bad_col = i; break; }
         }
@@ -1710,6 +1782,12 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
             W->z0[k] = MID(z0_inv, i, k); W->dp0[k] = MID(dp0_inv, i, k); W->dpdry0[k] = MID(dpdry0_inv, i, k);
             W->qv0[k] = MID(qv0_inv, i, k); W->ql0[k] = MID(ql0_inv, i, k); W->qi0[k] = MID(qi0_inv, i, k);
             W->t0[k] = MID(t0_inv, i, k); W->s0[k] = MID(s0_inv, i, k);
+            if (zw) W->omega[k] = MID(omega_inv, i, k);
+            if (PERT(i, k)) {
+                W->qv0[k] = W->qv0[k] + QPERT(i);
+                W->t0[k] = W->t0[k] + TPERT(i);
+                W->s0[k] = W->s0[k] + c->cp * TPERT(i);
+            }
             for (int m = 1; m <= ncnst; ++m)
                 W->tr0[m][k] = tr0_inv[((size_t)(m - 1) * mkx + (mkx - k)) * mix + i];
             double tsp, qsp, enerr;
@@ -1724,7 +1802,8 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
         }
         W->pblh = pblh[i];
         W->cush = cush[i];
-        int crc = column(c, W, mkx, ntr, dt);
+        W->tpert = zw ? TPERT(i) : 0.0;
+        int crc = column(c, W, mkx, ntr, dt, zw);
         if (crc != 0 && rc == 0) { rc = crc; if (bad_col) *bad_col = i; }
         /* flip back, uwshcu.F90:434-469 */
         cush[i] = W->cush;
@@ -1751,6 +1830,54 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
     }
 #undef MID
 #undef INT
+#undef PERT
+#undef TPERT
+#undef QPERT
     free(W);
     return rc;
+}
+
+int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int ncnst, double dt,
+    const double *ps0_inv, const double *zs0_inv, const double *p0_inv, const double *z0_inv,
+    const double *dp0_inv, const double *u0_inv, const double *v0_inv, const double *qv0_inv,
+    const double *ql0_inv, const double *qi0_inv, const double *t0_inv, const double *s0_inv,
+    const double *tr0_inv, const double *tke_inv, const double *cldfrct_inv,
+    const double *concldfrct_inv, const double *pblh, double *cush,
+    double *umf_inv, double *slflx_inv, double *qtflx_inv, double *flxprc1_inv, double *flxsnow1_inv,
+    double *qvten_inv, double *qlten_inv, double *qiten_inv, double *sten_inv, double *uten_inv,
+    double *vten_inv, double *trten_inv, double *qrten_inv, double *qsten_inv, double *precip,
+    double *snow, double *evapc_inv, double *cufrc_inv, double *qcu_inv, double *qlu_inv,
+    double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
+    const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col)
+{
+    (void)cldfrct_inv; (void)concldfrct_inv;
+    return inv_common(c, NULL, mix, mkx, iend, ncnst, dt, ps0_inv, zs0_inv, p0_inv, z0_inv, dp0_inv, u0_inv,
+        v0_inv, qv0_inv, ql0_inv, qi0_inv, t0_inv, s0_inv, tr0_inv, tke_inv, pblh, cush, NULL, NULL, NULL,
+        umf_inv, slflx_inv, qtflx_inv, flxprc1_inv, flxsnow1_inv, qvten_inv, qlten_inv, qiten_inv, sten_inv,
+        uten_inv, vten_inv, trten_inv, qrten_inv, qsten_inv, precip, snow, evapc_inv, cufrc_inv, qcu_inv,
+        qlu_inv, qiu_inv, cbmf, qc_inv, rliq, cnt_inv, cnb_inv, dpdry0_inv, wu_inv, diag, bad_col);
+}
+
+int orc_compute_zwshcu_inv(const orc_consts *c, const orc_zw *zw, int mix, int mkx, int iend, int ncnst,
+    double dt, const double *ps0_inv, const double *zs0_inv, const double *p0_inv, const double *z0_inv,
+    const double *dp0_inv, const double *u0_inv, const double *v0_inv, const double *qv0_inv,
+    const double *ql0_inv, const double *qi0_inv, const double *t0_inv, const double *s0_inv,
+    const double *tr0_inv, const double *tke_inv, const double *pblh, double *cush,
+    const double *omega_inv, const double *tpert, const double *qpert,
+    double *umf_inv, double *slflx_inv, double *qtflx_inv, double *flxprc1_inv, double *flxsnow1_inv,
+    double *qvten_inv, double *qlten_inv, double *qiten_inv, double *sten_inv, double *uten_inv,
+    double *vten_inv, double *trten_inv, double *qrten_inv, double *qsten_inv, double *precip,
+    double *snow, double *evapc_inv, double *cufrc_inv, double *qcu_inv, double *qlu_inv,
+    double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
+    const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col)
+{
+    if (!zw) {
+        if (bad_col) *bad_col = 0;
+        return -4;
+    }
+    return inv_common(c, zw, mix, mkx, iend, ncnst, dt, ps0_inv, zs0_inv, p0_inv, z0_inv, dp0_inv, u0_inv,
+        v0_inv, qv0_inv, ql0_inv, qi0_inv, t0_inv, s0_inv, tr0_inv, tke_inv, pblh, cush, omega_inv, tpert, qpert,
+        umf_inv, slflx_inv, qtflx_inv, flxprc1_inv, flxsnow1_inv, qvten_inv, qlten_inv, qiten_inv, sten_inv,
+        uten_inv, vten_inv, trten_inv, qrten_inv, qsten_inv, precip, snow, evapc_inv, cufrc_inv, qcu_inv,
+        qlu_inv, qiu_inv, cbmf, qc_inv, rliq, cnt_inv, cnb_inv, dpdry0_inv, wu_inv, diag, bad_col);
 }

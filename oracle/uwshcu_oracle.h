@@ -96,6 +96,37 @@ int orc_compute_uwshcu_inv(const orc_consts *c, int mix, int mkx, int iend, int 
     double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
     const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col);
 
+/*
+ * The modified scheme compute_zwshcu_inv (zwshcu.F90:336-628 -> compute_zwshcu
+ * zwshcu.F90:630-4969): the same column physics with the options below
+ * (zwshcu.F90:30-57; zmh_cloud_sh is cloud_fraction.F90:52).  Flags are 0 / 1.
+ */
+typedef struct orc_zw {
+    int lf, ent, cloud_sh, pertt, pertw, pad_;
+    double fwcrit, ent0, ufrcd, ufrc0, ufrcz, tpertmax, qpertmax;
+} orc_zw;
+
+/*
+ * compute_zwshcu_inv with the arguments of orc_compute_uwshcu_inv plus
+ * omega_inv (mix, mkx) in CAM level order and tpert, qpert (mix).  The other
+ * inputs of the reference (gradt, gradq, landfrac, lat, lon, massflx_sh) feed
+ * only options this restatement does not carry (zwshcu.F90:518-540, 2319,
+ * 2382) and are not taken.  rpen and rkm come from orc_consts (the reference
+ * reads uwshcu_rpen and zmh_rkm from its namelist, zwshcu.F90:43-44,1251).
+ */
+int orc_compute_zwshcu_inv(const orc_consts *c, const orc_zw *zw, int mix, int mkx, int iend, int ncnst,
+    double dt, const double *ps0_inv, const double *zs0_inv, const double *p0_inv, const double *z0_inv,
+    const double *dp0_inv, const double *u0_inv, const double *v0_inv, const double *qv0_inv,
+    const double *ql0_inv, const double *qi0_inv, const double *t0_inv, const double *s0_inv,
+    const double *tr0_inv, const double *tke_inv, const double *pblh, double *cush,
+    const double *omega_inv, const double *tpert, const double *qpert,
+    double *umf_inv, double *slflx_inv, double *qtflx_inv, double *flxprc1_inv, double *flxsnow1_inv,
+    double *qvten_inv, double *qlten_inv, double *qiten_inv, double *sten_inv, double *uten_inv,
+    double *vten_inv, double *trten_inv, double *qrten_inv, double *qsten_inv, double *precip,
+    double *snow, double *evapc_inv, double *cufrc_inv, double *qcu_inv, double *qlu_inv,
+    double *qiu_inv, double *cbmf, double *qc_inv, double *rliq, double *cnt_inv, double *cnb_inv,
+    const double *dpdry0_inv, double *wu_inv, int *diag, int *bad_col);
+
 #ifdef __cplusplus
 }
 #endif

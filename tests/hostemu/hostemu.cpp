@@ -24,17 +24,26 @@ extern "C" int hostemu_last_deferred() { return g_deferred; }
 static int g_window = 0;
 extern "C" void hostemu_set_window(int w) { g_window = w; }
 extern "C" void hostemu_set_staged(int on) { g_staged = on; }
+// the modified scheme (Column<double, double, true>) for the next hostemu_forward
+// calls while set; zw == nullptr returns to the UW scheme
+static const uwshcu_zw_params* g_zw = nullptr;
+static const double *g_omega = nullptr, *g_tpert = nullptr, *g_qpert = nullptr;
+extern "C" void hostemu_set_zw(const uwshcu_zw_params* zw, const double* omega, const double* tpert,
+                               const double* qpert) {
+    g_zw = zw; g_omega = omega; g_tpert = tpert; g_qpert = qpert;
+}
 
-template <class R>
+template <class R, bool ZW = false>
 static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
+    using Col = Column<R, double, ZW>;
     if (g_staged) {
         a.env_ready = 1;
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < ncol; ++i)
-            for (int k = 0; k <= pver; ++k) Column<R>(a, T, i, i).setup_level1(k);
+            for (int k = 0; k <= pver; ++k) Col(a, T, i, i).setup_level1(k);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < ncol; ++i)
-            for (int k = 1; k <= pver; ++k) Column<R>(a, T, i, i).setup_level2(k);
+            for (int k = 1; k <= pver; ++k) Col(a, T, i, i).setup_level2(k);
     }
     std::vector<int> deferred((size_t)ncol);
     std::vector<double> state((size_t)NRESUME * (size_t)a.wld, -777.0);
@@ -45,7 +54,7 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
         a.defer_state = state.data();
     }
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < ncol; ++i) Column<R>(a, T, i, i).run();
+    for (int i = 0; i < ncol; ++i) Col(a, T, i, i).run();
     a.env_ready = 0;
     a.defer_list = nullptr;
     a.defer_count = nullptr;
@@ -56,7 +65,7 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
         a.resume = 1;
         a.slot0 = 0;
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int j = 0; j < ndefer; ++j) Column<R>(a, T, deferred[j], deferred[j]).run();
+        for (int j = 0; j < ndefer; ++j) Col(a, T, deferred[j], deferred[j]).run();
         g_deferred = ndefer;
         a.resume = 0;
         a.env_ready = 0;
@@ -99,8 +108,15 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     std::vector<double> cush_in(cush, cush + ncol);
     a.amb_flag = fp32 ? amb.data() : nullptr;
     const Thermo T{c, estbl};
-    if (!fp32) run_columns<double>(a, T, ncol, pver);
-    else run_columns<float>(a, T, ncol, pver);
+    if (g_zw) {
+        if (fp32) return UWSHCU_ERR_ARG;
+        a.zw = *g_zw; a.omega = g_omega; a.tpert = g_tpert; a.qpert = g_qpert;
+        run_columns<double, true>(a, T, ncol, pver);
+    } else if (!fp32) {
+        run_columns<double>(a, T, ncol, pver);
+    } else {
+        run_columns<float>(a, T, ncol, pver);
+    }
     int nlist = 0;
     if (precision == UWSHCU_MIXED) {
         std::vector<int> list;

@@ -57,13 +57,14 @@ def _p(a):
 
 
 def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, checked=False, stats=None,
-        staged=False, defer=False):
+        staged=False, defer=False, zw=None):
     """Run the host-compiled column code on a generator chunk.
 
     ``staged`` runs the level-parallel set-up first, as the GPU pipeline does;
     ``defer`` stops the columns that need a second convection pass and reruns them
     afterwards from a list, as the GPU pipeline does for launches of 131072 columns
-    and more.
+    and more.  ``zw`` (a ``_capi.ZwParams``) runs the modified scheme instead, on a
+    chunk that carries ``omega``, ``tpert`` and ``qpert``.
 
     Returns (outputs dict, diag [ncol, NDIAG], rc, bad_col) like ``oracle.run``.
     The constants and table come from the product library's host functions, which
@@ -97,7 +98,13 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
     L = lib(checked)
     L.hostemu_set_staged(1 if staged else 0)
     L.hostemu_set_defer(1 if defer else 0)
-    rc = L.hostemu_forward(*args)
+    if zw is not None:
+        zins = [np.ascontiguousarray(f[k]) for k in ("omega", "tpert", "qpert")]
+        L.hostemu_set_zw(C.byref(zw), *[_p(a) for a in zins])
+    try:
+        rc = L.hostemu_forward(*args)
+    finally:
+        L.hostemu_set_zw(None, None, None, None)
     L.hostemu_set_staged(0)
     L.hostemu_set_defer(0)
     if stats is not None:

@@ -38,6 +38,17 @@ def test_struct_layout_matches_header():
     # uwshcu_chunk: two ints, then 44 array pointers
     assert C.sizeof(_capi.ChunkDesc) == 8 + 8 * 44
     assert _capi.ChunkDesc.ps0.offset == 8 and _capi.ChunkDesc.cnb.offset == 8 + 8 * 43
+    # uwshcu_zw_params: six ints, then seven doubles (same layout as the oracle's orc_zw)
+    assert C.sizeof(_capi.ZwParams) == 4 * 6 + 8 * 7 == C.sizeof(oracle.ZW)
+    assert _capi.ZwParams.fwcrit.offset == 24 and _capi.ZwParams.qpertmax.offset == 72
+
+
+def test_zw_defaults_equal_the_oracle_defaults():
+    par, z = _capi.zw_default_params()
+    assert (par.rpen, par.rle, par.rkm) == (15.0, 0.1, 10.0)
+    o = oracle.zw_defaults()
+    for name, _ in _capi.ZwParams._fields_:
+        assert getattr(z, name) == getattr(o, name), name
 
 
 def test_default_constants_equal_the_oracle():
@@ -74,6 +85,11 @@ def test_device_entry_points_reject_bad_arguments_without_a_gpu():
     rc = lib.uwshcu_forward_device(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
     assert rc == -10
     rc = lib.uwshcu_forward_host(None, 0, C.byref(par), 10, 10, 30, 3, C.c_double(1800.0), *nul, None, None)
+    assert rc == -10
+    _, z = _capi.zw_default_params()
+    nulz = [C.c_void_p(0)] * len(_capi.ZW_ARRAY_ARGS)
+    rc = lib.uwshcu_zw_forward_device(None, C.byref(par), C.byref(z), 10, 10, 30, 3, C.c_double(1800.0), *nulz,
+                                      None, None)
     assert rc == -10
     desc = (_capi.ChunkDesc * 2)()
     rc = lib.uwshcu_forward_chunks_device(None, 0, C.byref(par), 2, desc, 30, 3, C.c_double(1800.0), None)
