@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--pver", type=int, default=35)
     ap.add_argument("--ncnst", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--scheme", choices=["uw", "zw"], default="uw",
+                    help="uw: compute_uwshcu_inv; zw: the modified scheme compute_zwshcu_inv (FP64, default options)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--inversion-frac", type=float, default=0.25)
     ap.add_argument("--cpu-sample", type=int, default=8192, help="columns timed on the CPU oracle")
@@ -115,15 +117,21 @@ def main():
     import torch
 
     from uwshcu_amd import _capi, device
-    from uwshcu_amd.generator import algorithmic_bytes_per_column, make_chunk
+    from uwshcu_amd.generator import add_zw_inputs, algorithmic_bytes_per_column, make_chunk
 
     torch.cuda.set_device(0)
     prec = _capi.FP64 if args.precision == "fp64" else _capi.MIXED
+    zw = args.scheme == "zw"
+    if zw and (prec != _capi.FP64 or args.adjoint or args.chunks > 1 or args.tile > 1):
+        sys.exit("bench.py: --scheme zw times the FP64 forward alone")
     if args.ncol % args.tile:
         sys.exit("bench.py: --ncol must be a multiple of --tile")
     ptop = 100.0 if args.pver > 60 else 200.0
     ch = make_chunk(args.ncol // args.tile, args.pver, args.ncnst, seed=args.seed,
                     inversion_frac=args.inversion_frac, ptop=ptop)
+    if zw:
+        add_zw_inputs(ch, args.seed)
+        zw_par, zw_opts = _capi.zw_default_params()
     ws_gib = args.workspace_gib or (160.0 if args.adjoint else None)
     ctx = device.Context(0, workspace_limit=int(ws_gib * (1 << 30)) if ws_gib else None)
     inp = device.chunk_to_device(ch, "cuda:0")
@@ -135,7 +143,10 @@ def main():
 
     def fwd(precision):
         cush.copy_(inp["cush"])
-        ctx.forward_into(inp, out, cush, dt=ch.dt, precision=precision, stream=stream)
+        if zw:
+            ctx.zw_forward_into(inp, out, cush, zw_opts, dt=ch.dt, params=zw_par, stream=stream)
+        else:
+            ctx.forward_into(inp, out, cush, dt=ch.dt, precision=precision, stream=stream)
 
     extra = {}
     if args.adjoint:
@@ -194,6 +205,13 @@ def main():
                            "batched_ms": float(np.mean(per_step_ms)),
                            "speedup": float(np.mean(loop_ms)) / float(np.mean(per_step_ms))}
     else:
+        if zw:
+            # the UW scheme on the same chunk with the same rpen and rkm, for comparison
+            def uw_fwd():
+                cush.copy_(inp["cush"])
+                ctx.forward_into(inp, out, cush, dt=ch.dt, params=zw_par, stream=stream)
+            uw_ms, _ = timed(uw_fwd, args.steps, args.warmup, stream)
+            extra["uw_forward_ms_same_chunk_and_params"] = float(np.mean(uw_ms))
         per_step_ms, wall = timed(lambda: fwd(prec), args.steps, args.warmup, stream)
     rc, bad = ctx.status(stream)
     if rc:
@@ -225,6 +243,7 @@ def main():
         mode = "fp64 forward + adjoint"
     if args.chunks > 1:
         mode += f" over {args.chunks} chunks in one call"
+    scheme = "compute_zwshcu_inv (default options, rpen 15, rkm 10)" if zw else "compute_uwshcu_inv"
     res = {
         "metric": METRIC,
         "value": cols_per_s,
@@ -239,7 +258,7 @@ def main():
         "dtype": "f64" if (prec == _capi.FP64 or args.adjoint) else "f32+f64",
         "data": "synthetic" if args.tile == 1 else f"synthetic, {args.ncol // args.tile} distinct columns tiled {args.tile}x",
         "config": {
-            "workload": f"{mode} of compute_uwshcu_inv, {args.ncol} columns x {args.pver} levels x "
+            "workload": f"{mode} of {scheme}, {args.ncol} columns x {args.pver} levels x "
                         f"{args.ncnst} constituents, dt={ch.dt:g} s",
             "ncol": args.ncol, "pver": args.pver, "ncnst": args.ncnst,
             "convecting_fraction": conv_frac, "inversion_frac": args.inversion_frac, "seed": args.seed,
@@ -273,7 +292,8 @@ def measured_traffic(mode, args):
             ent = json.load(f).get(key)
     except OSError:
         return None
-    return ent["traffic_bytes"] if ent and "adjoint" not in mode and "chunks" not in mode else None
+    ok = ent and "adjoint" not in mode and "chunks" not in mode and args.scheme == "uw"
+    return ent["traffic_bytes"] if ok else None
 
 
 def mixed_error(mix, f64, nfix, ncol):
@@ -323,16 +343,24 @@ def cpu_leg(args, ch, out, cush):
     nthr = args.cpu_threads
     small = Chunk(min(256, n), ch.pver, ch.ncnst, ch.dt,
                   {k: np.ascontiguousarray(v[..., :min(256, n)]) for k, v in ch.fields.items()})
-    oracle.run(small, nthreads=nthr)                      # warm the library and the thread pool
+    if args.scheme == "zw":
+        zc = oracle.default_consts(rpen=15.0, rkm=10.0)
+
+        def orun(chunk, nthreads, portable=False):
+            return oracle.run_zw(chunk, oracle.zw_defaults(), zc, portable=portable, nthreads=nthreads)
+    else:
+        def orun(chunk, nthreads, portable=False):
+            return oracle.run(chunk, nthreads=nthreads, portable=portable)
+    orun(small, nthr)                                     # warm the library and the thread pool
     t0 = time.perf_counter()
-    ref, rdiag, rc, _ = oracle.run(sub, nthreads=nthr)    # host libm: the faithful reading of the reference
+    ref, rdiag, rc, _ = orun(sub, nthr)                   # host libm: the faithful reading of the reference
     dt_cpu = time.perf_counter() - t0
     n1 = min(2048, n)                                     # one thread, on a smaller prefix
     one = Chunk(n1, ch.pver, ch.ncnst, ch.dt, {k: np.ascontiguousarray(v[..., :n1]) for k, v in ch.fields.items()})
     t1 = time.perf_counter()
-    oracle.run(one, nthreads=1)
+    orun(one, 1)
     dt_one = time.perf_counter() - t1
-    pm, pmdiag, _, _ = oracle.run(sub, nthreads=nthr, portable=True)
+    pm, pmdiag, _, _ = orun(sub, nthr, portable=True)
     got = {k: v.cpu().numpy()[..., :n] for k, v in out.items() if k != "diag"}
     got["cush"] = cush.cpu().numpy()[:n]
     diag = out["diag"].cpu().numpy()[:, :n].T
@@ -344,7 +372,7 @@ def cpu_leg(args, ch, out, cush):
     return {
         "cpu_baseline": {"value": n / dt_cpu, "unit": "columns/s", "cores": nthr, "kind": "port",
                          "sample": f"first {n} columns of the same chunk, FP64 C oracle (host libm), "
-                                   f"OpenMP over columns, {nthr} threads",
+                                   f"{nthr} threads over columns",
                          "single_thread_value": n1 / dt_one, "single_thread_sample": f"first {n1} columns, 1 thread",
                          "host_cpu": host_cpu()},
         "error_vs_oracle": {

@@ -201,21 +201,30 @@ struct Column {
     }
     UWSHCU_HD inline D in(const double* p, int k) const {
         UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
-        if constexpr (ZW) {
-            // compute_zwshcu_inv's PBL perturbation (zwshcu.F90:547-566): tpert / qpert,
-            // capped, added to t0 / qv0 in every layer whose bottom interface is not
-            // above pblh (s0 gets cp*tpert there too; the column physics never reads s0)
-            if (a.zw.pertt && (p == a.t0 || p == a.qv0) && !(a.zs0[row(k - 1)] > a.pblh[i])) {
-                if (p == a.t0) return p[row(k)] + fmin(a.tpert[i], a.zw.tpertmax);
-                return p[row(k)] + fmin(a.qpert[i], a.zw.qpertmax);
-            }
-        }
         if constexpr (kAD) {
             const int base = p == a.t0 ? ADI_T0 : p == a.qv0 ? ADI_QV0 : p == a.u0 ? ADI_U0 : p == a.v0 ? ADI_V0
                            : p == a.tke ? ADI_TKE : -1;
             if (base >= 0) return D(p[row(k)], base * KS + k);
         }
         return D(p[row(k)]);
+    }
+    // t0 and qv0 of layer k as the column physics sees them.  The modified scheme's
+    // wrapper (zwshcu.F90:547-566) adds tpert / qpert, capped, to t0 / qv0 in every
+    // layer whose bottom interface is not above pblh (and cp*tpert to s0, which the
+    // column physics never reads).
+    UWSHCU_HD inline D in_t0(int k) const {
+        const D v = in(a.t0, k);
+        if constexpr (ZW) {
+            if (a.zw.pertt && !(a.zs0[row(k - 1)] > a.pblh[i])) return v + fmin(a.tpert[i], a.zw.tpertmax);
+        }
+        return v;
+    }
+    UWSHCU_HD inline D in_qv0(int k) const {
+        const D v = in(a.qv0, k);
+        if constexpr (ZW) {
+            if (a.zw.pertt && !(a.zs0[row(k - 1)] > a.pblh[i])) return v + fmin(a.qpert[i], a.zw.qpertmax);
+        }
+        return v;
     }
     UWSHCU_HD inline D& out(double* p, int k) const {
         UW_CHECK(k <= mkx && k >= (iface(p) ? 0 : 1) && i >= 0 && i < a.ncol);
@@ -397,12 +406,12 @@ struct Column {
         E(E_THV0MID, k) = 0.0;
         if (k < 1) return;
         D qsp, enerr;
-        const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
+        const int doit = findsp_point(T, in_qv0(k), in_t0(k), in(a.p0, k), qsp, enerr);
         if (doit == 0) record_error(a.err, UWSHCU_ERR_FINDSP_CONV, i);
         else if (doit == 2 && enerr > 1.e-4) record_error(a.err, UWSHCU_ERR_FINDSP_ENTH, i);
         W(W_QW0, k) = R(qsp);
         E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
-        const D qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
+        const D qv0 = in_qv0(k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in_t0(k);
         const D qt0 = (qv0 + ql0 + qi0);
         E(A_QT0, k) = qt0;
         E(A_THL0, k) = (t0 - C.xlv * ql0 / C.cp - C.xls * qi0 / C.cp) / E(A_EXN0, k);
@@ -490,7 +499,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
     if (!a.env_ready)
     for (int k = 1; k <= mkx; ++k) {
         D qsp, enerr;
-        const int doit = findsp_point(T, in(a.qv0, k), in(a.t0, k), in(a.p0, k), qsp, enerr);
+        const int doit = findsp_point(T, in_qv0(k), in_t0(k), in(a.p0, k), qsp, enerr);
         if (doit == 0) record_error(a.err, UWSHCU_ERR_FINDSP_CONV, i);
         else if (doit == 2 && enerr > 1.e-4) record_error(a.err, UWSHCU_ERR_FINDSP_ENTH, i);
         W(W_QW0, k) = R(qsp);
@@ -501,7 +510,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
     for (int k = 1; k <= mkx; ++k) E(A_EXN0, k) = upow(in(a.p0, k) / C.p00, C.rovcp);
     for (int k = 0; k <= mkx; ++k) E(A_EXNS0, k) = upow(in(a.ps0, k) / C.p00, C.rovcp);
     for (int k = 1; k <= mkx; ++k) {
-        const D qv0 = in(a.qv0, k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in(a.t0, k);
+        const D qv0 = in_qv0(k), ql0 = in(a.ql0, k), qi0 = in(a.qi0, k), t0 = in_t0(k);
         const D qt0 = (qv0 + ql0 + qi0);
         E(A_QT0, k) = qt0;
         E(A_THL0, k) = (t0 - C.xlv * ql0 / C.cp - C.xls * qi0 / C.cp) / E(A_EXN0, k);
@@ -1413,10 +1422,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
             if (final_pass) { out(a.flxprc1, mkx) = 0.0; out(a.flxsnow1, mkx) = 0.0; }
             for (int k = mkx; k >= 1; --k) {
                 const R dp0k = R(in(a.dp0, k));
-                const R t0k = R(in(a.t0, k)), qv0k = R(in(a.qv0, k));
+                const R t0k = R(in_t0(k)), qv0k = R(in_qv0(k));
                 R snowmlt = (t0k > R(273.16)) ? fmax(R(0), flxsnow_k * g / dp0k) : R(0);
                 D es, qs;
-                T.qsat_es_qs(in(a.t0, k), in(a.p0, k), es, qs);
+                T.qsat_es_qs(in_t0(k), in(a.p0, k), es, qs);
                 const R subsat = fmax((R(1) - qv0k / R(qs)), R(0));
                 R evprain = kevp * subsat * sqrt(flxrain_k + snowmlt * dp0k / g);
                 R evpsnow = kevp * subsat * sqrt(fmax(flxsnow_k - snowmlt * dp0k / g, R(0)));
@@ -1477,7 +1486,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
             bool have_carry = false;
             for (int k = mkx; k >= 1; --k) {
                 D qvten = out(a.qvten, k), qlten = out(a.qlten, k), qiten = out(a.qiten, k), sten = out(a.sten, k);
-                D qv = in(a.qv0, k) + qvten * dtd;
+                D qv = in_qv0(k) + qvten * dtd;
                 D ql = in(a.ql0, k) + qlten * dtd;
                 D qi = in(a.qi0, k) + qiten * dtd;
                 if (have_carry) {
@@ -1609,10 +1618,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         if (iter == 1) {
             cush_s = cush;
             for (int k = 1; k <= mkx; ++k) {
-                const D qv0s = in(a.qv0, k) + out(a.qvten, k) * dtd;
+                const D qv0s = in_qv0(k) + out(a.qvten, k) * dtd;
                 const D ql0s = in(a.ql0, k) + out(a.qlten, k) * dtd;
                 const D qi0s = in(a.qi0, k) + out(a.qiten, k) * dtd;
-                const D t0s = in(a.t0, k) + out(a.sten, k) * dtd / C.cp;
+                const D t0s = in_t0(k) + out(a.sten, k) * dtd / C.cp;
                 const D qt0 = (qv0s + ql0s + qi0s);
                 E(B_QT0, k) = qt0;
                 E(B_THL0, k) = (t0s - C.xlv * ql0s / C.cp - C.xls * qi0s / C.cp) / E(A_EXN0, k);
