@@ -874,21 +874,25 @@ int uwshcu_status(uwshcu_ctx* ctx, void* stream, int* bad_col) {
     return h[0];
 }
 
-int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver,
-                        int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
-                        const double* z0, const double* dp0, const double* u0, const double* v0, const double* qv0,
-                        const double* ql0, const double* qi0, const double* t0, const double* s0,
-                        const double* tr0, const double* tke, const double* cldfrc, const double* concldfrc,
-                        const double* pblh, double* cush, double* umf, double* slflx, double* qtflx,
-                        double* flxprc1, double* flxsnow1, double* qvten, double* qlten, double* qiten,
-                        double* sten, double* uten, double* vten, double* trten, double* qrten, double* qsten,
-                        double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
-                        double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
-                        const double* dpdry0, double* wu, int* diag, int* bad_col) {
-    (void)cldfrc;
-    (void)concldfrc;
+// Host-array forward: stage the inputs to the device, run, copy the outputs back.
+// zwp != nullptr selects the modified scheme, whose host inputs omega_h (pcols, pver)
+// and tpert_h, qpert_h (pcols) are staged too.
+static int forward_host_impl(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol,
+                             int pver, int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
+                             const double* z0, const double* dp0, const double* u0, const double* v0,
+                             const double* qv0, const double* ql0, const double* qi0, const double* t0,
+                             const double* s0, const double* tr0, const double* tke, const double* pblh,
+                             double* cush, double* umf, double* slflx, double* qtflx, double* flxprc1,
+                             double* flxsnow1, double* qvten, double* qlten, double* qiten, double* sten,
+                             double* uten, double* vten, double* trten, double* qrten, double* qsten,
+                             double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
+                             double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
+                             const double* dpdry0, double* wu, int* diag, int* bad_col,
+                             const uwshcu_zw_params* zwp, const double* omega_h, const double* tpert_h,
+                             const double* qpert_h) {
     if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
         return UWSHCU_ERR_ARG;
+    if (zwp && (!omega_h || !tpert_h || !qpert_h)) return UWSHCU_ERR_ARG;
     if (bad_col) *bad_col = -1;
     if (ncol == 0) return UWSHCU_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -896,7 +900,7 @@ int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par
     const size_t L = (size_t)pcols * pver, I = (size_t)pcols * (pver + 1), S = (size_t)pcols;
     const size_t T = (size_t)pcols * pver * ncnst;
     // staging layout: inputs then outputs, each a full (pcols, ...) array
-    const size_t n_in = 3 * I + 11 * L + T + 2 * S;
+    const size_t n_in = 3 * I + 11 * L + T + 2 * S + (zwp ? L + 2 * S : 0);
     const size_t n_out = 6 * I + 14 * L + T + 6 * S;
     const size_t bytes = (n_in + n_out) * sizeof(double);
     if (bytes > ctx->stage_bytes) {
@@ -936,6 +940,12 @@ int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par
     ch.tr0 = h2d(tr0, T);
     ch.pblh = h2d(pblh, S);
     ch.cush = const_cast<double*>(h2d(cush, S));
+    ZwInputs zin{zwp, nullptr, nullptr, nullptr};
+    if (zwp) {
+        zin.omega = h2d(omega_h, L);
+        zin.tpert = h2d(tpert_h, S);
+        zin.qpert = h2d(qpert_h, S);
+    }
     ch.umf = take(I); ch.slflx = take(I); ch.qtflx = take(I); ch.flxprc1 = take(I); ch.flxsnow1 = take(I);
     ch.wu = take(I);
     ch.qvten = take(L); ch.qlten = take(L); ch.qiten = take(L); ch.sten = take(L); ch.uten = take(L);
@@ -944,29 +954,79 @@ int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par
     ch.trten = take(T);
     ch.precip = take(S); ch.snow = take(S); ch.cbmf = take(S); ch.rliq = take(S); ch.cnt = take(S); ch.cnb = take(S);
     if (!ok) return UWSHCU_ERR_HIP;
-    int rc = run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, ddiag, st);
+    int rc = run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, ddiag, st, zwp ? &zin : nullptr);
     if (rc) return rc;
-    auto d2h = [&](double* h, const double* d, size_t n) {
-        if (n) ok = ok && hip_ok(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    // outputs: the first ncol columns of each row only (the caller's padding columns
+    // ncol..pcols-1 stay as they were, as in the reference, which writes 1:iend)
+    const size_t rows_i = (size_t)pver + 1, rows_l = (size_t)pver, rows_t = (size_t)pver * ncnst;
+    auto d2h = [&](void* h, const void* d, size_t rows, size_t elem) {
+        if (rows)
+            ok = ok && hip_ok(hipMemcpy2DAsync(h, (size_t)pcols * elem, d, (size_t)pcols * elem, (size_t)ncol * elem,
+                                               rows, hipMemcpyDeviceToHost, st));
     };
-    d2h(cush, ch.cush, S);
-    d2h(umf, ch.umf, I); d2h(slflx, ch.slflx, I); d2h(qtflx, ch.qtflx, I); d2h(flxprc1, ch.flxprc1, I);
-    d2h(flxsnow1, ch.flxsnow1, I); d2h(wu, ch.wu, I);
-    d2h(qvten, ch.qvten, L); d2h(qlten, ch.qlten, L); d2h(qiten, ch.qiten, L); d2h(sten, ch.sten, L);
-    d2h(uten, ch.uten, L); d2h(vten, ch.vten, L); d2h(qrten, ch.qrten, L); d2h(qsten, ch.qsten, L);
-    d2h(evapc, ch.evapc, L); d2h(cufrc, ch.cufrc, L); d2h(qcu, ch.qcu, L); d2h(qlu, ch.qlu, L);
-    d2h(qiu, ch.qiu, L); d2h(qc, ch.qc, L);
-    d2h(trten, ch.trten, T);
-    d2h(precip, ch.precip, S); d2h(snow, ch.snow, S); d2h(cbmf, ch.cbmf, S); d2h(rliq, ch.rliq, S);
-    d2h(cnt, ch.cnt, S); d2h(cnb, ch.cnb, S);
-    if (diag && ok)
-        ok = hip_ok(hipMemcpyAsync(diag, ddiag, (size_t)UWSHCU_NDIAG * pcols * sizeof(int), hipMemcpyDeviceToHost, st));
+    const size_t e8 = sizeof(double);
+    d2h(cush, ch.cush, 1, e8);
+    d2h(umf, ch.umf, rows_i, e8); d2h(slflx, ch.slflx, rows_i, e8); d2h(qtflx, ch.qtflx, rows_i, e8);
+    d2h(flxprc1, ch.flxprc1, rows_i, e8); d2h(flxsnow1, ch.flxsnow1, rows_i, e8); d2h(wu, ch.wu, rows_i, e8);
+    d2h(qvten, ch.qvten, rows_l, e8); d2h(qlten, ch.qlten, rows_l, e8); d2h(qiten, ch.qiten, rows_l, e8);
+    d2h(sten, ch.sten, rows_l, e8); d2h(uten, ch.uten, rows_l, e8); d2h(vten, ch.vten, rows_l, e8);
+    d2h(qrten, ch.qrten, rows_l, e8); d2h(qsten, ch.qsten, rows_l, e8); d2h(evapc, ch.evapc, rows_l, e8);
+    d2h(cufrc, ch.cufrc, rows_l, e8); d2h(qcu, ch.qcu, rows_l, e8); d2h(qlu, ch.qlu, rows_l, e8);
+    d2h(qiu, ch.qiu, rows_l, e8); d2h(qc, ch.qc, rows_l, e8);
+    d2h(trten, ch.trten, rows_t, e8);
+    d2h(precip, ch.precip, 1, e8); d2h(snow, ch.snow, 1, e8); d2h(cbmf, ch.cbmf, 1, e8); d2h(rliq, ch.rliq, 1, e8);
+    d2h(cnt, ch.cnt, 1, e8); d2h(cnb, ch.cnb, 1, e8);
+    if (diag) d2h(diag, ddiag, UWSHCU_NDIAG, sizeof(int));
     int h[2] = {0, -1};
     ok = ok && hip_ok(hipMemcpyAsync(h, ctx->d_err, sizeof(h), hipMemcpyDeviceToHost, st)) &&
          hip_ok(hipStreamSynchronize(st)) && hip_ok(uwshcu::launch_clear_status(ctx->d_err, st));
     if (!ok) return UWSHCU_ERR_HIP;
     if (bad_col) *bad_col = h[0] ? h[1] : -1;
     return h[0];
+}
+
+int uwshcu_forward_host(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver,
+                        int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
+                        const double* z0, const double* dp0, const double* u0, const double* v0, const double* qv0,
+                        const double* ql0, const double* qi0, const double* t0, const double* s0,
+                        const double* tr0, const double* tke, const double* cldfrc, const double* concldfrc,
+                        const double* pblh, double* cush, double* umf, double* slflx, double* qtflx,
+                        double* flxprc1, double* flxsnow1, double* qvten, double* qlten, double* qiten,
+                        double* sten, double* uten, double* vten, double* trten, double* qrten, double* qsten,
+                        double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
+                        double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
+                        const double* dpdry0, double* wu, int* diag, int* bad_col) {
+    (void)cldfrc;
+    (void)concldfrc;
+    return forward_host_impl(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ps0, zs0, p0, z0, dp0, u0, v0, qv0,
+                             ql0, qi0, t0, s0, tr0, tke, pblh, cush, umf, slflx, qtflx, flxprc1, flxsnow1, qvten,
+                             qlten, qiten, sten, uten, vten, trten, qrten, qsten, precip, snow, evapc, cufrc, qcu,
+                             qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu, diag, bad_col, nullptr, nullptr,
+                             nullptr, nullptr);
+}
+
+int uwshcu_zw_forward_host(uwshcu_ctx* ctx, const uwshcu_params* par, const uwshcu_zw_params* zw, int pcols,
+                           int ncol, int pver, int ncnst, double dt, const double* ps0, const double* zs0,
+                           const double* p0, const double* z0, const double* dp0, const double* u0, const double* v0,
+                           const double* qv0, const double* ql0, const double* qi0, const double* t0,
+                           const double* s0, const double* tr0, const double* tke, const double* cldfrc,
+                           const double* concldfrc, const double* pblh, double* cush, const double* omega,
+                           const double* tpert, const double* qpert, double* umf, double* slflx, double* qtflx,
+                           double* flxprc1, double* flxsnow1, double* qvten, double* qlten, double* qiten,
+                           double* sten, double* uten, double* vten, double* trten, double* qrten, double* qsten,
+                           double* precip, double* snow, double* evapc, double* cufrc, double* qcu, double* qlu,
+                           double* qiu, double* cbmf, double* qc, double* rliq, double* cnt, double* cnb,
+                           const double* dpdry0, double* wu, int* diag, int* bad_col) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!zw) return UWSHCU_ERR_ARG;
+    uwshcu_params pz;
+    uwshcu_zw_default_params(&pz, nullptr);
+    return forward_host_impl(ctx, UWSHCU_FP64, par ? par : &pz, pcols, ncol, pver, ncnst, dt, ps0, zs0, p0, z0, dp0,
+                             u0, v0, qv0, ql0, qi0, t0, s0, tr0, tke, pblh, cush, umf, slflx, qtflx, flxprc1,
+                             flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten, qrten, qsten, precip, snow,
+                             evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu, diag, bad_col, zw,
+                             omega, tpert, qpert);
 }
 
 void bxcompute_uwshcu_inv_(int* pcols, int* pver, int* ncol, int* pcnst, double* ztodt, double* ps0_inv,

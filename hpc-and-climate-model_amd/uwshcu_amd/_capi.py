@@ -109,7 +109,7 @@ SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
            "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity",
            "uwshcu_forward_chunks_device", "uwshcu_merge_shallow_device", "uwshcu_zw_default_params",
-           "uwshcu_zw_forward_device")
+           "uwshcu_zw_forward_device", "uwshcu_zw_forward_host")
 
 _lib = None
 
@@ -149,6 +149,8 @@ def lib():
     L.uwshcu_zw_default_params.argtypes = [C.POINTER(Params), C.POINTER(ZwParams)]
     L.uwshcu_zw_forward_device.argtypes = [vp, C.POINTER(Params), C.POINTER(ZwParams), C.c_int, C.c_int, C.c_int,
                                            C.c_int, dp] + [vp] * len(ZW_ARRAY_ARGS) + [vp, vp]
+    L.uwshcu_zw_forward_host.argtypes = [vp, C.POINTER(Params), C.POINTER(ZwParams), C.c_int, C.c_int, C.c_int,
+                                         C.c_int, dp] + [vp] * len(ZW_ARRAY_ARGS) + [vp, ip]
     L.uwshcu_forward_host.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, ip]
     L.uwshcu_status.argtypes = [vp, vp, ip]
     L.uwshcu_set_dropin_params.argtypes = [C.POINTER(Params)]

@@ -123,6 +123,24 @@ int uwshcu_zw_forward_device(uwshcu_ctx *ctx, const uwshcu_params *par, const uw
     double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
 
+/* uwshcu_zw_forward_device on host arrays, like uwshcu_forward_host: stages the
+ * inputs (with omega, tpert, qpert) through context-owned device memory, runs, and
+ * returns after the outputs are back; returns the fatal-condition status and its
+ * column.  This is the entry point a Fortran driver binds in place of its call of
+ * compute_zwshcu_inv (convect_shallow.F90:717-733). */
+int uwshcu_zw_forward_host(uwshcu_ctx *ctx, const uwshcu_params *par, const uwshcu_zw_params *zw,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    const double *omega, const double *tpert, const double *qpert,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, int *bad_col);
+
 /* Forward physics over many chunks in one launch.  The reference calls the scheme
  * once per physics chunk (physpkg.F90:782-803 -> convect_shallow.F90:605-619, a few
  * dozen to a thousand columns each); this entry point takes a timestep's chunks

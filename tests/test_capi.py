@@ -91,6 +91,9 @@ def test_device_entry_points_reject_bad_arguments_without_a_gpu():
     rc = lib.uwshcu_zw_forward_device(None, C.byref(par), C.byref(z), 10, 10, 30, 3, C.c_double(1800.0), *nulz,
                                       None, None)
     assert rc == -10
+    rc = lib.uwshcu_zw_forward_host(None, C.byref(par), C.byref(z), 10, 10, 30, 3, C.c_double(1800.0), *nulz,
+                                    None, None)
+    assert rc == -10
     desc = (_capi.ChunkDesc * 2)()
     rc = lib.uwshcu_forward_chunks_device(None, 0, C.byref(par), 2, desc, 30, 3, C.c_double(1800.0), None)
     assert rc == -10

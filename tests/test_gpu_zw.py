@@ -110,3 +110,42 @@ def test_zw_zero_columns_and_a_ragged_chunk():
     np.testing.assert_array_equal(cush.cpu().numpy()[:77], ref["cush"])
     np.testing.assert_array_equal(got["diag"][:77], rdiag)
     torch.cuda.synchronize()
+
+
+def test_zw_host_array_entry_point():
+    """uwshcu_zw_forward_host: the entry point a Fortran driver binds in place of
+    compute_zwshcu_inv, on host arrays with pcols > ncol."""
+    import ctypes as C
+    from uwshcu_amd import _capi, device
+    full = add_zw_inputs(make_chunk(300, 30, 3, seed=27, inversion_frac=1.0, inversion_dt=1.0), 27)
+    ncol = 257
+    opts = dict(ent=1, pertt=1, pertw=1, cloud_sh=1)
+    small = full.copy()
+    small.ncol = ncol
+    small.fields = {k: np.ascontiguousarray(v[..., :ncol]) for k, v in full.fields.items()}
+    ref, rdiag, rc, _ = oracle.run_zw(small, oracle.zw_defaults(**opts), oracle.default_consts(rpen=15.0, rkm=10.0),
+                                      portable=True)
+    assert rc == 0
+    f = full.fields
+    n, Lv, nc = full.ncol, full.pver, full.ncnst
+    outs = {k: np.full((Lv + 1, n), -7.0) for k in ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "wu")}
+    outs.update({k: np.full((Lv, n), -7.0) for k in ("qvten", "qlten", "qiten", "sten", "uten", "vten", "qrten",
+                                                      "qsten", "evapc", "cufrc", "qcu", "qlu", "qiu", "qc")})
+    outs["trten"] = np.full((nc, Lv, n), -7.0)
+    outs.update({k: np.full(n, -7.0) for k in ("precip", "snow", "cbmf", "rliq", "cnt", "cnb")})
+    cush = f["cush"].copy()
+    diag = np.zeros((_capi.NDIAG, n), dtype=np.int32)
+    pd = lambda a: a.ctypes.data_as(C.c_void_p)
+    par, z = _capi.zw_default_params(**opts)
+    ctx = device.default_context()
+    arrays = {**f, **outs, "cush": cush}
+    args = [ctx._h, C.byref(par), C.byref(z), n, ncol, Lv, nc, C.c_double(full.dt)]
+    args += [pd(np.ascontiguousarray(arrays[k])) if k in f else pd(arrays[k]) for k in _capi.ZW_ARRAY_ARGS]
+    bad = C.c_int(0)
+    rc = _capi.lib().uwshcu_zw_forward_host(*args, pd(diag), C.byref(bad))
+    assert rc == 0 and bad.value == -1
+    outs["cush"] = cush
+    for k in ref:
+        np.testing.assert_array_equal(outs[k][..., :ncol], ref[k], err_msg=k)
+        assert np.all(outs[k][..., ncol:] == (f["cush"][ncol:] if k == "cush" else -7.0)), k
+    np.testing.assert_array_equal(np.ascontiguousarray(diag[:, :ncol].T), rdiag)
