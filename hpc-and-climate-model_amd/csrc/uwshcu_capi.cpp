@@ -191,7 +191,8 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
             ctx->d_defer_state = nullptr;
             ctx->defer_cols = 0;
         }
-        if (!hip_ok(hipMalloc((void**)&ctx->d_defer, ((size_t)nb + 1) * sizeof(int)))) return UWSHCU_ERR_NOMEM;
+        if (!hip_ok(hipMalloc((void**)&ctx->d_defer, (3 * (size_t)nb + (size_t)nb / 256 + 8) * sizeof(int))))
+            return UWSHCU_ERR_NOMEM;
         if (!hip_ok(hipMalloc((void**)&ctx->d_defer_state, (size_t)uwshcu::NRESUME * ((size_t)nb + 1) * sizeof(double)))) {
             (void)hipFree(ctx->d_defer);
             ctx->d_defer = nullptr;
@@ -226,7 +227,7 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
         a.qpert = zw->qpert;
     }
     if (ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
-        a.defer_list = ctx->d_defer + 1;
+        a.defer_list = ctx->d_defer + 2;   // two counts, then the lists and marks (see launch_column_pipeline)
         a.defer_count = ctx->d_defer;
         a.defer_state = ctx->d_defer_state;   // [field][slot], a.wld = nb apart
     }

@@ -96,8 +96,14 @@ struct FwdArgs {
     int* defer_list;
     int* defer_count;
     double* defer_state;
-    // 1: continue each listed column (colmap) from that point, in the workspace slot it
-    // had in the deferring launch (column - slot0), whose set-A environment is intact
+    // 1 (with defer_list): a column that reaches the cloud-base closure of its first
+    // pass stops there instead, saves its scalars and lists itself, so that only the
+    // triggered columns run the convection stages, packed densely in a launch of their own
+    int defer_at_closure;
+    int* defer_mark;         // optional [wld]: mark the deferred column's slot 1 instead of listing it
+    // 1: continue each listed column (colmap) at the closure of its first pass; 2: at its
+    // second convection pass.  Each runs in the workspace slot it had in the deferring
+    // launch (column - slot0), whose set-A environment is intact.
     int resume;
     int slot0;
     AdArgs ad;               // adjoint kernel only
@@ -136,7 +142,7 @@ UWSHCU_HD inline int ad_inputs(int mkx) { return ADI_PARAMS * (mkx + 1) + 3; }
 
 // Exit reasons reported in diag row 0 (same numbering as the oracle).
 // Scalars of a column deferred at its second convection pass (FwdArgs::defer_state).
-enum { NRESUME = 40 };
+enum { NRESUME = 56 };
 
 enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
@@ -523,8 +529,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
     }
 
-    zero_pass_state();
-    for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
+    if (!a.resume) {   // a continued column's arrays hold what its earlier launch wrote
+        zero_pass_state();
+        for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
+    }
 
     D cush = D(a.cush_in ? a.cush_in[i] : a.cush[i]);
     const D pblh = D(a.pblh[i]);
@@ -553,6 +561,18 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         io(20, qtu_top); io(21, rho0j); io(22, drage); io(23, bogbot); io(24, bogtop); io(25, wtwb); io(26, th);
         io(27, qvj); io(28, qlj); io(29, qij); io(30, kinv); io(31, klcl); io(32, klfc); io(33, krel);
         io(34, kbup); io(35, kpen); io(36, limit_ufrc); io(37, limit_ppen); io(38, limit_emf); io(39, amb);
+        // the first pass's values that its implicit-CIN restore reads
+        io(40, kinv_o); io(41, klcl_o); io(42, klfc_o); io(43, plcl_o); io(44, plfc_o); io(45, tkeavg_o);
+        io(46, thvlmin_o); io(47, qtsrc_o); io(48, thvlsrc_o); io(49, thlsrc_o); io(50, usrc_o); io(51, vsrc_o);
+        io(52, thv0lcl_o); io(53, cin_i); io(54, cinlcl_i); io(55, ke);
+    };
+    auto defer_here = [&](bool mark) {   // save the scalars and list (or mark) the column
+        resume_io([&](int f, auto& x) {
+            if constexpr (std::is_integral<std::remove_reference_t<decltype(x)>>::value) rstate(f) = (double)x;
+            else rstate(f) = (double)val(x);
+        });
+        if (mark) a.defer_mark[iw] = 1;
+        else a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
     };
     if (a.resume) {
         resume_io([&](int f, auto& x) { x = decltype(+x)(rstate(f)); });
@@ -566,7 +586,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         goto L333;
     }
 
-    for (int iter = a.resume ? 2 : 1; iter <= 2; ++iter) {
+    for (int iter = a.resume ? a.resume : 1; iter <= 2; ++iter) {
         exit_iter = iter;
         const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
@@ -575,9 +595,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         const D tscaleh = cush;
         cush = -1.0;
 
-        if (a.resume) {
-            // a deferred column continues here with the scalars of its implicit-CIN restore
-            zero_pass_state();
+        if (a.resume == iter) {
+            // a deferred column continues with the scalars it saved: at the second pass
+            // those of its implicit-CIN restore, whose convection starts afresh
+            if (iter == 2) zero_pass_state();
         } else {
         // ---- S1 kinv (uwshcu.F90:1455-1468) ----
         kinv = 1;
@@ -690,14 +711,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                 kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
                 thvlmin = thvlmin_o; qtsrc = qtsrc_o; thvlsrc = thvlsrc_o; thlsrc = thlsrc_o; usrc = usrc_o;
                 vsrc = vsrc_o; thv0lcl = thv0lcl_o;
-                if (a.defer_list) {
+                if ((a.defer_list || a.defer_mark) && !a.defer_at_closure) {
                     // the second pass runs in a launch of its own: save, list, stop
                     cush = tscaleh;
-                    resume_io([&](int f, auto& x) {
-                        if constexpr (std::is_integral<std::remove_reference_t<decltype(x)>>::value) rstate(f) = (double)x;
-                        else rstate(f) = (double)val(x);
-                    });
-                    a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
+                    defer_here(a.defer_mark != nullptr);
                     return;
                 }
                 zero_pass_state();
@@ -713,6 +730,9 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
 
         // From here on the environment is set A (the input state).
         {
+        R wtw = R(0);
+        // a column continued at its first pass (resume 1) resumes at the buoyancy sorting
+        if (!(a.resume == 1 && iter == 1)) {
         // ---- S8 release level (uwshcu.F90:2035-2043) ----
         if (klcl < kinv) {
             krel = kinv;
@@ -725,7 +745,6 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         }
 
         // ---- S9 cloud-base closure (uwshcu.F90:2092-2184; zwshcu.F90:2275-2384), FP64 ----
-        R wtw;
         {
             D wcrit = sqrt(2.0 * cin * 1.0);
             D sigmaw = sqrt(1.0 * tkeavg + 5.e-4);
@@ -811,6 +830,15 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
             W(W_VU, krel - 1) = R(vsrc) + vplus;
         }
         for (int m = 1; m <= ncnst; ++m) WT(1, m, krel - 1) = R(trin(m, 1));   // trsrc = tr0(1,m)
+
+        if (iter == 1 && a.defer_at_closure && (a.defer_mark || a.defer_list)) {
+            // it convects: the plume stages run in a launch of their own, packed with the
+            // other columns that got this far
+            cush = tscaleh;
+            defer_here(a.defer_mark != nullptr);
+            return;
+        }
+        }   // S8-S10
 
         // ---- S11 buoyancy sorting (uwshcu.F90:2296-2952) ----
         R scaleh = R(tscaleh);

@@ -72,22 +72,88 @@ __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const u
 }
 
 static __global__ void clear_count_kernel(int* count) {
-    if (threadIdx.x == 0) *count = 0;
+    if (threadIdx.x < 2) count[threadIdx.x] = 0;
+}
+
+static __global__ __launch_bounds__(256) void fill_zero_kernel(int* p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+// Ordered compaction of marks[0, n) into list (column col0 + slot, ascending) and
+// *count, in three steps: per-256 block sums, their exclusive scan (one workgroup),
+// then each block's entries at its offset.  Keeping the column order keeps the
+// listed columns' workspace accesses as coalesced as the marks allow.
+static __device__ inline int block_prefix_256(int flag, int* s_wave, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(flag);
+    const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wave] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave) before += s_wave[w];
+        total += s_wave[w];
+    }
+    return before + in_wave;
+}
+static __global__ __launch_bounds__(256) void mark_block_sum_kernel(const int* marks, int n, int* bsum) {
+    __shared__ int s_wave[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    int total;
+    block_prefix_256(g < n ? marks[g] : 0, s_wave, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+static __global__ __launch_bounds__(1024) void block_sum_scan_kernel(int* bsum, int nb, int* count) {
+    __shared__ int s[1024];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int b = c0 + (int)threadIdx.x;
+        const int v = b < nb ? bsum[b] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan
+            const int add = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (b < nb) bsum[b] = carry + s[threadIdx.x] - v;   // exclusive
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = carry;
+}
+static __global__ __launch_bounds__(256) void mark_scatter_kernel(const int* marks, int n, int col0, const int* boff,
+                                                                  int* list) {
+    __shared__ int s_wave[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int flag = g < n ? marks[g] : 0;
+    int total;
+    const int pos = block_prefix_256(flag, s_wave, total);
+    if (flag) list[boff[blockIdx.x] + pos] = col0 + g;
 }
 
 // The forward pass of columns [col0, col0 + n) (or of the mapped columns): the two
 // level-parallel set-up stages, then the column kernel.
 //
-// Large launches (at least two wavefronts per SIMD) also split off the columns whose
-// implicit-CIN pass convects a second time (a few per cent of them): the column
-// kernel stops each of those where its second convection pass begins, saves its
-// scalars and lists it (defer_list, defer_state), and a second column kernel
-// continues the listed columns from there in their own workspace slots.  Otherwise
-// nearly every wavefront would carry one or two such columns through a second
-// convection pass while its other lanes idle.
+// Large launches (at least two wavefronts per SIMD) run the column pass in three
+// launches (two where a column's working set is large, without step 1's list), so
+// that wavefronts are not held by a few of their lanes:
+//  1. every column runs the environment, CIN and trigger stages; a column that
+//     reaches the cloud-base closure saves its scalars and lists itself (list 1);
+//  2. the listed (triggered) columns, packed densely, run the convection stages of
+//     the first pass; a column whose implicit-CIN pass convects a second time (a few
+//     per cent) saves its scalars there and lists itself again (list 2);
+//  3. the columns of list 2 run their second convection pass.
+// Each column keeps its workspace slot throughout.  Without the split, a wavefront
+// of a launch with 10 % convecting columns still runs the whole plume for the few
+// that convect, and nearly every wavefront carries one or two second passes.
 //
-// args0.defer_list / defer_count / defer_state (nullable) give the list's storage
-// (n entries, NRESUME x wld doubles).
+// args0.defer_list / defer_count / defer_state (nullable) give the lists' storage:
+// defer_count[0..1]; defer_list[0..3n + n/256 + 1): list 1, list 2, the marks of list 1
+// and the compaction's block sums; and NRESUME x wld doubles.
 template <class R, bool ZW = false>
 hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_consts* dconsts, const double* destbl,
                                   hipStream_t stream) {
@@ -105,10 +171,22 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
                        dconsts, destbl);
     const bool large = n >= 2 * 1024 * 64;
     const bool split = large && defer_list && defer_count && args0.defer_state && !args.colmap;
+    // The closure split pays where a column's working set is small: measured 1-20 %
+    // faster at 35 levels and 3 constituents (18 KB of workspace per column), 11 %
+    // slower at 72 levels and 10 constituents (54 KB), where the gaps in the packed
+    // list cost the second launch more than the idle lanes it saves.
+    const bool closure_split =
+        split && ws_bytes_per_column(args.mkx, args.ncnst, sizeof(R), sizeof(double)) <= (size_t)32 * 1024;
+    int* const marks = defer_list + 2 * (size_t)n;
+    int* const bsum = defer_list + 3 * (size_t)n;
+    const unsigned nblk = (unsigned)((n + 255) / 256);
     if (split) {
         hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, stream, defer_count);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(nblk), dim3(256), 0, stream, marks, n);
         args.defer_list = defer_list;
         args.defer_count = defer_count;
+        args.defer_mark = marks;
+        args.defer_at_closure = closure_split ? 1 : 0;
     }
     // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build
     if (large)
@@ -116,17 +194,47 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     else
         hipLaunchKernelGGL((forward_kernel<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     if (split) {
-        // continue the listed columns at their second convection pass
+        auto compact = [&](int* list, int* count) {   // marks -> list in column order
+            hipLaunchKernelGGL(mark_block_sum_kernel, dim3(nblk), dim3(256), 0, stream, (const int*)marks, n, bsum);
+            hipLaunchKernelGGL(block_sum_scan_kernel, dim3(1), dim3(1024), 0, stream, bsum, (int)nblk, count);
+            hipLaunchKernelGGL(mark_scatter_kernel, dim3(nblk), dim3(256), 0, stream, (const int*)marks, n,
+                               args0.col0, (const int*)bsum, list);
+        };
         FwdArgs b = args0;
-        b.env_ready = 1;
-        b.resume = 1;
-        b.slot0 = args0.col0;
-        b.defer_list = nullptr;
-        b.defer_count = nullptr;
-        b.colmap = defer_list;
-        b.colcount = defer_count;
-        b.col0 = 0;
-        hipLaunchKernelGGL((forward_kernel<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts, destbl);
+        if (closure_split) {
+            compact(defer_list, defer_count);
+            hipLaunchKernelGGL(fill_zero_kernel, dim3(nblk), dim3(256), 0, stream, marks, n);
+            // 2. the triggered columns from the closure of their first pass; up to all n
+            // of them, so the occupancy-2 build
+            b.env_ready = 1;
+            b.resume = 1;
+            b.slot0 = args0.col0;
+            b.defer_at_closure = 0;
+            b.defer_mark = marks;
+            b.defer_list = nullptr;
+            b.defer_count = nullptr;
+            b.colmap = defer_list;
+            b.colcount = defer_count;
+            b.col0 = 0;
+            hipLaunchKernelGGL((forward_kernel_occ2<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts,
+                               destbl);
+        } else {
+            // the first launch marked the columns that need a second convection pass
+            b.env_ready = 1;
+            b.slot0 = args0.col0;
+            b.col0 = 0;
+        }
+        compact(defer_list + n, defer_count + 1);
+        // 3. the second convection passes (a few per cent of the columns)
+        FwdArgs c = b;
+        c.resume = 2;
+        c.defer_mark = nullptr;
+        c.defer_list = nullptr;
+        c.defer_count = nullptr;
+        c.defer_at_closure = 0;
+        c.colmap = defer_list + n;
+        c.colcount = defer_count + 1;
+        hipLaunchKernelGGL((forward_kernel<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, c, n, dconsts, destbl);
     }
     return hipGetLastError();
 }

@@ -14,12 +14,15 @@ using namespace uwshcu;
 // 1: run the level-parallel set-up (setup_level1/2 over every column and level)
 // before the columns, as the GPU pipeline does; 0: the columns do it themselves.
 static int g_staged = 0;
-// 1: as the GPU pipeline does for large launches, stop columns that need a second
-// convection pass, list them and rerun them whole afterwards
+// 1: as the GPU pipeline does for large launches, stop the triggered columns at the
+// closure of their first pass and continue them from a list, then the same for the
+// columns that need a second convection pass
 static int g_defer = 0;
-static int g_deferred = 0;   // columns the last deferring run continued in its second launch
+static int g_deferred = 0;   // columns the last deferring run continued at their second pass
+static int g_triggered = 0;  // columns the last deferring run continued at their closure
 extern "C" void hostemu_set_defer(int on) { g_defer = on; }
 extern "C" int hostemu_last_deferred() { return g_deferred; }
+extern "C" int hostemu_last_triggered() { return g_triggered; }
 // backward-sweep window (entries) used by hostemu_adjoint; 0 = none
 static int g_window = 0;
 extern "C" void hostemu_set_window(int w) { g_window = w; }
@@ -45,28 +48,41 @@ static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
         for (int i = 0; i < ncol; ++i)
             for (int k = 1; k <= pver; ++k) Col(a, T, i, i).setup_level2(k);
     }
-    std::vector<int> deferred((size_t)ncol);
+    // the GPU pipeline's split of large launches: (1) every column up to the
+    // cloud-base closure of its first pass, listing the triggered ones; (2) those from
+    // there, listing the columns that need a second convection pass; (3) those
+    std::vector<int> listed((size_t)ncol), second((size_t)ncol);
     std::vector<double> state((size_t)NRESUME * (size_t)a.wld, -777.0);
-    int ndefer = 0;
+    int nlisted = 0, nsecond = 0;
     if (g_defer) {
-        a.defer_list = deferred.data();
-        a.defer_count = &ndefer;
+        a.defer_list = listed.data();
+        a.defer_count = &nlisted;
         a.defer_state = state.data();
+        a.defer_at_closure = 1;
     }
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ncol; ++i) Col(a, T, i, i).run();
     a.env_ready = 0;
     a.defer_list = nullptr;
     a.defer_count = nullptr;
+    a.defer_at_closure = 0;
     if (g_defer) {
         // continue the listed columns in their own slots, whose set-A environment the
-        // first pass (or the level-parallel set-up) left in place
+        // first launch (or the level-parallel set-up) left in place
         a.env_ready = 1;
-        a.resume = 1;
         a.slot0 = 0;
+        a.resume = 1;
+        a.defer_list = second.data();
+        a.defer_count = &nsecond;
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int j = 0; j < ndefer; ++j) Col(a, T, deferred[j], deferred[j]).run();
-        g_deferred = ndefer;
+        for (int j = 0; j < nlisted; ++j) Col(a, T, listed[j], listed[j]).run();
+        a.defer_list = nullptr;
+        a.defer_count = nullptr;
+        a.resume = 2;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < nsecond; ++j) Col(a, T, second[j], second[j]).run();
+        g_deferred = nsecond;
+        g_triggered = nlisted;
         a.resume = 0;
         a.env_ready = 0;
         a.defer_state = nullptr;

@@ -61,9 +61,9 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
     """Run the host-compiled column code on a generator chunk.
 
     ``staged`` runs the level-parallel set-up first, as the GPU pipeline does;
-    ``defer`` stops the columns that need a second convection pass and reruns them
-    afterwards from a list, as the GPU pipeline does for launches of 131072 columns
-    and more.  ``zw`` (a ``_capi.ZwParams``) runs the modified scheme instead, on a
+    ``defer`` splits the column pass as the GPU pipeline does for launches of 131072
+    columns and more: the triggered columns continue from the cloud-base closure in a
+    second pass over a list, and those needing a second convection pass in a third.  ``zw`` (a ``_capi.ZwParams``) runs the modified scheme instead, on a
     chunk that carries ``omega``, ``tpert`` and ``qpert``.
 
     Returns (outputs dict, diag [ncol, NDIAG], rc, bad_col) like ``oracle.run``.
@@ -110,6 +110,7 @@ def run(chunk, precision=_capi.FP64, params=None, consts=None, estbl=None, check
     if stats is not None:
         stats["recomputed"] = nfix.value
         stats["deferred"] = L.hostemu_last_deferred() if defer else 0
+        stats["triggered"] = L.hostemu_last_triggered() if defer else 0
     out["cush"] = cush
     return out, np.ascontiguousarray(diag.T), rc, bad.value
 

@@ -141,8 +141,16 @@ def test_72_levels_10_constituents():
 
 def test_launch_large_enough_for_the_two_wavefront_kernel():
     """From 131072 columns on, the column kernel is the build that lets two wavefronts
-    share a SIMD; its results must be the same bits."""
+    share a SIMD, and the column pass is split: the columns that reach the buoyancy
+    sorting continue, packed, in a second launch, and second convection passes in a
+    third.  Its results must be the same bits."""
     _check(make_chunk(163840, 35, 3, seed=9), "163840x35x3")
+
+
+def test_large_launch_at_72_levels():
+    """At 72 levels and 10 constituents a large launch splits off only the second
+    convection passes (the column's working set is too large for the first split to pay)."""
+    _check(make_chunk(131072, 72, 10, seed=10, ptop=100.0), "131072x72x10")
 
 
 @pytest.mark.parametrize("ncol,pver,ncnst", [(1, 30, 3), (100, 30, 0), (200, 12, 2), (64, 5, 1)])
