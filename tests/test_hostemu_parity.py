@@ -46,15 +46,17 @@ def test_level_parallel_setup_gives_identical_results(ncol, pver, ncnst, precisi
 @pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("ncol,pver,ncnst,inv", [(1024, 35, 3, 0.5), (256, 72, 10, 0.3)])
 def test_deferred_second_pass_columns_give_identical_results(ncol, pver, ncnst, inv, precision):
-    """Large GPU launches stop the columns whose implicit-CIN pass convects a second
-    time, save their scalars, list them and continue them in a second launch.  The
-    outputs and the integer diagnostics must equal the single-launch run bit for bit."""
+    """Large GPU launches stop the columns that reach the buoyancy sorting, save their
+    scalars and continue them from a list in a second launch, and do the same for the
+    columns whose implicit-CIN pass convects a second time in a third.  The outputs
+    and the integer diagnostics must equal the single-launch run bit for bit."""
     ch = make_chunk(ncol, pver, ncnst, seed=90 + pver, inversion_frac=inv, ptop=100.0 if pver > 60 else 200.0)
     a, ad, _, _ = hostemu.run(ch, precision=precision, staged=True)
     st = {}
     b, bd, _, _ = hostemu.run(ch, precision=precision, staged=True, defer=True, stats=st)
     assert (ad[:, 7] == 2).any(), "no column took the second pass"
     assert st["deferred"] > 0
+    assert (ad[:, 7] > 0).sum() <= st["triggered"] < ncol     # every convecting column was continued
     np.testing.assert_array_equal(ad, bd)
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
