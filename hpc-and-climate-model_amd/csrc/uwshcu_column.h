@@ -396,9 +396,13 @@ struct Column {
         cout(a.cnb) = D(mk + 1) - D(mk); // cnb = mkx flipped
     }
 
-    UWSHCU_HD void run() const { run_body(); }
+    UWSHCU_HD void run() const { run_body<true>(); }
     // the column pass itself, expanded into each forward kernel so that each gets its
-    // own register allocation (the two builds differ in their register budget)
+    // own register allocation (the two builds differ in their register budget).
+    // kSplit = false leaves out what only the first and second launches of a split
+    // large launch do (stop at, and continue from, the buoyancy sorting), which the
+    // one-wavefront-per-SIMD build never runs.
+    template <bool kSplit>
     UWSHCU_HD __attribute__((always_inline)) void run_body() const;
 
     // Level-parallel set-up (one thread per column and level), the part of S0 and of
@@ -472,6 +476,7 @@ struct Column {
 };
 
 template <class R, class D, bool ZW>
+template <bool kSplit>
 UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const {
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
@@ -529,7 +534,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
     }
 
-    if (!a.resume) {   // a continued column's arrays hold what its earlier launch wrote
+    if (!kSplit || !a.resume) {   // a column continued at its first pass keeps what launch 1 wrote
         zero_pass_state();
         for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
     }
@@ -561,10 +566,13 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         io(20, qtu_top); io(21, rho0j); io(22, drage); io(23, bogbot); io(24, bogtop); io(25, wtwb); io(26, th);
         io(27, qvj); io(28, qlj); io(29, qij); io(30, kinv); io(31, klcl); io(32, klfc); io(33, krel);
         io(34, kbup); io(35, kpen); io(36, limit_ufrc); io(37, limit_ppen); io(38, limit_emf); io(39, amb);
-        // the first pass's values that its implicit-CIN restore reads
-        io(40, kinv_o); io(41, klcl_o); io(42, klfc_o); io(43, plcl_o); io(44, plfc_o); io(45, tkeavg_o);
-        io(46, thvlmin_o); io(47, qtsrc_o); io(48, thvlsrc_o); io(49, thlsrc_o); io(50, usrc_o); io(51, vsrc_o);
-        io(52, thv0lcl_o); io(53, cin_i); io(54, cinlcl_i); io(55, ke);
+        // the first pass's values that its implicit-CIN restore reads (a column continued
+        // at its second pass has used them already)
+        if constexpr (kSplit) {
+            io(40, kinv_o); io(41, klcl_o); io(42, klfc_o); io(43, plcl_o); io(44, plfc_o); io(45, tkeavg_o);
+            io(46, thvlmin_o); io(47, qtsrc_o); io(48, thvlsrc_o); io(49, thlsrc_o); io(50, usrc_o);
+            io(51, vsrc_o); io(52, thv0lcl_o); io(53, cin_i); io(54, cinlcl_i); io(55, ke);
+        }
     };
     auto defer_here = [&](bool mark) {   // save the scalars and list (or mark) the column
         resume_io([&](int f, auto& x) {
@@ -586,7 +594,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         goto L333;
     }
 
-    for (int iter = a.resume ? a.resume : 1; iter <= 2; ++iter) {
+    for (int iter = a.resume ? (kSplit ? a.resume : 2) : 1; iter <= 2; ++iter) {
         exit_iter = iter;
         const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
@@ -595,10 +603,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         const D tscaleh = cush;
         cush = -1.0;
 
-        if (a.resume == iter) {
+        if (kSplit ? a.resume == iter : a.resume != 0) {
             // a deferred column continues with the scalars it saved: at the second pass
             // those of its implicit-CIN restore, whose convection starts afresh
-            if (iter == 2) zero_pass_state();
+            if (!kSplit || iter == 2) zero_pass_state();
         } else {
         // ---- S1 kinv (uwshcu.F90:1455-1468) ----
         kinv = 1;
@@ -711,7 +719,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                 kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
                 thvlmin = thvlmin_o; qtsrc = qtsrc_o; thvlsrc = thvlsrc_o; thlsrc = thlsrc_o; usrc = usrc_o;
                 vsrc = vsrc_o; thv0lcl = thv0lcl_o;
-                if ((a.defer_list || a.defer_mark) && !a.defer_at_closure) {
+                if (kSplit && (a.defer_list || a.defer_mark) && !a.defer_at_closure) {
                     // the second pass runs in a launch of its own: save, list, stop
                     cush = tscaleh;
                     defer_here(a.defer_mark != nullptr);
@@ -730,9 +738,9 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
 
         // From here on the environment is set A (the input state).
         {
-        R wtw = R(0);
+        R wtw;   // set by the closure, or at the start of the buoyancy sorting
         // a column continued at its first pass (resume 1) resumes at the buoyancy sorting
-        if (!(a.resume == 1 && iter == 1)) {
+        if (kSplit ? !(a.resume == 1 && iter == 1) : true) {
         // ---- S8 release level (uwshcu.F90:2035-2043) ----
         if (klcl < kinv) {
             krel = kinv;
@@ -831,7 +839,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
         }
         for (int m = 1; m <= ncnst; ++m) WT(1, m, krel - 1) = R(trin(m, 1));   // trsrc = tr0(1,m)
 
-        if (iter == 1 && a.defer_at_closure && (a.defer_mark || a.defer_list)) {
+        if (kSplit && iter == 1 && a.defer_at_closure && (a.defer_mark || a.defer_list)) {
             // it convects: the plume stages run in a launch of their own, packed with the
             // other columns that got this far
             cush = tscaleh;

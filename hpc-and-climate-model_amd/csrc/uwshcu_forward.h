@@ -9,7 +9,7 @@ namespace uwshcu {
 // One thread per column; the saturation table is staged into LDS once per workgroup.
 // Thread j of the launch works in workspace slot j on column col0 + j, or, with a
 // column map, on column colmap[col0 + j] while col0 + j < *colcount.
-template <class R, bool ZW = false>
+template <class R, bool ZW, bool SPLIT>
 __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const uwshcu_consts* __restrict__ consts,
                                              const double* __restrict__ estbl) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
@@ -27,7 +27,7 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
     const int slot = args.resume ? col - args.slot0 : j;
     const Thermo T{consts, s_estbl};
     const Column<R, double, ZW> c(args, T, col, slot);
-    c.run_body();
+    c.template run_body<SPLIT>();
 }
 
 // Two register budgets for the same column code.  forward_kernel lets it use all 512
@@ -38,13 +38,13 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
 template <class R, bool ZW = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1)))
 void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts, const double* __restrict__ estbl) {
-    forward_body<R, ZW>(args, n, consts, estbl);
+    forward_body<R, ZW, false>(args, n, consts, estbl);   // never the first two launches of a split
 }
 template <class R, bool ZW = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
 void forward_kernel_occ2(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                          const double* __restrict__ estbl) {
-    forward_body<R, ZW>(args, n, consts, estbl);
+    forward_body<R, ZW, true>(args, n, consts, estbl);
 }
 
 // Level-parallel set-up: thread t of the launch handles workspace slot t % n and
