@@ -177,8 +177,8 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     // list cost the second launch more than the idle lanes it saves.
     const bool closure_split =
         split && ws_bytes_per_column(args.mkx, args.ncnst, sizeof(R), sizeof(double)) <= (size_t)32 * 1024;
-    int* const marks = defer_list + 2 * (size_t)n;
-    int* const bsum = defer_list + 3 * (size_t)n;
+    int* const marks = split ? defer_list + 2 * (size_t)n : nullptr;
+    int* const bsum = split ? defer_list + 3 * (size_t)n : nullptr;
     const unsigned nblk = (unsigned)((n + 255) / 256);
     if (split) {
         hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, stream, defer_count);
