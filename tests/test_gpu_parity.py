@@ -258,12 +258,14 @@ def test_host_pointer_entry_and_fortran_dropin_symbol():
         np.testing.assert_array_equal(outs[k], pm[k], err_msg=k)
 
 
-def test_mixed_precision_pipeline():
+@pytest.mark.parametrize("ncol", [8192, 140000])
+def test_mixed_precision_pipeline(ncol):
     """MIXED = FP32 plume pass + FP64 conservation/branch check + FP64 recompute of the
     listed columns.  Columns it changed equal the FP64 path bit for bit; the others
-    equal the FP32 pass.  The error of both against FP64 is printed."""
+    equal the FP32 pass.  The error of both against FP64 is printed.  At 140000
+    columns the FP32 pass runs as the split large launch."""
     from uwshcu_amd import _capi, device
-    ch = make_chunk(8192, 35, 3, seed=33, inversion_frac=0.3)
+    ch = make_chunk(ncol, 35, 3, seed=33, inversion_frac=0.3)
     ctx = device.default_context()
     inp = device.chunk_to_device(ch)
     res = {}
@@ -275,10 +277,10 @@ def test_mixed_precision_pipeline():
     changed = np.zeros(ch.ncol, bool)
     for k in f64:
         changed |= (mix[k] != raw[k]).reshape(-1, ch.ncol).any(axis=0)
-    print(f"\n[mixed 8192x35x3] recomputed {nfix} columns, {int(changed.sum())} changed", flush=True)
+    print(f"\n[mixed {ncol}x35x3] recomputed {nfix} columns, {int(changed.sum())} changed", flush=True)
     for name in ("raw", "mixed"):
         w = _worst_per_column(res[name], f64)
-        print(f"[mixed 8192x35x3] {name}: per-column error vs FP64 median {np.median(w):.2e} "
+        print(f"[mixed {ncol}x35x3] {name}: per-column error vs FP64 median {np.median(w):.2e} "
               f"p99 {np.quantile(w, 0.99):.2e}", flush=True)
     assert changed.sum() <= nfix < ch.ncol
     for k in f64:
