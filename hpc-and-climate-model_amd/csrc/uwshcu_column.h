@@ -147,6 +147,14 @@ enum { NRESUME = 56 };
 enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
 
+// Every helper of the column pass is expanded in place.  A call would pass the column
+// object, and with it the kernel arguments, by address: the compiler then keeps a copy
+// of the arguments in scratch, reloads each array pointer from it, and, unable to tell
+// global from LDS memory behind those pointers, issues flat instructions.  Expanded,
+// the pointers stay in scalar registers and every access is a global or LDS one
+// (scratch 1.2 KB -> 44 B per lane in the one-wavefront build).
+#define UW_INLINE __attribute__((always_inline))
+
 // R: bulk (plume) arithmetic type; D: the FP64 type (double, or AD in the adjoint).
 // ZW: the modified scheme compute_zwshcu (zwshcu.F90:630-4969), FP64 only; its
 // differences from compute_uwshcu are marked where they apply.
@@ -299,12 +307,12 @@ struct Column {
     // Environment at layer interfaces (uwshcu.F90:1275-1299, 4303-4327) for set base eb
     // (A_THL0 or B_THL0), from thl0/qt0 already stored in that set.  Returns false on a
     // conden failure.
-    UWSHCU_HD bool interface_env(int eb) const {
+    UWSHCU_HD UW_INLINE bool interface_env(int eb) const {
         for (int k = 1; k <= mkx; ++k)
             if (!interface_env_level(eb, k)) return false;
         return true;
     }
-    UWSHCU_HD bool interface_env_level(int eb, int k) const {
+    UWSHCU_HD UW_INLINE bool interface_env_level(int eb, int k) const {
         const int thl = eb, qt = eb + 1, ssthl = eb + 2, ssqt = eb + 3;
         const int thvb = eb == A_THL0 ? A_THV0BOT : B_THV0BOT, thvt = thvb + 1;
         const int thvlb = eb == A_THL0 ? A_THVL0BOT : B_THVL0BOT, thvlt = thvlb + 1;
@@ -328,7 +336,7 @@ struct Column {
     // Saturation state of the layer-k mean environment, evaluated once per column
     // (uwshcu.F90:2390-2397 evaluate it in every buoyancy-sorting iteration with the
     // same arguments).  Returns false when the saturation adjustment fails.
-    UWSHCU_HD bool env_mid(int k, D& thv0j, D& qs0) const {
+    UWSHCU_HD UW_INLINE bool env_mid(int k, D& thv0j, D& qs0) const {
         D& tv = E(E_THV0MID, k);
         if (tv == 0.0) {
             D th, qv, ql, qi, es, qs;
@@ -347,7 +355,7 @@ struct Column {
 
     // Zero the column's outputs and the working arrays the reference zeroes before a
     // pass (uwshcu.F90:1333-1408, 1817-1869).
-    UWSHCU_HD void zero_pass_state() const {
+    UWSHCU_HD UW_INLINE void zero_pass_state() const {
         for (int k = 0; k <= mkx; ++k) {
             out(a.umf, k) = 0.0;
             out(a.wu, k) = 0.0;
@@ -378,7 +386,7 @@ struct Column {
         }
     }
 
-    UWSHCU_HD void write_exit(int mk) const {
+    UWSHCU_HD UW_INLINE void write_exit(int mk) const {
         // uwshcu.F90:4448-4482; flxprc1/flxsnow1 keep their initial zero (uwshcu.F90:1090-1091)
         for (int k = 0; k <= mk; ++k) {
             out(a.umf, k) = 0.0; out(a.wu, k) = 0.0; out(a.slflx, k) = 0.0; out(a.qtflx, k) = 0.0;

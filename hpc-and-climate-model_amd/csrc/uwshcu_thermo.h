@@ -86,7 +86,7 @@ struct ThermoT {
     }
 
     // Saturation adjustment, uwshcu.F90:4718-4785.  Outputs th, qv, ql, qi; returns id_check.
-    UWSHCU_HD int conden(D p, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
+    UWSHCU_HD __attribute__((always_inline)) int conden(D p, D thl, D qt, D& th, D& qv, D& ql, D& qi) const {
         return conden_x(p, exnf(p), thl, qt, th, qv, ql, qi);
     }
     // The same with the Exner function of p supplied by the caller (ex must equal
