@@ -879,27 +879,36 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
 
             for (int k = krel; k <= mkx - 1; ++k) {
                 const int km1 = k - 1;
-                R thlue = W(W_THLU, km1);
-                R qtue = W(W_QTU, km1);
-                R wue = R(out(a.wu, km1));
+                // Read once for both iter_xc passes: the plume at interface k-1 and the
+                // environment of layer k, which the loop body does not write.
+                const R thlu_km1 = W(W_THLU, km1), qtu_km1 = W(W_QTU, km1);
+                const R uu_km1 = W(W_UU, km1), vu_km1 = W(W_VU, km1), thvu_km1 = W(W_THVU, km1);
+                const R wu_km1 = R(out(a.wu, km1));
+                const R z0k = R(in(a.z0, k)), dp0k = R(in(a.dp0, k)), thv0top_k = Er(A_THV0TOP, k);
+                const R ss_thl = Er(A_SSTHL0, k), ss_qt = Er(A_SSQT0, k), ss_u = Er(A_SSU0, k), ss_v = Er(A_SSV0, k);
+                const D ps0k = in(a.ps0, k), exns0k = E(A_EXNS0, k);
+                R thlue = thlu_km1;
+                R qtue = qtu_km1;
+                R wue = wu_km1;
                 wtwb = wtw;
                 const R umf_km1 = R(out(a.umf, km1));
                 R umfk = 0, ferk = 0, fdrk = 0;
+                // the environment at pe does not change between the iter_xc passes
+                R thv0j;
+                D es, qs;
+                if (k == krel) {
+                    if (T.conden_x(pe, exe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    thv0j = thv_of(th, qvj, qlj, qij);
+                    T.qsat_es_qs(D(thle * exne), pe, es, qs);
+                } else {
+                    D tv;
+                    if (!env_mid(k, tv, qs)) { exit_code = EX_CONDEN; goto L333; }
+                    thv0j = R(tv);
+                }
+                rho0j = pe / (r * thv0j * exne);
+                const R excess0 = qte - R(qs);
                 for (int iter_xc = 1; iter_xc <= 2; ++iter_xc) {
-                    wtw = R(out(a.wu, km1)) * R(out(a.wu, km1));
-                    R thv0j;
-                    D es, qs;
-                    if (k == krel) {
-                        if (T.conden_x(pe, exe, thle, qte, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
-                        thv0j = thv_of(th, qvj, qlj, qij);
-                        T.qsat_es_qs(D(thle * exne), pe, es, qs);
-                    } else {
-                        D tv;
-                        if (!env_mid(k, tv, qs)) { exit_code = EX_CONDEN; goto L333; }
-                        thv0j = R(tv);
-                    }
-                    rho0j = pe / (r * thv0j * exne);
-                    const R excess0 = qte - R(qs);
+                    wtw = wu_km1 * wu_km1;
                     if (T.conden_x(pe, exe, thlue, qtue, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
@@ -964,16 +973,16 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     near(xc - R(0.5), R(1));
                     const R ee2 = xc * xc;
                     const R ud2 = R(1) - R(2) * xc + xc * xc;
-                    R reik = (R(0.5) * rkm / R(in(a.z0, k)) / g / rho0j);
+                    R reik = (R(0.5) * rkm / z0k / g / rho0j);
                     if (xc > R(0.5))
-                        reik = fmin(reik, R(0.9) * ulog(R(in(a.dp0, k)) / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
+                        reik = fmin(reik, R(0.9) * ulog(dp0k / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
                     W(W_REI, k) = reik;
                     ferk = reik * ee2;
                     fdrk = reik * ud2;
                     if constexpr (ZW) {
                         if (a.zw.ent) {
                             // zwshcu.F90:2818-2844: entrainment from the updraft buoyancy
-                            const R bog = rbuoy * (W(W_THVU, km1) / thvebot - R(1)) * g;
+                            const R bog = rbuoy * (thvu_km1 / thvebot - R(1)) * g;
                             ferk = R(a.zw.ent0) * bog / fmax(wue * wue, R(1.0e-4));
                             const R fer_min = bog > R(1.e-4) ? R(1.e-4) : R(0);
                             ferk = fmin(fmax(ferk, fer_min), R(3.0e-2));
@@ -984,12 +993,11 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     umfk = umf_km1 * uexp(dpe * (ferk - fdrk));
                     W(W_EMF, k) = R(0);
                     R thluk, qtuk, uuk, vuk;
-                    const R ss_thl = Er(A_SSTHL0, k), ss_qt = Er(A_SSQT0, k), ss_u = Er(A_SSU0, k), ss_v = Er(A_SSV0, k);
                     if (ferk * dpe < R(1.e-4)) {
-                        thluk = W(W_THLU, km1) + (thle + ss_thl * dpe / R(2) - W(W_THLU, km1)) * ferk * dpe;
-                        qtuk = W(W_QTU, km1) + (qte + ss_qt * dpe / R(2) - W(W_QTU, km1)) * ferk * dpe;
-                        uuk = W(W_UU, km1) + (ue + ss_u * dpe / R(2) - W(W_UU, km1)) * ferk * dpe - PGFc * ss_u * dpe;
-                        vuk = W(W_VU, km1) + (ve + ss_v * dpe / R(2) - W(W_VU, km1)) * ferk * dpe - PGFc * ss_v * dpe;
+                        thluk = thlu_km1 + (thle + ss_thl * dpe / R(2) - thlu_km1) * ferk * dpe;
+                        qtuk = qtu_km1 + (qte + ss_qt * dpe / R(2) - qtu_km1) * ferk * dpe;
+                        uuk = uu_km1 + (ue + ss_u * dpe / R(2) - uu_km1) * ferk * dpe - PGFc * ss_u * dpe;
+                        vuk = vu_km1 + (ve + ss_v * dpe / R(2) - vu_km1) * ferk * dpe - PGFc * ss_v * dpe;
                         for (int m = 1; m <= ncnst; ++m) {
                             const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
                             WT(1, m, k) = WT(1, m, km1) + (tre + WT(0, m, k) * dpe / R(2) - WT(1, m, km1)) * ferk * dpe;
@@ -997,13 +1005,13 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     } else {
                         const R ef = uexp(-ferk * dpe);
                         thluk = (thle + ss_thl / ferk - ss_thl * dpe / R(2)) -
-                                (thle + ss_thl * dpe / R(2) - W(W_THLU, km1) + ss_thl / ferk) * ef;
+                                (thle + ss_thl * dpe / R(2) - thlu_km1 + ss_thl / ferk) * ef;
                         qtuk = (qte + ss_qt / ferk - ss_qt * dpe / R(2)) -
-                               (qte + ss_qt * dpe / R(2) - W(W_QTU, km1) + ss_qt / ferk) * ef;
+                               (qte + ss_qt * dpe / R(2) - qtu_km1 + ss_qt / ferk) * ef;
                         uuk = (ue + (R(1) - PGFc) * ss_u / ferk - ss_u * dpe / R(2)) -
-                              (ue + ss_u * dpe / R(2) - W(W_UU, km1) + (R(1) - PGFc) * ss_u / ferk) * ef;
+                              (ue + ss_u * dpe / R(2) - uu_km1 + (R(1) - PGFc) * ss_u / ferk) * ef;
                         vuk = (ve + (R(1) - PGFc) * ss_v / ferk - ss_v * dpe / R(2)) -
-                              (ve + ss_v * dpe / R(2) - W(W_VU, km1) + (R(1) - PGFc) * ss_v / ferk) * ef;
+                              (ve + ss_v * dpe / R(2) - vu_km1 + (R(1) - PGFc) * ss_v / ferk) * ef;
                         for (int m = 1; m <= ncnst; ++m) {
                             const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
                             const R sst = WT(0, m, k);
@@ -1011,17 +1019,17 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                                           (tre + sst * dpe / R(2) - WT(1, m, km1) + sst / ferk) * ef;
                         }
                     }
-                    if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                    if (T.conden_x(ps0k, exns0k, thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     near(qlj + qij - criqc, criqc);
                     if ((R(qlj) + R(qij)) > criqc) {
                         const R qlr = R(qlj), qir = R(qij);
                         const R exql = ((qlr + qir) - criqc) * qlr / (qlr + qir);
                         const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
                         qtuk = qtuk - exql - exqi;
-                        thluk = thluk + (xlv / cp / Er(A_EXNS0, k)) * exql + (xls / cp / Er(A_EXNS0, k)) * exqi;
+                        thluk = thluk + (xlv / cp / R(exns0k)) * exql + (xls / cp / R(exns0k)) * exqi;
                         W(W_DWTEN, k) = exql;
                         W(W_DITEN, k) = exqi;
-                        if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
+                        if (T.conden_x(ps0k, exns0k, thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     } else {
                         W(W_DWTEN, k) = R(0);
                         W(W_DITEN, k) = R(0);
@@ -1033,8 +1041,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     out(a.umf, k) = D(umfk);
                     const R thvuk = thv_of(th, qvj, qlj, qij);
                     W(W_THVU, k) = thvuk;
-                    bogbot = rbuoy * (W(W_THVU, km1) / thvebot - R(1));
-                    bogtop = rbuoy * (thvuk / Er(A_THV0TOP, k) - R(1));
+                    bogbot = rbuoy * (thvu_km1 / thvebot - R(1));
+                    bogtop = rbuoy * (thvuk / thv0top_k - R(1));
                     const R delbog = bogtop - bogbot;
                     drage = ferk * (R(1) + rdrag);
                     const R expfac = uexp(R(-2) * drage * dpe);
@@ -1046,8 +1054,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     near(wtw, wtwb);
                     near(bogtop, R(1));
                     if (wtw > R(0)) {
-                        thlue = R(0.5) * (W(W_THLU, km1) + thluk);
-                        qtue = R(0.5) * (W(W_QTU, km1) + qtuk);
+                        thlue = R(0.5) * (thlu_km1 + thluk);
+                        qtue = R(0.5) * (qtu_km1 + qtuk);
                         wue = R(0.5) * sqrt(fmax(wtwb + wtw, R(0)));
                     } else {
                         break;   // go to 111
@@ -1061,7 +1069,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     const R wuk = sqrt(wtw);
                     out(a.wu, k) = D(wuk);
                     if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
-                    const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
+                    const R rhos0j = R(ps0k) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + thv0top_k) * R(exns0k));
                     R ufrck = umfk / (rhos0j * wuk);
                     near(ufrck - rmaxfrac, rmaxfrac);
                     if (ufrck > rmaxfrac) {
