@@ -998,10 +998,6 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                         qtuk = qtu_km1 + (qte + ss_qt * dpe / R(2) - qtu_km1) * ferk * dpe;
                         uuk = uu_km1 + (ue + ss_u * dpe / R(2) - uu_km1) * ferk * dpe - PGFc * ss_u * dpe;
                         vuk = vu_km1 + (ve + ss_v * dpe / R(2) - vu_km1) * ferk * dpe - PGFc * ss_v * dpe;
-                        for (int m = 1; m <= ncnst; ++m) {
-                            const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
-                            WT(1, m, k) = WT(1, m, km1) + (tre + WT(0, m, k) * dpe / R(2) - WT(1, m, km1)) * ferk * dpe;
-                        }
                     } else {
                         const R ef = uexp(-ferk * dpe);
                         thluk = (thle + ss_thl / ferk - ss_thl * dpe / R(2)) -
@@ -1012,12 +1008,6 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                               (ue + ss_u * dpe / R(2) - uu_km1 + (R(1) - PGFc) * ss_u / ferk) * ef;
                         vuk = (ve + (R(1) - PGFc) * ss_v / ferk - ss_v * dpe / R(2)) -
                               (ve + ss_v * dpe / R(2) - vu_km1 + (R(1) - PGFc) * ss_v / ferk) * ef;
-                        for (int m = 1; m <= ncnst; ++m) {
-                            const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
-                            const R sst = WT(0, m, k);
-                            WT(1, m, k) = (tre + sst / ferk - sst * dpe / R(2)) -
-                                          (tre + sst * dpe / R(2) - WT(1, m, km1) + sst / ferk) * ef;
-                        }
                     }
                     if (T.conden_x(ps0k, exns0k, thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     near(qlj + qij - criqc, criqc);
@@ -1062,6 +1052,23 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     }
                 }
                 // 111 continue
+                // Updraft tracers (uwshcu.F90:2592-2614).  Nothing in an iter_xc pass reads
+                // them and each pass overwrites the last, so they are formed once, from the
+                // fer(k) of the pass that ended the loop.
+                if (ferk * dpe < R(1.e-4)) {
+                    for (int m = 1; m <= ncnst; ++m) {
+                        const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
+                        WT(1, m, k) = WT(1, m, km1) + (tre + WT(0, m, k) * dpe / R(2) - WT(1, m, km1)) * ferk * dpe;
+                    }
+                } else {
+                    const R ef = uexp(-ferk * dpe);
+                    for (int m = 1; m <= ncnst; ++m) {
+                        const R tre = tr_mid ? R(trin(m, ktr)) : R(trin(m, ktr)) + WT(0, m, ktr) * pe_tr;
+                        const R sst = WT(0, m, k);
+                        WT(1, m, k) = (tre + sst / ferk - sst * dpe / R(2)) -
+                                      (tre + sst * dpe / R(2) - WT(1, m, km1) + sst / ferk) * ef;
+                    }
+                }
                 if (umfk == R(0)) wtw = R(-1);
                 if (bogtop > R(0) && wtw > R(0)) kbup = k;
                 if (wtw <= R(0)) { kpen = k; break; }   // go to 45
