@@ -893,6 +893,9 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                 wtwb = wtw;
                 const R umf_km1 = R(out(a.umf, km1));
                 R umfk = 0, ferk = 0, fdrk = 0;
+                // Level-k results of an iter_xc pass.  Each pass overwrites the last and none
+                // reads them back, so they are stored once, after the passes.
+                R reik = 0, thluk = 0, qtuk = 0, uuk = 0, vuk = 0, thvuk = 0, dwtenk = 0, ditenk = 0;
                 // the environment at pe does not change between the iter_xc passes
                 R thv0j;
                 D es, qs;
@@ -973,10 +976,9 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     near(xc - R(0.5), R(1));
                     const R ee2 = xc * xc;
                     const R ud2 = R(1) - R(2) * xc + xc * xc;
-                    R reik = (R(0.5) * rkm / z0k / g / rho0j);
+                    reik = (R(0.5) * rkm / z0k / g / rho0j);
                     if (xc > R(0.5))
                         reik = fmin(reik, R(0.9) * ulog(dp0k / g / dt / umf_km1 + R(1)) / dpe / (R(2) * xc - R(1)));
-                    W(W_REI, k) = reik;
                     ferk = reik * ee2;
                     fdrk = reik * ud2;
                     if constexpr (ZW) {
@@ -988,11 +990,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                             ferk = fmin(fmax(ferk, fer_min), R(3.0e-2));
                         }
                     }
-                    W(W_FER, k) = ferk;
-                    W(W_FDR, k) = fdrk;
                     umfk = umf_km1 * uexp(dpe * (ferk - fdrk));
-                    W(W_EMF, k) = R(0);
-                    R thluk, qtuk, uuk, vuk;
                     if (ferk * dpe < R(1.e-4)) {
                         thluk = thlu_km1 + (thle + ss_thl * dpe / R(2) - thlu_km1) * ferk * dpe;
                         qtuk = qtu_km1 + (qte + ss_qt * dpe / R(2) - qtu_km1) * ferk * dpe;
@@ -1017,20 +1015,14 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                         const R exqi = ((qlr + qir) - criqc) * qir / (qlr + qir);
                         qtuk = qtuk - exql - exqi;
                         thluk = thluk + (xlv / cp / R(exns0k)) * exql + (xls / cp / R(exns0k)) * exqi;
-                        W(W_DWTEN, k) = exql;
-                        W(W_DITEN, k) = exqi;
+                        dwtenk = exql;
+                        ditenk = exqi;
                         if (T.conden_x(ps0k, exns0k, thluk, qtuk, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                     } else {
-                        W(W_DWTEN, k) = R(0);
-                        W(W_DITEN, k) = R(0);
+                        dwtenk = R(0);
+                        ditenk = R(0);
                     }
-                    W(W_THLU, k) = thluk;
-                    W(W_QTU, k) = qtuk;
-                    W(W_UU, k) = uuk;
-                    W(W_VU, k) = vuk;
-                    out(a.umf, k) = D(umfk);
-                    const R thvuk = thv_of(th, qvj, qlj, qij);
-                    W(W_THVU, k) = thvuk;
+                    thvuk = thv_of(th, qvj, qlj, qij);
                     bogbot = rbuoy * (thvu_km1 / thvebot - R(1));
                     bogtop = rbuoy * (thvuk / thv0top_k - R(1));
                     const R delbog = bogtop - bogbot;
@@ -1052,6 +1044,18 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     }
                 }
                 // 111 continue
+                W(W_REI, k) = reik;
+                W(W_FER, k) = ferk;
+                W(W_FDR, k) = fdrk;
+                W(W_EMF, k) = R(0);
+                W(W_DWTEN, k) = dwtenk;
+                W(W_DITEN, k) = ditenk;
+                W(W_THLU, k) = thluk;
+                W(W_QTU, k) = qtuk;
+                W(W_UU, k) = uuk;
+                W(W_VU, k) = vuk;
+                out(a.umf, k) = D(umfk);
+                W(W_THVU, k) = thvuk;
                 // Updraft tracers (uwshcu.F90:2592-2614).  Nothing in an iter_xc pass reads
                 // them and each pass overwrites the last, so they are formed once, from the
                 // fer(k) of the pass that ended the loop.
