@@ -876,22 +876,30 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
             R pe_tr = pe - R(in(a.p0, krel));
             int ktr = krel;            // layer whose (tr0, sstr0) define tre, and offset pe_tr
             bool tr_mid = false;       // after the first layer tre = tr0(k) exactly
+            // The plume at interface k-1: read from the workspace for the first level, then
+            // carried over from the level below, which stored the same values.
+            R c_thlu = W(W_THLU, krel - 1), c_qtu = W(W_QTU, krel - 1), c_uu = W(W_UU, krel - 1);
+            R c_vu = W(W_VU, krel - 1), c_thvu = W(W_THVU, krel - 1);
+            R c_wu = R(out(a.wu, krel - 1)), c_umf = R(out(a.umf, krel - 1));
 
             for (int k = krel; k <= mkx - 1; ++k) {
                 const int km1 = k - 1;
-                // Read once for both iter_xc passes: the plume at interface k-1 and the
-                // environment of layer k, which the loop body does not write.
-                const R thlu_km1 = W(W_THLU, km1), qtu_km1 = W(W_QTU, km1);
-                const R uu_km1 = W(W_UU, km1), vu_km1 = W(W_VU, km1), thvu_km1 = W(W_THVU, km1);
-                const R wu_km1 = R(out(a.wu, km1));
+                // Read once for both iter_xc passes: the environment of layer k, which the loop
+                // body does not write, and (issued early) that of layer k+1 for the next level.
+                const R thlu_km1 = c_thlu, qtu_km1 = c_qtu;
+                const R uu_km1 = c_uu, vu_km1 = c_vu, thvu_km1 = c_thvu;
+                const R wu_km1 = c_wu;
                 const R z0k = R(in(a.z0, k)), dp0k = R(in(a.dp0, k)), thv0top_k = Er(A_THV0TOP, k);
                 const R ss_thl = Er(A_SSTHL0, k), ss_qt = Er(A_SSQT0, k), ss_u = Er(A_SSU0, k), ss_v = Er(A_SSV0, k);
                 const D ps0k = in(a.ps0, k), exns0k = E(A_EXNS0, k);
+                const D p0_kp1 = in(a.p0, k + 1), dp0_kp1 = in(a.dp0, k + 1), exn0_kp1 = E(A_EXN0, k + 1);
+                const R thv0bot_kp1 = Er(A_THV0BOT, k + 1), thl0_kp1 = Er(A_THL0, k + 1), qt0_kp1 = Er(A_QT0, k + 1);
+                const D u0_kp1 = in(a.u0, k + 1), v0_kp1 = in(a.v0, k + 1);
                 R thlue = thlu_km1;
                 R qtue = qtu_km1;
                 R wue = wu_km1;
                 wtwb = wtw;
-                const R umf_km1 = R(out(a.umf, km1));
+                const R umf_km1 = c_umf;
                 R umfk = 0, ferk = 0, fdrk = 0;
                 // Level-k results of an iter_xc pass.  Each pass overwrites the last and none
                 // reads them back, so they are stored once, after the passes.
@@ -1080,7 +1088,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                     const R wuk = sqrt(wtw);
                     out(a.wu, k) = D(wuk);
                     if (wuk > R(100)) { exit_code = EX_WU; goto L333; }
-                    const R rhos0j = R(ps0k) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + thv0top_k) * R(exns0k));
+                    const R rhos0j = R(ps0k) / (r * R(0.5) * (thv0bot_kp1 + thv0top_k) * R(exns0k));
                     R ufrck = umfk / (rhos0j * wuk);
                     near(ufrck - rmaxfrac, rmaxfrac);
                     if (ufrck > rmaxfrac) {
@@ -1091,17 +1099,24 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
                         W(W_FDR, k) = ferk - ulog(umfk / umf_km1) / dpe;
                     }
                     W(W_UFRC, k) = ufrck;
+                    c_wu = R(D(wuk));
+                    c_umf = R(D(umfk));
                 }
-                pe = R(in(a.p0, k + 1));
-                dpe = R(in(a.dp0, k + 1));
-                exne = Er(A_EXN0, k + 1);
-                if constexpr (sizeof(R) == sizeof(double)) exe = E(A_EXN0, k + 1);   // = exnf(pe), pe = p0(k+1)
+                c_thlu = thluk;
+                c_qtu = qtuk;
+                c_uu = uuk;
+                c_vu = vuk;
+                c_thvu = thvuk;
+                pe = R(p0_kp1);
+                dpe = R(dp0_kp1);
+                exne = R(exn0_kp1);
+                if constexpr (sizeof(R) == sizeof(double)) exe = exn0_kp1;   // = exnf(pe), pe = p0(k+1)
                 else exe = T.exnf(D(pe));
-                thvebot = Er(A_THV0BOT, k + 1);
-                thle = Er(A_THL0, k + 1);
-                qte = Er(A_QT0, k + 1);
-                ue = R(in(a.u0, k + 1));
-                ve = R(in(a.v0, k + 1));
+                thvebot = thv0bot_kp1;
+                thle = thl0_kp1;
+                qte = qt0_kp1;
+                ue = R(u0_kp1);
+                ve = R(v0_kp1);
                 ktr = k + 1;
                 tr_mid = true;
             }
