@@ -18,6 +18,7 @@ __global__ __launch_bounds__(256) void merge_shallow_kernel(int pcols, int ncol,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pcols) return;
     f.massflx_sh[i] = f.cbmf[i];               // pbuf(massflx_sh_idx)(1:pcols) = cbmf(1:pcols)
+    f.freqsh[i] = 0.0;                         // :790 freqsh(:) = 0 over all pcols
     if (i >= ncol) return;
     const size_t P = (size_t)pcols;
     // convect_shallow.F90:616-620: in-cloud water, shallow rain + snow production

@@ -37,7 +37,12 @@ class UwshcuFunction(torch.autograd.Function):
         out = context.alloc_outputs(pcols, pver, ncnst)
         cush = inp["cush"].clone()
         context.forward_into(inp, out, cush, dt=dt, params=par)
-        fctx.inp = inp
+        # Every input tensor goes through save_for_backward, so an in-place change
+        # between forward and backward raises autograd's version error instead of
+        # re-recording the tape on changed values.
+        names = tuple(inp.keys())
+        fctx.save_for_backward(*(inp[k] for k in names))
+        fctx.names = names
         fctx.par = pv
         fctx.context = context
         fctx.dt = dt
@@ -50,7 +55,8 @@ class UwshcuFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(fctx, *grad_out):
-        context, inp = fctx.context, fctx.inp
+        context = fctx.context
+        inp = dict(zip(fctx.names, fctx.saved_tensors))
         pver, pcols = inp["p0"].shape
         ncnst = inp["tr0"].shape[0] if inp["tr0"].dim() == 3 else 0
         cot = {k: g for k, g in zip(OUTPUTS, grad_out) if g is not None}

@@ -32,6 +32,13 @@ Column recipe (all random draws from one ``numpy.random.Generator(PCG64(seed))``
 * free-troposphere RH falls from 0.90 at the PBL top (not the surface) to 0.2 at
   500 hPa; with the surface value the lower free troposphere is dry enough that
   uncapped shallow PBLs rarely convect.
+
+``recipe="baseline"`` is the stated column recipe of the benchmark configurations
+word for word: 6.5 K/km from the surface (no well-mixed PBL), RH 0.85 at the surface
+falling linearly in pressure to 0.2 at 500 hPa plus the clipped noise at every
+layer, ``qv = RH * qs(T, p)`` throughout, and a +3 K inversion (``inversion_dt``
+default 3) above the PBL on ``inversion_frac`` (0.25) of the columns.  The random
+draws are the same as the default recipe's, in the same order.
 """
 from __future__ import annotations
 
@@ -124,7 +131,13 @@ class Chunk:
 
 
 def make_chunk(ncol: int, pver: int = 30, ncnst: int = 3, seed: int = 0, dt: float = 1800.0,
-               inversion_frac: float = 0.25, ptop: float = 200.0, inversion_dt: float = 7.0) -> Chunk:
+               inversion_frac: float = 0.25, ptop: float = 200.0, inversion_dt: float | None = None,
+               recipe: str = "pbl") -> Chunk:
+    if recipe not in ("pbl", "baseline"):
+        raise ValueError(f"unknown recipe {recipe!r}")
+    well_mixed = recipe == "pbl"
+    if inversion_dt is None:
+        inversion_dt = 7.0 if well_mixed else 3.0
     rng = np.random.default_rng(np.random.PCG64(seed))
     nl, ni = pver, pver + 1
     a, b = hybrid_coefficients(pver, ptop)
@@ -141,6 +154,9 @@ def make_chunk(ncol: int, pver: int = 30, ncnst: int = 3, seed: int = 0, dt: flo
 
     def temp_of_z(z):
         zb = pblh[None, :]
+        if not well_mixed:
+            t = tsfc[None, :] - 6.5e-3 * np.minimum(z, 16000.0)
+            return t + np.where(inv[None, :] & (z > zb), inversion_dt, 0.0)
         t_pbl = tsfc[None, :] - gamma_d * np.minimum(z, zb)
         t = t_pbl - 6.5e-3 * (np.minimum(z, 16000.0) - np.minimum(np.minimum(z, 16000.0), zb))
         return t + np.where(inv[None, :] & (z > zb), inversion_dt, 0.0)
@@ -153,7 +169,10 @@ def make_chunk(ncol: int, pver: int = 30, ncnst: int = 3, seed: int = 0, dt: flo
     # (hypsometric estimate of its pressure) to 0.2 at 500 hPa
     p_pbl = ps * np.exp(-pblh / ((RAIR / GRAV) * tsfc))
     rh_base = np.where(inv, RH_FT_BASE_CAPPED, RH_FT_BASE)
-    rh = np.clip(rh_of_p(pmid, p_pbl, rh_base) + rh_noise, 0.05, 1.0)
+    if well_mixed:
+        rh = np.clip(rh_of_p(pmid, p_pbl, rh_base) + rh_noise, 0.05, 1.0)
+    else:
+        rh = np.clip(rh_of_p(pmid, ps, np.full(ncol, 0.85)) + rh_noise, 0.05, 1.0)
     qv_sfc = 0.85 * qsat_table(tsfc, ps)
     zi = np.zeros((ni, ncol))
     zm = np.zeros((nl, ncol))
@@ -167,7 +186,10 @@ def make_chunk(ncol: int, pver: int = 30, ncnst: int = 3, seed: int = 0, dt: flo
             tk = temp_of_z(zmid[None, :])[0]
             qs_k = qsat_table(tk, pmid[k])
             # well-mixed PBL: the surface specific humidity, capped at 98 % RH
-            qk = np.where(zmid < pblh, np.minimum(qv_sfc, 0.98 * qs_k), rh[k] * qs_k)
+            if well_mixed:
+                qk = np.where(zmid < pblh, np.minimum(qv_sfc, 0.98 * qs_k), rh[k] * qs_k)
+            else:
+                qk = rh[k] * qs_k
             tv = tk * (1.0 + ZVIR * qk)
             zmid = zb + rog * tv * np.log(pint[k + 1] / pmid[k])
         t[k], qv[k], zm[k] = tk, qk, zmid

@@ -162,7 +162,8 @@ int uwshcu_forward_chunks_device(uwshcu_ctx *ctx, int precision, const uwshcu_pa
  *   freqsh = 1 if max(cmfmc2(i, 1:pver)) <= 0 else 0; cmfmc(i, 1:pver) += cmfmc2(i, 1:pver);
  *   cnt = min(cnt, cnt2); cnb = max(cnb, cnb2); pcnt = pmid(i, int(cnt)); pcnb = pmid(i, int(cnb));
  *   rprdtot = rprdsh + rprddp; qc += qc2; rliq += rliq2;
- * and massflx_sh(1:pcols) = cbmf(1:pcols).  Level indices outside 1..pver are clamped
+ * and, over all pcols columns, freqsh = 0 for i >= ncol (convect_shallow.F90:790) and
+ * massflx_sh(1:pcols) = cbmf(1:pcols).  Level indices outside 1..pver are clamped
  * to that range for pcnt/pcnb (the driver would read out of bounds).  The driver's
  * copy ptend%q(:, :, 4:pcnst) = trten(:, :, 4:pcnst) is a plain device copy of this
  * library's trten and is left to the caller.  Asynchronous on `stream`. */

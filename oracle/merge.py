@@ -17,7 +17,9 @@ FIELDS_OUT = ("icwmr", "rprdsh", "rprdtot", "cmfsl", "cmflq", "freqsh", "pcnt", 
 
 def merge(f: dict, ncol: int, pver: int, latvap: float) -> dict:
     """Returns the in/out and out fields after the merge (copies; ``f`` is not changed).
-    Columns >= ncol keep their input values (out fields: their ``f`` values)."""
+    Columns >= ncol keep their input values (out fields: their ``f`` values), except
+    freqsh, which the driver zeroes over all pcols first, and massflx_sh, copied over
+    all pcols."""
     o = {k: np.array(f[k], copy=True) for k in FIELDS_IO + FIELDS_OUT}
     n = slice(0, ncol)
     o["icwmr"][:, n] = f["qcu"][:, n]                                   # :616  icwmr = iccmr_UW
@@ -25,7 +27,8 @@ def merge(f: dict, ncol: int, pver: int, latvap: float) -> dict:
     o["massflx_sh"][:] = f["cbmf"][:]                                   # :623  massflx_sh(1:pcols) = cbmf(1:pcols)
     o["cmfsl"][:, n] = f["slflx"][:, n]                                 # :653  cmfsl = slflx
     o["cmflq"][:, n] = f["qtflx"][:, n] * latvap                        # :654  cmflq = qtflx * latvap
-    mx = f["umf"][:pver, n].max(axis=0)                                 # :790-795 freqsh
+    o["freqsh"][:] = 0.0                                                # :790  freqsh(:) = 0
+    mx = f["umf"][:pver, n].max(axis=0)                                 # :791-795 freqsh
     o["freqsh"][n] = np.where(mx <= 0.0, 1.0, 0.0)
     o["cmfmc"][:pver, n] = f["cmfmc"][:pver, n] + f["umf"][:pver, n]    # :806
     cnt = np.where(f["cnt2"][n] < f["cnt"][n], f["cnt2"][n], f["cnt"][n])   # :816

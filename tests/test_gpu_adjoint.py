@@ -104,3 +104,22 @@ def test_autograd_backward_matches_the_adjoint():
         np.testing.assert_allclose(inp[k].grad.cpu().numpy(), hg[k], rtol=1e-12, atol=1e-300, err_msg=k)
     np.testing.assert_allclose(params.grad.numpy(), hg["par"].sum(axis=1), rtol=1e-10)
     assert not inp["pblh"].grad.any()
+
+
+def test_autograd_rejects_inputs_changed_in_place():
+    """Inputs are kept through save_for_backward: an in-place change between forward and
+    backward raises autograd's version error instead of giving gradients of other values."""
+    import torch
+    from uwshcu_amd import device
+    from uwshcu_amd.autograd import uwshcu
+    ch = make_chunk(64, 35, 3, seed=65)
+    for name in ("t0", "p0"):
+        inp = device.chunk_to_device(ch, "cuda:0")
+        inp["t0"].requires_grad_(True)
+        params = torch.tensor([10.0, 0.1, 14.0], dtype=torch.float64, requires_grad=True)
+        out = uwshcu(inp, params, dt=ch.dt)
+        loss = out["sten"].sum()
+        with torch.no_grad():
+            inp[name].add_(0.5)
+        with pytest.raises(RuntimeError, match="inplace"):
+            loss.backward()

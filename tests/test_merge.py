@@ -40,8 +40,9 @@ def test_merge_restatement_known_values():
     assert o["cmfmc"][0, 0] == f["cmfmc"][0, 0] + 0.02 and o["cmfmc"][pver, 1] == f["cmfmc"][pver, 1]
     assert o["qc"][0, 1] == f["qc"][0, 1] + f["qc2"][0, 1] and o["rliq"][1] == f["rliq"][1] + f["rliq2"][1]
     np.testing.assert_array_equal(o["massflx_sh"], f["cbmf"])        # all pcols, ncol notwithstanding
-    for k in ("cnt", "cnb", "qc", "icwmr", "freqsh", "pcnt"):       # column 2 >= ncol: untouched
+    for k in ("cnt", "cnb", "qc", "icwmr", "pcnt"):                 # column 2 >= ncol: untouched
         np.testing.assert_array_equal(o[k][..., 2], f[k][..., 2], err_msg=k)
+    assert f["freqsh"][2] != 0.0 and o["freqsh"][2] == 0.0           # :790 freqsh(:) = 0 over all pcols
 
 
 def test_merge_abi_without_a_gpu():
@@ -65,3 +66,6 @@ def test_device_merge_equals_the_restatement():
     torch.cuda.synchronize()
     for k, v in want.items():
         np.testing.assert_array_equal(d[k].cpu().numpy(), v, err_msg=k)
+    # padding columns: freqsh zeroed (convect_shallow.F90:790), massflx_sh = cbmf
+    assert np.all(f["freqsh"][ncol:] != 0.0)
+    np.testing.assert_array_equal(d["freqsh"][ncol:].cpu().numpy(), 0.0)
