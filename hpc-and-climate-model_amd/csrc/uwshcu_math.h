@@ -8,8 +8,7 @@
 // path therefore evaluates its transcendental functions with the sequences below,
 // which use only correctly rounded +, -, *, /, floor, frexp and ldexp and are
 // compiled without fused multiply-add: a CPU build and this GPU build give
-// identical bits.  Accuracy against glibc: exp <= 1 ulp, log and pow <= 2 ulp,
-// erfc <= 3e-13 relative.
+// identical bits.  Accuracy: exp <= 1 ulp, log and pow <= 2 ulp, erfc <= 4 ulp.
 //
 // Methods:
 //   exp   Cody-Waite reduction x = k ln2 + r (ln2 split so that k*ln2_hi is exact),
@@ -17,9 +16,13 @@
 //   log   x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh((m-1)/(m+1)) by its
 //         odd series to degree 23.
 //   pow   exp(y log x) for x > 0.
-//   erfc  x < 2: 1 - erf(x), erf from the positive series
-//         2x/sqrt(pi) e^{-x^2} sum (2x^2)^n / (1*3*...*(2n+1)) (48 terms);
-//         x >= 2: Laplace's continued fraction, evaluated backwards from depth 60.
+//   erfc  W. J. Cody's rational Chebyshev approximations (Math. Comp. 23 (1969)
+//         631-637; CALERF of Cody's SPECFUN package), the algorithm of CESM's
+//         error_function module that the scheme calls (uwshcu.F90:5, 2116, 2145,
+//         4964): |x| <= 0.46875: 1 - x R1(x^2); 0.46875 < |x| <= 4: e^{-x^2} R2(|x|);
+//         4 < |x| < 26.543: e^{-x^2} (1/sqrt(pi) - z R3(z)) / |x| with z = 1/x^2;
+//         beyond, 0.  e^{-x^2} = e^{-s^2} e^{-(|x|-s)(|x|+s)}, s = |x| truncated to a
+//         multiple of 1/16.  Negative x outside the first interval: 2 - erfc(|x|).
 #ifndef UWSHCU_MATH_H
 #define UWSHCU_MATH_H
 
@@ -94,32 +97,68 @@ UWSHCU_HD inline double pm_pow(double x, double y) {
     return pm_exp(y * pm_log(x));
 }
 
+// Coefficients of Cody's R1 (erf, first interval), R2 and R3 (erfc, second and third
+// intervals): numerators a, c, p and denominators b, d, q, in the order his Horner
+// recurrences consume them.
+struct CodyErfc {
+    static constexpr double a[5] = {3.16112374387056560e00, 1.13864154151050156e02, 3.77485237685302021e02,
+                                    3.20937758913846947e03, 1.85777706184603153e-1};
+    static constexpr double b[4] = {2.36012909523441209e01, 2.44024637934444173e02, 1.28261652607737228e03,
+                                    2.84423683343917062e03};
+    static constexpr double c[9] = {5.64188496988670089e-1, 8.88314979438837594e00, 6.61191906371416295e01,
+                                    2.98635138197400131e02, 8.81952221241769090e02, 1.71204761263407058e03,
+                                    2.05107837782607147e03, 1.23033935479799725e03, 2.15311535474403846e-8};
+    static constexpr double d[8] = {1.57449261107098347e01, 1.17693950891312499e02, 5.37181101862009858e02,
+                                    1.62138957456669019e03, 3.29079923573345963e03, 4.36261909014324716e03,
+                                    3.43936767414372164e03, 1.23033935480374942e03};
+    static constexpr double p[6] = {3.05326634961232344e-1, 3.60344899949804439e-1, 1.25781726111229246e-1,
+                                    1.60837851487422766e-2, 6.58749161529837803e-4, 1.63153871373020978e-2};
+    static constexpr double q[5] = {2.56852019228982242e00, 1.87295284992346047e00, 5.27905102951428412e-1,
+                                    6.05183413124413191e-2, 2.33520497626869185e-3};
+};
+
 UWSHCU_HD inline double pm_erfc(double x) {
-    const double two_over_sqrtpi = 1.12837916709551257390;
-    const double inv_sqrtpi = 0.56418958354775628695;
+    using K = CodyErfc;
     if (x != x) return x;
-    bool neg = false;
-    if (x < 0.0) {
-        neg = true;
-        x = -x;
+    const double y = fabs(x);
+    if (y <= 0.46875) {
+        const double z = y > 1.11e-16 ? y * y : 0.0;
+        double num = K::a[4] * z, den = z;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            num = (num + K::a[i]) * z;
+            den = (den + K::b[i]) * z;
+        }
+        return 1.0 - x * (num + K::a[3]) / (den + K::b[3]);
     }
     double r;
-    if (x < 2.0) {
-        const double t = 2.0 * (x * x);
-        double term = 1.0, sum = 1.0;
-        for (int n = 1; n <= 48; ++n) {
-            term = term * t / (double)(2 * n + 1);
-            sum = sum + term;
+    if (y <= 4.0) {
+        double num = K::c[8] * y, den = y;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            num = (num + K::c[i]) * y;
+            den = (den + K::d[i]) * y;
         }
-        r = 1.0 - two_over_sqrtpi * x * pm_exp(-(x * x)) * sum;
-    } else if (x > 27.3) {
+        r = (num + K::c[7]) / (den + K::d[7]);
+    } else if (y >= 26.543) {
         r = 0.0;
     } else {
-        double f = x;
-        for (int n = 60; n >= 1; --n) f = x + (0.5 * (double)n) / f;
-        r = pm_exp(-(x * x)) * inv_sqrtpi / f;
+        const double z = 1.0 / (y * y);
+        double num = K::p[5] * z, den = z;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            num = (num + K::p[i]) * z;
+            den = (den + K::q[i]) * z;
+        }
+        r = z * (num + K::p[4]) / (den + K::q[4]);
+        r = (5.6418958354775628695e-1 - r) / y;
     }
-    return neg ? 2.0 - r : r;
+    if (y < 26.543) {
+        const double s = floor(y * 16.0) / 16.0;
+        const double del = (y - s) * (y + s);
+        r = pm_exp(-s * s) * pm_exp(-del) * r;
+    }
+    return x < 0.0 ? 2.0 - r : r;
 }
 
 // Overloads used by the templated column code: the FP64 path gets the portable

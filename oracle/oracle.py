@@ -84,6 +84,11 @@ def lib(portable: bool = False):
         L.orc_slope.argtypes = [C.c_int, dp, dp, dp]
         L.orc_qsinvert.argtypes = [pc, d, d, d]
         L.orc_qsinvert.restype = d
+        for fn in ("orc_xexp", "orc_xlog", "orc_xerfc"):
+            getattr(L, fn).argtypes = [d]
+            getattr(L, fn).restype = d
+        L.orc_xpow.argtypes = [d, d]
+        L.orc_xpow.restype = d
         L.orc_compute_alpha.argtypes = [d, d]
         L.orc_compute_alpha.restype = d
         L.orc_compute_mumin2.argtypes = [d, d, d]
@@ -98,6 +103,9 @@ def lib(portable: bool = False):
         L.orc_findsp_point.argtypes = [pc, d, d, d, dp, dp, dp]
         L.orc_findsp_point.restype = C.c_int
         L.orc_compute_uwshcu_inv_omp.restype = C.c_int
+        L.orc_set_probe.argtypes = [dp]
+        L.orc_probe_layout.argtypes = [C.POINTER(C.c_int)] * 3
+        L.orc_probe_layout.restype = C.c_int
         L.orc_compute_zwshcu_inv.restype = C.c_int
         _libs[portable] = L
     return _libs[portable]
@@ -143,6 +151,143 @@ def run(chunk, consts: Consts | None = None, nthreads: int = 0, portable: bool =
     rc = lib(portable).orc_compute_uwshcu_inv_omp(*args)
     out["cush"] = cush
     return out, diag, rc, bad.value
+
+
+# Stage-probe record (uwshcu_oracle.h, ORC_P_* and ORC_PL_*), in enum order.
+PROBE_SCALARS = ("closure", "wcrit", "sigmaw", "mu", "rho0inv", "cbmf", "winv", "wlcl", "ufrclcl", "cinlcl",
+                 "rbuoy", "rmaxfrac", "cbmflimit", "mumin2_applied",
+                 "ppen_set", "wtwb", "drage", "bogbot", "bogtop", "rho0j", "dpen", "ppen",
+                 "krel", "kbup", "kpen", "rpen", "PGFc", "g", "criqc", "emf_set", "limit_emf")
+PROBE_LEVELS = ("set", "case", "xc", "wue", "cridis", "thvj", "thv0j", "xsat", "thvxsat", "thlue", "qtue",
+                "excessu", "excess0", "thle", "qte", "ue", "ve", "pe", "dpe", "fer", "fdr",
+                "thlu0", "qtu0", "uu0", "vu0", "thlu1", "qtu1", "uu1", "vu1", "ssthl", "ssqt", "ssu", "ssv",
+                "umf0", "umf1", "rei", "wtwb_k", "wtw_k", "drage_k", "bogbot_k", "bogtop_k", "rho0j_k",
+                "emf", "thlu_emf", "qtu_emf", "thl0", "qt0", "ps0", "p0", "ssthl0", "ssqt0", "umf", "reif", "dp0")
+
+
+def column_chunk(chunk, col: int):
+    """The one-column chunk holding column ``col`` of ``chunk``."""
+    from copy import copy
+    one = copy(chunk)
+    one.fields = {k: np.ascontiguousarray(v[..., col:col + 1]) for k, v in chunk.fields.items()}
+    one.ncol = 1
+    return one
+
+
+def probe(chunk, col: int, consts: Consts | None = None, portable: bool = False):
+    """Run the oracle on column ``col`` alone with the stage probe on.  Returns (outputs,
+    diag row, record) where record maps PROBE_SCALARS names to floats and PROBE_LEVELS
+    names to arrays over UW levels 0..ORC_PRB_KMAX-1."""
+    L = lib(portable)
+    ns, km, nf = C.c_int(), C.c_int(), C.c_int()
+    total = L.orc_probe_layout(C.byref(ns), C.byref(km), C.byref(nf))
+    assert (ns.value, nf.value) == (len(PROBE_SCALARS), len(PROBE_LEVELS)), "probe layout differs"
+    buf = np.zeros(total)
+    L.orc_set_probe(buf.ctypes.data_as(C.POINTER(C.c_double)))
+    try:
+        out, diag, rc, _ = run(column_chunk(chunk, col), consts=consts, nthreads=1, portable=portable)
+    finally:
+        L.orc_set_probe(None)
+    rec = {k: float(buf[i]) for i, k in enumerate(PROBE_SCALARS)}
+    base = ns.value
+    for f, k in enumerate(PROBE_LEVELS):
+        rec[k] = buf[base + f * km.value: base + (f + 1) * km.value].copy()
+    assert rc == 0
+    return out, diag[0], rec
+
+
+LIB_PATH_COUNT = os.path.join(HERE, "liboracle_count.so")
+LIB_PATH_DUAL = os.path.join(HERE, "liboracle_dual.so")
+OP_KINDS = ("add", "mul", "div", "sqrt", "cmp", "other")
+_IN_ORDER = ("ps0", "zs0", "p0", "z0", "dp0", "u0", "v0", "qv0", "ql0", "qi0", "t0", "s0", "tr0", "tke", "cldfrc",
+             "concldfrc", "pblh", "cush", "dpdry0")
+_OUT_ORDER = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "qiten", "sten", "uten", "vten",
+              "trten", "qrten", "qsten", "precip", "snow", "evapc", "cufrc", "qcu", "qlu", "qiu", "cbmf", "qc", "rliq",
+              "cnt", "cnb", "wu")
+
+
+def count_ops(chunk, cols=None, consts: Consts | None = None):
+    """FP64 operations per column of the portable-math oracle (liboracle_count.so):
+    returns (ops [len(cols), 6] in OP_KINDS order, outputs, diag).  The outputs equal
+    run(portable=True)'s bit for bit: the counting number computes with doubles."""
+    if "count" not in _libs:
+        if not os.path.exists(LIB_PATH_COUNT):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle_count.so"], check=True)
+        Lc = C.CDLL(LIB_PATH_COUNT)
+        Lc.orc_default_consts.argtypes = [C.POINTER(Consts)]
+        Lc.orc_count_column.restype = C.c_int
+        _libs["count"] = Lc
+    Lc = _libs["count"]
+    c = consts or default_consts()
+    f = chunk.fields
+    n, L, ncnst = chunk.ncol, chunk.pver, chunk.ncnst
+    cols = range(n) if cols is None else cols
+    ins = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in _IN_ORDER}
+    ins["cush"] = ins["cush"].copy()
+    out = {k: np.zeros((L + 1, n)) for k in OUTPUT_IFACE}
+    out.update({k: np.zeros((L, n)) for k in OUTPUT_LAYER})
+    out["trten"] = np.zeros((ncnst, L, n))
+    for k in ("precip", "snow", "cbmf", "rliq", "cnt", "cnb"):
+        out[k] = np.zeros(n)
+    pin = (C.POINTER(C.c_double) * len(_IN_ORDER))(*[_p(ins[k]) for k in _IN_ORDER])
+    pout = (C.POINTER(C.c_double) * len(_OUT_ORDER))(*[_p(out[k]) for k in _OUT_ORDER])
+    diag = np.zeros((n, NDIAG), dtype=np.int32)
+    ops = np.zeros((len(cols), len(OP_KINDS)), dtype=np.uint64)
+    one = (C.c_ulonglong * len(OP_KINDS))()
+    for j, col in enumerate(cols):
+        rc = Lc.orc_count_column(C.byref(c), C.c_int(n), C.c_int(L), C.c_int(ncnst), C.c_double(chunk.dt),
+                                 C.c_int(int(col)), pin, pout,
+                                 C.cast(diag.ctypes.data + 4 * NDIAG * int(col), C.POINTER(C.c_int)), one)
+        assert rc == 0, (col, rc)
+        ops[j] = list(one)
+    out["cush"] = ins["cush"]
+    return ops, out, diag
+
+
+def jvp(chunk, tangents: dict, dpar=(0.0, 0.0, 0.0), par=(10.0, 0.1, 14.0), cols=None):
+    """Forward-mode derivative of the portable-math oracle (liboracle_dual.so, dual
+    numbers): the outputs and their tangents along the input direction ``tangents``
+    (arrays shaped like the chunk's fields, missing = zero) and the tunables' direction
+    ``dpar`` (d rpen, d rle, d rkm).  Returns (outputs, output tangents, diag) for the
+    columns ``cols`` (default all; other columns stay zero).  The values equal
+    run(portable=True)'s bit for bit."""
+    if "dual" not in _libs:
+        if not os.path.exists(LIB_PATH_DUAL):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle_dual.so"], check=True)
+        Ld = C.CDLL(LIB_PATH_DUAL)
+        Ld.orc_dual_jvp.restype = C.c_int
+        _libs["dual"] = Ld
+    Ld = _libs["dual"]
+    f = chunk.fields
+    n, L, ncnst = chunk.ncol, chunk.pver, chunk.ncnst
+    cols = range(n) if cols is None else cols
+    ins = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in _IN_ORDER}
+    tin = {k: np.ascontiguousarray(tangents[k], dtype=np.float64) for k in _IN_ORDER if k in tangents}
+    for k, v in tin.items():
+        assert v.shape == ins[k].shape, k
+
+    def zeros():
+        o = {k: np.zeros((L + 1, n)) for k in OUTPUT_IFACE}
+        o.update({k: np.zeros((L, n)) for k in OUTPUT_LAYER})
+        o["trten"] = np.zeros((ncnst, L, n))
+        for k in ("precip", "snow", "cbmf", "rliq", "cnt", "cnb"):
+            o[k] = np.zeros(n)
+        return o
+    out, tout = zeros(), zeros()
+    P = C.POINTER(C.c_double)
+    pin = (P * len(_IN_ORDER))(*[_p(ins[k]) for k in _IN_ORDER])
+    ptin = (P * len(_IN_ORDER))(*[(_p(tin[k]) if k in tin else None) for k in _IN_ORDER])
+    pout = (P * len(_OUT_ORDER))(*[_p(out[k]) for k in _OUT_ORDER])
+    ptout = (P * len(_OUT_ORDER))(*[_p(tout[k]) for k in _OUT_ORDER])
+    par_a = (C.c_double * 3)(*par)
+    dpar_a = (C.c_double * 3)(*dpar)
+    diag = np.zeros((n, NDIAG), dtype=np.int32)
+    for col in cols:
+        rc = Ld.orc_dual_jvp(C.c_int(n), C.c_int(L), C.c_int(ncnst), C.c_double(chunk.dt), C.c_int(int(col)), par_a,
+                             dpar_a, pin, ptin, pout, ptout,
+                             C.cast(diag.ctypes.data + 4 * NDIAG * int(col), C.POINTER(C.c_int)))
+        assert rc == 0, (col, rc)
+    return out, tout, diag
 
 
 def run_zw(chunk, zw: ZW, consts: Consts | None = None, portable: bool = False, nthreads: int = 1):

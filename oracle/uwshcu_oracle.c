@@ -33,10 +33,12 @@
 
 /* Transcendental functions of the column physics: the host libm by default, or
  * the portable FP64 versions of orc_math.h (-DORC_PORTABLE_MATH) that the HIP
- * kernel reproduces bit for bit.  The saturation table builder always uses the
- * host libm (the table is data shared by both sides). */
-#ifdef ORC_PORTABLE_MATH
+ * kernel reproduces bit for bit.  erfc is Cody's rational approximation in both
+ * flavours (the algorithm of CESM's error_function module), evaluated with the
+ * flavour's exp.  The saturation table builder always uses the host libm (the
+ * table is data shared by both sides). */
 #include "orc_math.h"
+#ifdef ORC_PORTABLE_MATH
 #define XEXP orc_pm_exp
 #define XLOG orc_pm_log
 #define XPOW orc_pm_pow
@@ -45,7 +47,7 @@
 #define XEXP exp
 #define XLOG log
 #define XPOW pow
-#define XERFC erfc
+#define XERFC orc_libm_erfc
 #endif
 
 #define KMAX 200            /* mkx must be < KMAX */
@@ -53,6 +55,28 @@
 
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 static inline double dmax(double a, double b) { return a > b ? a : b; }
+
+/* Stage probe (test infrastructure): when a buffer is set, the column code records
+ * the internal quantities of the listed stages there (layout in uwshcu_oracle.h),
+ * each overwritten by later passes, so that after a one-column call the record holds
+ * the values of the pass whose results the column kept last. */
+#ifdef __cplusplus
+#define ORC_TLS thread_local   /* orc_generic.cpp compiles this file as C++ */
+#else
+#define ORC_TLS _Thread_local
+#endif
+static ORC_TLS double *g_prb = NULL;
+void orc_set_probe(double *buf) { g_prb = buf; }
+int orc_probe_layout(int *nscal, int *kmax, int *nfield)
+{
+    *nscal = ORC_P_NSCAL;
+    *kmax = ORC_PRB_KMAX;
+    *nfield = ORC_PL_NFIELD;
+    return ORC_P_NSCAL + ORC_PRB_KMAX * ORC_PL_NFIELD;
+}
+#define PRB(i) g_prb[(i)]
+#define PRBL(f, k) g_prb[ORC_P_NSCAL + (size_t)(f) * ORC_PRB_KMAX + (size_t)(k)]
+#define PROBE(stmt) do { if (g_prb) { stmt; } } while (0)
 
 /* ------------------------------------------------------------------------- */
 /* Constants                                                                  */
@@ -315,8 +339,13 @@ double orc_compute_alpha(double del_CIN, double ke)
     return x0;
 }
 
-/* uwshcu.F90:4952-4979; erfc is libm's (the CAM error_function module is not
- * part of the reference tree). */
+double orc_xexp(double x) { return XEXP(x); }
+double orc_xlog(double x) { return XLOG(x); }
+double orc_xpow(double x, double y) { return XPOW(x, y); }
+double orc_xerfc(double x) { return XERFC(x); }
+
+/* uwshcu.F90:4952-4979; erfc is Cody's (orc_math.h), the algorithm of the CAM
+ * error_function module, which is not part of the reference tree. */
 double orc_compute_mumin2(double mulcl, double rmaxfrac, double mulow)
 {
     double x0 = mulow, x1, ex, ef, exf, f, fs;
@@ -919,6 +948,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
             mu = wcrit / sigmaw / 1.4142;
             if (mu >= 3.0) { exit_code = EX_MU3; id_exit = 1; goto L333; }
         }
+        PROBE(PRB(ORC_P_MUMIN2_APPLIED) = 0);
         cbmf = (rho0inv * sigmaw / 2.5066) * XEXP(-(mu * mu));
         {
             double cbmflimit = 0.9 * dp0[kinv - 1] / g / dt;
@@ -936,14 +966,20 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
             if (mu > mumin2) { mumin2_fail = 1; }
             mu = dmax(mu, mumin2);
             if (mu == mumin2) limit_ufrc = 1;
+            PROBE(PRB(ORC_P_MUMIN2_APPLIED) = (mu == mumin2) ? 1 : 0);
         }
         if (mu == mumin1) limit_ufrc = 1;
         cbmf = (rho0inv * sigmaw / 2.5066) * XEXP(-(mu * mu));
         winv = sigmaw * (2.0 / 2.5066) * XEXP(-(mu * mu)) / XERFC(mu);
         wtw = winv * winv - 2.0 * cinlcl * rbuoy;
+        PROBE(PRB(ORC_P_CLOSURE) = 1; PRB(ORC_P_WCRIT) = wcrit; PRB(ORC_P_SIGMAW) = sigmaw; PRB(ORC_P_MU) = mu;
+              PRB(ORC_P_RHO0INV) = rho0inv; PRB(ORC_P_CBMF) = cbmf; PRB(ORC_P_WINV) = winv;
+              PRB(ORC_P_CINLCL) = cinlcl; PRB(ORC_P_RBUOY) = rbuoy; PRB(ORC_P_RMAXFRAC) = rmaxfrac;
+              PRB(ORC_P_CBMFLIMIT) = 0.9 * dp0[kinv - 1] / g / dt; PRB(ORC_P_WLCL) = 0; PRB(ORC_P_UFRCLCL) = 0);
         if (wtw <= 0.0) { exit_code = EX_WTW; id_exit = 1; goto L333; }
         wlcl = sqrt(wtw);
         ufrclcl = cbmf / wlcl / rho0inv;
+        PROBE(PRB(ORC_P_WLCL) = wlcl; PRB(ORC_P_UFRCLCL) = ufrclcl);
         wrel = wlcl;
         if (ufrclcl <= 0.0001) { exit_code = EX_UFRC; id_exit = 1; goto L333; }
         ufrc[krel - 1] = ufrclcl;
@@ -980,6 +1016,9 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
         scaleh = tscaleh;
         if (tscaleh < 0.0) scaleh = 1000.0;
         for (iter_scaleh = 1; iter_scaleh <= 3; ++iter_scaleh) {
+            PROBE(for (int f_ = 0; f_ < ORC_PL_NFIELD; ++f_) for (int k_ = 0; k_ < ORC_PRB_KMAX; ++k_) PRBL(f_, k_) = 0.0;
+                  PRB(ORC_P_PPEN_SET) = 0; PRB(ORC_P_KRELP) = krel; PRB(ORC_P_PGFC) = PGFc; PRB(ORC_P_G) = g;
+                  PRB(ORC_P_CRIQC) = criqc);
             kbup = krel;
             kpen = krel;
             wtw = wlcl * wlcl;
@@ -1024,6 +1063,7 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                     orc_qsat(c, qsat_arg, pe, &es, &qs, &gam);
                     excessu = qtue - qs;
                     cridis = rle * scaleh;
+                    double xsat_p = 0.0, thvxsat_p = 0.0;   /* probe copies */
                     if ((excessu <= 0.0 && excess0 <= 0.0) || (excessu >= 0.0 && excess0 >= 0.0)) {
                         xc = dmin(1.0, dmax(0.0, 1.0 - 2.0 * rbuoy * g * cridis / (wue * wue) * (1.0 - thvj / thv0j)));
                     } else {
@@ -1033,6 +1073,8 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                         orc_conden(c, pe, thlxsat, qtxsat, &thj, &qvj, &qlj, &qij, &qse, &id_check);
                         if (id_check == 1) { exit_code = EX_CONDEN; id_exit = 1; goto L333; }
                         thvxsat = THV(thj, qvj, qlj, qij);
+                        xsat_p = xsat;
+                        thvxsat_p = thvxsat;
                         for (kk = 1; kk <= 2; ++kk) {
                             if (kk == 1) {
                                 thv_x0 = thvj;
@@ -1063,6 +1105,12 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                         if (x_cu == xsat) xc = dmax(x_cu, x_en);
                         else xc = x_cu;
                     }
+                    PROBE(int sat_ = !((excessu <= 0.0 && excess0 <= 0.0) || (excessu >= 0.0 && excess0 >= 0.0));
+                          PRBL(ORC_PL_CASE, k) = sat_ ? 2 : 1; PRBL(ORC_PL_XC, k) = xc; PRBL(ORC_PL_WUE, k) = wue;
+                          PRBL(ORC_PL_CRIDIS, k) = cridis; PRBL(ORC_PL_THVJ, k) = thvj; PRBL(ORC_PL_THV0J, k) = thv0j;
+                          PRBL(ORC_PL_XSAT, k) = xsat_p; PRBL(ORC_PL_THVXSAT, k) = thvxsat_p;
+                          PRBL(ORC_PL_THLUE, k) = thlue; PRBL(ORC_PL_QTUE, k) = qtue; PRBL(ORC_PL_EXCESSU, k) = excessu;
+                          PRBL(ORC_PL_EXCESS0, k) = excess0);
                     ee2 = xc * xc;
                     ud2 = 1.0 - 2.0 * xc + xc * xc;
                     rei[k] = (0.5 * rkm / z0[k] / g / rho0j);
@@ -1104,6 +1152,16 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                                            (tre[m] + W->sstr0[m][k] * dpe / 2.0 - W->tru[m][km1] +
                                             W->sstr0[m][k] / fer[k]) * ef;
                     }
+                    PROBE(PRBL(ORC_PL_SET, k) = 1; PRBL(ORC_PL_THLE, k) = thle; PRBL(ORC_PL_QTE, k) = qte;
+                          PRBL(ORC_PL_UE, k) = ue; PRBL(ORC_PL_VE, k) = ve; PRBL(ORC_PL_PE, k) = pe;
+                          PRBL(ORC_PL_DPE, k) = dpe; PRBL(ORC_PL_FER, k) = fer[k]; PRBL(ORC_PL_FDR, k) = fdr[k];
+                          PRBL(ORC_PL_THLU0, k) = thlu[km1]; PRBL(ORC_PL_QTU0, k) = qtu[km1];
+                          PRBL(ORC_PL_UU0, k) = uu[km1]; PRBL(ORC_PL_VU0, k) = vu[km1];
+                          PRBL(ORC_PL_THLU1, k) = thlu[k]; PRBL(ORC_PL_QTU1, k) = qtu[k];
+                          PRBL(ORC_PL_UU1, k) = uu[k]; PRBL(ORC_PL_VU1, k) = vu[k];
+                          PRBL(ORC_PL_SSTHL, k) = ssthl0[k]; PRBL(ORC_PL_SSQT, k) = ssqt0[k];
+                          PRBL(ORC_PL_SSU, k) = ssu0[k]; PRBL(ORC_PL_SSV, k) = ssv0[k];
+                          PRBL(ORC_PL_UMF0, k) = umf[km1]; PRBL(ORC_PL_UMF1, k) = umf[k]; PRBL(ORC_PL_REI, k) = rei[k]);
                     orc_conden(c, ps0[k], thlu[k], qtu[k], &thj, &qvj, &qlj, &qij, &qse, &id_check);
                     if (id_check == 1) { exit_code = EX_CONDEN; id_exit = 1; goto L333; }
                     if ((qlj + qij) > criqc) {
@@ -1131,6 +1189,9 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                                                  (rho0j * drage);
                     else
                         wtw = wtw + dpe * (bogbot + bogtop) / rho0j;
+                    PROBE(PRBL(ORC_PL_WTWB, k) = wtwb; PRBL(ORC_PL_WTW, k) = wtw; PRBL(ORC_PL_DRAGE, k) = drage;
+                          PRBL(ORC_PL_BOGBOT, k) = bogbot; PRBL(ORC_PL_BOGTOP, k) = bogtop;
+                          PRBL(ORC_PL_RHO0J, k) = rho0j);
                     if (wtw > 0.0) {
                         thlue = 0.5 * (thlu[km1] + thlu[k]);
                         qtue = 0.5 * (qtu[km1] + qtu[k]);
@@ -1185,6 +1246,9 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
             } else {
                 ppen = orc_compute_ppen(wtwb, drage, bogbot, bogtop, rho0j, dp0[kpen]);
             }
+            PROBE(PRB(ORC_P_PPEN_SET) = 1; PRB(ORC_P_WTWB) = wtwb; PRB(ORC_P_DRAGE) = drage; PRB(ORC_P_BOGBOT) = bogbot;
+                  PRB(ORC_P_BOGTOP) = bogtop; PRB(ORC_P_RHO0J) = rho0j; PRB(ORC_P_DPEN) = dp0[kpen];
+                  PRB(ORC_P_PPEN) = ppen; PRB(ORC_P_KPEN) = kpen);
             if (ppen == -dp0[kpen] || ppen == 0.0) limit_ppen = 1;
             if (fer[kpen] * (-ppen) < 1.e-4) {
                 thlu_top = thlu[kpen - 1] +
@@ -1263,6 +1327,17 @@ static int column(const orc_consts *c, colws *W, int mkx, int ntr, double dt, co
                 }
             }
         }
+
+        PROBE(PRB(ORC_P_KBUP) = kbup; PRB(ORC_P_KPEN) = kpen; PRB(ORC_P_RPEN) = rpen; PRB(ORC_P_EMF_SET) = 1;
+              PRB(ORC_P_LIMIT_EMF) = limit_emf;
+              for (int k_ = 0; k_ <= mkx && k_ < ORC_PRB_KMAX; ++k_) {
+                  PRBL(ORC_PL_EMF, k_) = emf[k_]; PRBL(ORC_PL_THLU_EMF, k_) = thlu_emf[k_];
+                  PRBL(ORC_PL_QTU_EMF, k_) = qtu_emf[k_]; PRBL(ORC_PL_THL0, k_) = k_ >= 1 ? thl0[k_] : 0.0;
+                  PRBL(ORC_PL_QT0, k_) = k_ >= 1 ? qt0[k_] : 0.0; PRBL(ORC_PL_PS0, k_) = ps0[k_];
+                  PRBL(ORC_PL_P0, k_) = k_ >= 1 ? p0[k_] : 0.0; PRBL(ORC_PL_SSTHL0, k_) = k_ >= 1 ? ssthl0[k_] : 0.0;
+                  PRBL(ORC_PL_SSQT0, k_) = k_ >= 1 ? ssqt0[k_] : 0.0; PRBL(ORC_PL_UMF, k_) = umf[k_];
+                  PRBL(ORC_PL_REIF, k_) = rei[k_]; PRBL(ORC_PL_DP0, k_) = k_ >= 1 ? dp0[k_] : 0.0;
+              });
 
         /* ---- S14 fluxes (uwshcu.F90:3255-3401) ---- */
         xsrc = qtsrc;

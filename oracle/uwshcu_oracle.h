@@ -40,7 +40,12 @@ typedef struct orc_consts {
 /* Fill CESM default constants (shr_const values) and build the Goff-Gratch table. */
 void orc_default_consts(orc_consts *c);
 
-/* Primitives, exposed for unit tests. */
+/* Primitives, exposed for unit tests.  orc_xexp/xlog/xpow/xerfc are the flavour's
+ * transcendental functions (host libm or portable; erfc is Cody's in both). */
+double orc_xexp(double x);
+double orc_xlog(double x);
+double orc_xpow(double x, double y);
+double orc_xerfc(double x);
 void   orc_gffgch(const orc_consts *c, double t, double *es, int itype);
 void   orc_gestbl(orc_consts *c, double tmn, double tmx, double trice, int ip);
 double orc_estblf(const orc_consts *c, double td);
@@ -66,6 +71,42 @@ void   orc_getbuoy(const orc_consts *c, double pbot, double thv0bot, double ptop
  * 2 converged, 4 too cold); *enerr = relative enthalpy error for converged points. */
 int    orc_findsp_point(const orc_consts *c, double q, double t, double p,
                         double *tsp, double *qsp, double *enerr);
+
+/* Stage probe (tests only).  orc_set_probe(buf) makes the column code of the calling
+ * thread record internal quantities into buf (NULL stops it); meaningful for calls of
+ * one column.  buf holds ORC_P_NSCAL scalars, then ORC_PL_NFIELD per-level fields of
+ * ORC_PRB_KMAX entries each (field f, UW level k at ORC_P_NSCAL + f*ORC_PRB_KMAX + k).
+ * Values are those of the last pass that wrote them; per-level fields are cleared at
+ * the start of each cloud-top (iter_scaleh) pass.  UW levels: interfaces 0..mkx,
+ * layers 1..mkx. */
+#define ORC_PRB_KMAX 128
+enum {
+    /* cloud-base closure (uwshcu.F90:2092-2184) */
+    ORC_P_CLOSURE, ORC_P_WCRIT, ORC_P_SIGMAW, ORC_P_MU, ORC_P_RHO0INV, ORC_P_CBMF, ORC_P_WINV, ORC_P_WLCL,
+    ORC_P_UFRCLCL, ORC_P_CINLCL, ORC_P_RBUOY, ORC_P_RMAXFRAC, ORC_P_CBMFLIMIT, ORC_P_MUMIN2_APPLIED,
+    /* cloud-top penetration (uwshcu.F90:2873-2899) */
+    ORC_P_PPEN_SET, ORC_P_WTWB, ORC_P_DRAGE, ORC_P_BOGBOT, ORC_P_BOGTOP, ORC_P_RHO0J, ORC_P_DPEN, ORC_P_PPEN,
+    /* levels and constants */
+    ORC_P_KRELP, ORC_P_KBUP, ORC_P_KPEN, ORC_P_RPEN, ORC_P_PGFC, ORC_P_G, ORC_P_CRIQC,
+    /* penetrative entrainment (uwshcu.F90:3055-3183) */
+    ORC_P_EMF_SET, ORC_P_LIMIT_EMF,
+    ORC_P_NSCAL
+};
+enum {
+    /* buoyancy sorting, per layer k of the final pass (uwshcu.F90:2306-2952) */
+    ORC_PL_SET, ORC_PL_CASE, ORC_PL_XC, ORC_PL_WUE, ORC_PL_CRIDIS, ORC_PL_THVJ, ORC_PL_THV0J, ORC_PL_XSAT,
+    ORC_PL_THVXSAT, ORC_PL_THLUE, ORC_PL_QTUE, ORC_PL_EXCESSU, ORC_PL_EXCESS0,
+    ORC_PL_THLE, ORC_PL_QTE, ORC_PL_UE, ORC_PL_VE, ORC_PL_PE, ORC_PL_DPE, ORC_PL_FER, ORC_PL_FDR,
+    ORC_PL_THLU0, ORC_PL_QTU0, ORC_PL_UU0, ORC_PL_VU0, ORC_PL_THLU1, ORC_PL_QTU1, ORC_PL_UU1, ORC_PL_VU1,
+    ORC_PL_SSTHL, ORC_PL_SSQT, ORC_PL_SSU, ORC_PL_SSV, ORC_PL_UMF0, ORC_PL_UMF1, ORC_PL_REI,
+    ORC_PL_WTWB, ORC_PL_WTW, ORC_PL_DRAGE, ORC_PL_BOGBOT, ORC_PL_BOGTOP, ORC_PL_RHO0J,
+    /* penetrative entrainment per interface k, and the environment per layer k */
+    ORC_PL_EMF, ORC_PL_THLU_EMF, ORC_PL_QTU_EMF, ORC_PL_THL0, ORC_PL_QT0, ORC_PL_PS0, ORC_PL_P0,
+    ORC_PL_SSTHL0, ORC_PL_SSQT0, ORC_PL_UMF, ORC_PL_REIF, ORC_PL_DP0,
+    ORC_PL_NFIELD
+};
+void orc_set_probe(double *buf);
+int  orc_probe_layout(int *nscal, int *kmax, int *nfield);
 
 /* Number of integer diagnostics per column written by orc_compute_uwshcu_inv. */
 #define ORC_NDIAG 12

@@ -252,9 +252,61 @@ def test_compute_alpha_is_lambert_w(k):
     assert a == pytest.approx((sps.lambertw(k).real / k), rel=1e-12)
 
 
-def test_erfc_matches_scipy():
-    for x in (0.0, 0.3, 0.906, 1.7, 2.9):
-        assert math.erfc(x) == pytest.approx(sps.erfc(x), rel=1e-15)
+def _ulps(got, exact):
+    import mpmath
+    return float(abs(mpmath.mpf(got) - exact) / np.spacing(abs(float(exact))))
+
+
+@pytest.mark.parametrize("portable", [False, True])
+def test_erfc_is_codys_rational_approximation(portable):
+    """Both flavours evaluate Cody's CALERF erfc (the algorithm of CESM's
+    error_function): within 6 ulp of a 120-bit erfc wherever the result is a normal
+    number, its three intervals joined at 0.46875 and 4, exactly 0 from Cody's IEEE
+    cut-off 26.543 on, and 2 - erfc(|x|) for negative x beyond the first interval."""
+    import mpmath
+    mpmath.mp.prec = 120
+    Lf = oracle.lib(portable)
+    xs = np.concatenate([np.linspace(-6.0, 26.4, 8001), [0.0, 1e-17, 0.46875, np.nextafter(0.46875, 1.0), 4.0,
+                                                        np.nextafter(4.0, 5.0), -0.46875, 26.0]])
+    worst = max(_ulps(Lf.orc_xerfc(float(x)), mpmath.erfc(mpmath.mpf(float(x)))) for x in xs)
+    assert worst <= 6.0, worst
+    assert Lf.orc_xerfc(0.0) == 1.0
+    assert Lf.orc_xerfc(26.543) == 0.0 and Lf.orc_xerfc(40.0) == 0.0 and Lf.orc_xerfc(26.5) > 0.0
+    for x in (0.5, 1.3, 3.9, 4.5, 12.0):
+        assert Lf.orc_xerfc(-x) == 2.0 - Lf.orc_xerfc(x)
+
+
+@pytest.mark.parametrize("portable", [False, True])
+def test_flavour_exp_and_log_accuracy(portable):
+    import mpmath
+    mpmath.mp.prec = 120
+    Lf = oracle.lib(portable)
+    rng = np.random.default_rng(3)
+    xe = np.concatenate([rng.uniform(-700.0, 700.0, 2000), rng.uniform(-1.0, 1.0, 2000)])
+    assert max(_ulps(Lf.orc_xexp(float(x)), mpmath.exp(mpmath.mpf(float(x)))) for x in xe) <= 1.0
+    xl = np.concatenate([np.exp(rng.uniform(-700.0, 700.0, 2000)), rng.uniform(0.5, 2.0, 2000)])
+    assert max(_ulps(Lf.orc_xlog(float(x)), mpmath.log(mpmath.mpf(float(x)))) for x in xl if x != 1.0) <= 2.0
+
+
+def test_product_math_equals_the_portable_oracle_math_on_the_host():
+    """The kernel's exp/log/pow/erfc (uwshcu_math.h, compiled for the host in the math
+    probe) give the portable oracle's bits; the GPU tests check GPU against host."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(__file__), "hostemu", "libmathprobe.so")
+    if not os.path.exists(path):
+        pytest.skip("libmathprobe.so not built")
+    P = C.CDLL(path)
+    rng = np.random.default_rng(9)
+    x = np.concatenate([np.linspace(-6.0, 27.0, 40001), rng.uniform(-30.0, 30.0, 20000)])
+    y = np.abs(x) + 0.5
+    out = np.zeros((len(x), 10))
+    P.probe_host(len(x), x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+    O = oracle.lib(True)
+    for col, fn, args in ((0, O.orc_xexp, (x,)), (1, O.orc_xlog, (y,)), (2, O.orc_xpow, (y, x)),
+                          (3, O.orc_xerfc, (x,))):
+        want = np.array([fn(*(float(a[i]) for a in args)) for i in range(len(x))])
+        np.testing.assert_array_equal(out[:, col], want)
 
 
 @pytest.mark.parametrize("mulcl,mulow", [(1.2, 0.906), (2.0, 1.0), (0.8, 0.906)])
