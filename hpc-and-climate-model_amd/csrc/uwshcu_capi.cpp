@@ -63,8 +63,17 @@ struct uwshcu_ctx {
     hipEvent_t ctab_event = nullptr;
     bool ctab_pending = false;
     hipStream_t stream = nullptr;
+    // this device's address of the static scratch array (uwshcu_forward_device_noalloc)
+    void* d_scratch = nullptr;
     std::mutex mu;
 };
+
+// Workspace of uwshcu_forward_device_noalloc: a fixed array in the code object's
+// device memory, one copy per GPU the library runs on, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it is a stand-in,
+// not a device address): uwshcu_ctx_create looks up this device's address once with
+// hipGetSymbolAddress and every launch uses that.
+static __device__ __align__(256) unsigned char scratch[UWSHCU_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -102,6 +111,24 @@ double goff_gratch(double t, int itype, double tmelt) {
 }
 
 bool hip_ok(hipError_t e) { return e == hipSuccess; }
+
+// Per-device address of `scratch`, looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) || !p) return UWSHCU_ERR_HIP;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return UWSHCU_OK;
+}
+
 
 int ensure_ws(uwshcu_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->ws_bytes) return UWSHCU_OK;
@@ -159,30 +186,36 @@ struct ZwInputs {
     const double *omega, *tpert, *qpert;
 };
 
+// fixed_ws (nullable): run in this caller-given workspace of fixed_bytes instead of the
+// context's, allocating nothing (FP64 only, one batch, no deferred-column lists).
 int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pcols, int ncol, int pver, int ncnst,
-               double dt, const Chunk& ch, int* diag, hipStream_t stream, const ZwInputs* zw = nullptr) {
+               double dt, const Chunk& ch, int* diag, hipStream_t stream, const ZwInputs* zw = nullptr,
+               void* fixed_ws = nullptr, size_t fixed_bytes = 0) {
     if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
         return UWSHCU_ERR_ARG;
     if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED && precision != UWSHCU_MIXED_UNCHECKED)
         return UWSHCU_ERR_ARG;
     if (zw && (precision != UWSHCU_FP64 || !zw->par || !zw->omega || !zw->tpert || !zw->qpert)) return UWSHCU_ERR_ARG;
-    if (ncol == 0) return UWSHCU_OK;
-    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
+    if (fixed_ws && (precision != UWSHCU_FP64 || zw)) return UWSHCU_ERR_ARG;
     const bool fp32 = precision != UWSHCU_FP64;
     const bool recompute = precision == UWSHCU_MIXED;
     const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, fp32 ? UWSHCU_MIXED : UWSHCU_FP64);
     const size_t per_col64 = uwshcu_workspace_bytes_per_column(pver, ncnst, UWSHCU_FP64);
-    // batches of columns that share one workspace
-    const long long nb = batch_columns(ctx, ncol, per_col);
+    // batches of columns that share one workspace; a fixed workspace holds the whole
+    // chunk (its leading dimension rounded up to a wavefront) or the call is refused
+    const long long nb = fixed_ws ? ((long long)ncol + 63) / 64 * 64 : batch_columns(ctx, ncol, per_col);
+    if (fixed_ws && per_col * (size_t)nb > fixed_bytes) return UWSHCU_ERR_SCRATCH;
+    if (ncol == 0) return UWSHCU_OK;
+    if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
     const long long nb64 = recompute ? batch_columns(ctx, ncol, per_col64) : 0;
     size_t need = per_col * (size_t)nb;
     if (recompute && per_col64 * (size_t)nb64 > need) need = per_col64 * (size_t)nb64;
-    int rc = ensure_ws(ctx, need);
+    int rc = fixed_ws ? UWSHCU_OK : ensure_ws(ctx, need);
     if (rc) return rc;
     if (recompute && (rc = ensure_mixed(ctx, (size_t)ncol))) return rc;
     // defer list (nb entries and a counter) and the deferred columns' scalars, for
     // large launches
-    if (nb >= 2 * 1024 * 64 && (size_t)nb + 1 > ctx->defer_cols) {
+    if (!fixed_ws && nb >= 2 * 1024 * 64 && (size_t)nb + 1 > ctx->defer_cols) {
         if (ctx->d_defer) {
             if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
             (void)hipFree(ctx->d_defer);
@@ -219,14 +252,14 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     a.cbmf = ch.cbmf; a.qc = ch.qc; a.rliq = ch.rliq; a.cnt = ch.cnt; a.cnb = ch.cnb; a.wu = ch.wu;
     a.diag = diag;
     a.err = ctx->d_err;
-    a.ws = ctx->d_ws;
+    a.ws = fixed_ws ? fixed_ws : ctx->d_ws;
     if (zw) {
         a.zw = *zw->par;
         a.omega = zw->omega;
         a.tpert = zw->tpert;
         a.qpert = zw->qpert;
     }
-    if (ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
+    if (!fixed_ws && ctx->d_defer && (size_t)nb + 1 <= ctx->defer_cols) {
         a.defer_list = ctx->d_defer + 2;   // two counts, then the lists and marks (see launch_column_pipeline)
         a.defer_count = ctx->d_defer;
         a.defer_state = ctx->d_defer_state;   // [field][slot], a.wld = nb apart
@@ -547,7 +580,7 @@ int uwshcu_ctx_create(const uwshcu_consts* c, const double* estbl, int device, u
               hip_ok(hipMemcpy(ctx->d_consts, &ctx->consts, sizeof(uwshcu_consts), hipMemcpyHostToDevice)) &&
               hip_ok(hipMemcpy(ctx->d_estbl, ctx->estbl, sizeof(ctx->estbl), hipMemcpyHostToDevice)) &&
               hip_ok(uwshcu::launch_clear_status(ctx->d_err, nullptr)) && hip_ok(hipStreamCreate(&ctx->stream)) &&
-              hip_ok(hipDeviceSynchronize());
+              scratch_address(device, &ctx->d_scratch) == UWSHCU_OK && hip_ok(hipDeviceSynchronize());
     if (!ok) {
         uwshcu_ctx_destroy(ctx);
         return UWSHCU_ERR_HIP;
@@ -641,6 +674,35 @@ int uwshcu_forward_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* p
              umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
              qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
     return run_device(ctx, precision, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream);
+}
+
+int uwshcu_forward_device_noalloc(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, int pver,
+                                  int ncnst, double dt, const double* ps0, const double* zs0, const double* p0,
+                                  const double* z0, const double* dp0, const double* u0, const double* v0,
+                                  const double* qv0, const double* ql0, const double* qi0, const double* t0,
+                                  const double* s0, const double* tr0, const double* tke, const double* cldfrc,
+                                  const double* concldfrc, const double* pblh, double* cush, double* umf,
+                                  double* slflx, double* qtflx, double* flxprc1, double* flxsnow1, double* qvten,
+                                  double* qlten, double* qiten, double* sten, double* uten, double* vten,
+                                  double* trten, double* qrten, double* qsten, double* precip, double* snow,
+                                  double* evapc, double* cufrc, double* qcu, double* qlu, double* qiu, double* cbmf,
+                                  double* qc, double* rliq, double* cnt, double* cnb, const double* dpdry0,
+                                  double* wu, int* diag, void* stream) {
+    (void)cldfrc;
+    (void)concldfrc;
+    if (!ctx || !ctx->d_scratch) return UWSHCU_ERR_ARG;
+    const void* req[] = {ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tke, pblh, cush, umf, slflx,
+                         qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, qrten, qsten, precip,
+                         snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, dpdry0, wu};
+    for (const void* p : req)
+        if (!p) return UWSHCU_ERR_ARG;
+    if (ncnst > 0 && (!tr0 || !trten)) return UWSHCU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Chunk ch{ps0, zs0, p0, z0, dp0, u0, v0, qv0, ql0, qi0, t0, s0, tr0, tke, pblh, dpdry0, cush,
+             umf, slflx, qtflx, flxprc1, flxsnow1, qvten, qlten, qiten, sten, uten, vten, trten,
+             qrten, qsten, precip, snow, evapc, cufrc, qcu, qlu, qiu, cbmf, qc, rliq, cnt, cnb, wu};
+    return run_device(ctx, UWSHCU_FP64, par, pcols, ncol, pver, ncnst, dt, ch, diag, (hipStream_t)stream, nullptr,
+                      ctx->d_scratch, UWSHCU_NOALLOC_SCRATCH_BYTES);
 }
 
 int uwshcu_zw_forward_device(uwshcu_ctx* ctx, const uwshcu_params* par, const uwshcu_zw_params* zw, int pcols,

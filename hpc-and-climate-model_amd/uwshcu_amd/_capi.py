@@ -29,7 +29,10 @@ ERRORS = {
     -12: "HIP runtime error",
     -13: "no usable GPU",
     -14: "adjoint tape capacity exceeded",
+    -15: "chunk needs more than the static scratch of the allocation-free entry point",
 }
+
+NOALLOC_SCRATCH_BYTES = 256 << 20   # UWSHCU_NOALLOC_SCRATCH_BYTES
 
 
 class Consts(C.Structure):
@@ -109,7 +112,7 @@ SYMBOLS = ("uwshcu_default_consts", "uwshcu_build_estbl", "uwshcu_default_params
            "uwshcu_set_dropin_params", "uwshcu_dropin_last_status", "uwshcu_ctx_set_mixed_tolerance",
            "uwshcu_mixed_recomputed", "uwshcu_adjoint_device", "uwshcu_ctx_set_tape_capacity",
            "uwshcu_forward_chunks_device", "uwshcu_merge_shallow_device", "uwshcu_zw_default_params",
-           "uwshcu_zw_forward_device", "uwshcu_zw_forward_host")
+           "uwshcu_zw_forward_device", "uwshcu_zw_forward_host", "uwshcu_forward_device_noalloc")
 
 _lib = None
 
@@ -146,6 +149,8 @@ def lib():
     L.uwshcu_workspace_bytes_per_column.restype = C.c_size_t
     head = [vp, C.c_int, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp]
     L.uwshcu_forward_device.argtypes = head + [vp] * len(ARRAY_ARGS) + [vp, vp]
+    L.uwshcu_forward_device_noalloc.argtypes = [vp, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_int, dp] + \
+        [vp] * len(ARRAY_ARGS) + [vp, vp]
     L.uwshcu_zw_default_params.argtypes = [C.POINTER(Params), C.POINTER(ZwParams)]
     L.uwshcu_zw_forward_device.argtypes = [vp, C.POINTER(Params), C.POINTER(ZwParams), C.c_int, C.c_int, C.c_int,
                                            C.c_int, dp] + [vp] * len(ZW_ARRAY_ARGS) + [vp, vp]

@@ -99,6 +99,35 @@ int uwshcu_forward_device(uwshcu_ctx *ctx, int precision, const uwshcu_params *p
     double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
 
+/* Size of the static device scratch of uwshcu_forward_device_noalloc (one copy per GPU). */
+#define UWSHCU_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* uwshcu_forward_device(ctx, UWSHCU_FP64, ...) for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No counterpart in the reference.  The call allocates nothing: its
+ * workspace is a static device array of UWSHCU_NOALLOC_SCRATCH_BYTES (256 MiB) per GPU,
+ * whose address uwshcu_ctx_create looks up.  The chunk needs
+ * uwshcu_workspace_bytes_per_column(pver, ncnst, UWSHCU_FP64) bytes for each of its ncol
+ * columns rounded up to a multiple of 64 (at 35 levels and 3 constituents up to 14 528
+ * columns, at 72 levels and 10 constituents up to 4 992); a larger chunk returns
+ * UWSHCU_ERR_SCRATCH and launches nothing.  Results equal uwshcu_forward_device's bit
+ * for bit.  Every call on one GPU shares that GPU's scratch, whichever context it comes
+ * through: calls whose work overlaps on one GPU (different streams, contexts or host
+ * threads without ordering between them) race and give wrong results.  Calls in one
+ * stream are ordered and safe.  The scratch keeps the last call's contents; no output
+ * depends on them. */
+int uwshcu_forward_device_noalloc(uwshcu_ctx *ctx, const uwshcu_params *par,
+    int pcols, int ncol, int pver, int ncnst, double dt,
+    const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
+    const double *u0, const double *v0, const double *qv0, const double *ql0, const double *qi0,
+    const double *t0, const double *s0, const double *tr0, const double *tke, const double *cldfrc,
+    const double *concldfrc, const double *pblh, double *cush,
+    double *umf, double *slflx, double *qtflx, double *flxprc1, double *flxsnow1,
+    double *qvten, double *qlten, double *qiten, double *sten, double *uten, double *vten, double *trten,
+    double *qrten, double *qsten, double *precip, double *snow, double *evapc,
+    double *cufrc, double *qcu, double *qlu, double *qiu, double *cbmf, double *qc, double *rliq,
+    double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag, void *stream);
+
 /* The modified scheme compute_zwshcu_inv (zwshcu.F90:336-628; caller
  * convect_shallow.F90:717-733) on device arrays, FP64: the arguments of
  * uwshcu_forward_device without the precision, plus the options `zw`

@@ -125,7 +125,8 @@ enum {
     UWSHCU_ERR_NOMEM          = -11,  // workspace allocation failed
     UWSHCU_ERR_HIP            = -12,  // a HIP runtime call failed
     UWSHCU_ERR_NO_DEVICE      = -13,  // no usable GPU
-    UWSHCU_ERR_TAPE           = -14   // adjoint: a column's tape exceeded its capacity
+    UWSHCU_ERR_TAPE           = -14,  // adjoint: a column's tape exceeded its capacity
+    UWSHCU_ERR_SCRATCH        = -15   // allocation-free entry point: the chunk needs more than its static scratch
 };
 
 #ifdef __cplusplus
