@@ -129,7 +129,6 @@ int scratch_address(int device, void** out) {
     return UWSHCU_OK;
 }
 
-
 int ensure_ws(uwshcu_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->ws_bytes) return UWSHCU_OK;
     if (ctx->d_ws) {
@@ -193,13 +192,15 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
                void* fixed_ws = nullptr, size_t fixed_bytes = 0) {
     if (!ctx || pcols < 1 || ncol < 0 || ncol > pcols || pver < 3 || ncnst < 0 || ncnst > UWSHCU_NCNST_MAX)
         return UWSHCU_ERR_ARG;
-    if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED && precision != UWSHCU_MIXED_UNCHECKED)
+    if (precision != UWSHCU_FP64 && precision != UWSHCU_MIXED && precision != UWSHCU_MIXED_UNCHECKED &&
+        precision != UWSHCU_MIXED_FAST_UNCHECKED)
         return UWSHCU_ERR_ARG;
     if (zw && (precision != UWSHCU_FP64 || !zw->par || !zw->omega || !zw->tpert || !zw->qpert)) return UWSHCU_ERR_ARG;
     if (fixed_ws && (precision != UWSHCU_FP64 || zw)) return UWSHCU_ERR_ARG;
-    const bool fp32 = precision != UWSHCU_FP64;
+    const bool fp32 = precision == UWSHCU_MIXED_UNCHECKED;   // the FP32-plume pass
+    const bool fast = precision == UWSHCU_MIXED || precision == UWSHCU_MIXED_FAST_UNCHECKED;   // fast FP64 pass
     const bool recompute = precision == UWSHCU_MIXED;
-    const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, fp32 ? UWSHCU_MIXED : UWSHCU_FP64);
+    const size_t per_col = uwshcu_workspace_bytes_per_column(pver, ncnst, precision);
     const size_t per_col64 = uwshcu_workspace_bytes_per_column(pver, ncnst, UWSHCU_FP64);
     // batches of columns that share one workspace; a fixed workspace holds the whole
     // chunk (its leading dimension rounded up to a wavefront) or the call is refused
@@ -275,8 +276,9 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     for (long long c0 = 0; c0 < ncol; c0 += nb) {
         const int n = (int)((ncol - c0) < nb ? (ncol - c0) : nb);
         a.col0 = (int)c0;
-        hipError_t e = zw ? uwshcu::launch_forward_zw(a, n, ctx->d_consts, ctx->d_estbl, stream)
+        hipError_t e = zw     ? uwshcu::launch_forward_zw(a, n, ctx->d_consts, ctx->d_estbl, stream)
                        : fp32 ? uwshcu::launch_forward_mixed(a, n, ctx->d_consts, ctx->d_estbl, stream)
+                       : fast ? uwshcu::launch_forward_fast(a, n, ctx->d_consts, ctx->d_estbl, stream)
                               : uwshcu::launch_forward_fp64(a, n, ctx->d_consts, ctx->d_estbl, stream);
         if (!hip_ok(e)) return UWSHCU_ERR_HIP;
     }
@@ -460,6 +462,7 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
 std::mutex g_dropin_mu;
 uwshcu_ctx* g_dropin_ctx = nullptr;
 uwshcu_params g_dropin_par{10.0, 0.1, 14.0};
+bool g_dropin_par_set = false;   // rpen has no default: init_uwshcu stops the model if it is unset
 int g_dropin_status = 0, g_dropin_badcol = -1;
 
 }  // namespace
@@ -557,7 +560,7 @@ int uwshcu_zw_default_params(uwshcu_params* p, uwshcu_zw_params* z) {
 }
 
 size_t uwshcu_workspace_bytes_per_column(int pver, int ncnst, int precision) {
-    const size_t elem = (precision == UWSHCU_FP64) ? sizeof(double) : sizeof(float);
+    const size_t elem = (precision == UWSHCU_MIXED_UNCHECKED) ? sizeof(float) : sizeof(double);
     return uwshcu::ws_bytes_per_column(pver, ncnst, elem);
 }
 
@@ -1105,6 +1108,14 @@ void bxcompute_uwshcu_inv_(int* pcols, int* pver, int* ncol, int* pcnst, double*
                            double* rliq, double* cnt_inv, double* cnb_inv, int* lchnk, double* dpdry0_inv,
                            double* wu_inv) {
     std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (!g_dropin_par_set) {
+        // uwshcu.F90:241-242: the reference stops the model when uwshcu_rpen is unset
+        std::fprintf(stderr, "bxcompute_uwshcu_inv_: uwshcu_rpen is not set; call uwshcu_set_dropin_params first "
+                             "(chunk %d not computed)\n", lchnk ? *lchnk : -1);
+        g_dropin_status = UWSHCU_ERR_PARAMS;
+        g_dropin_badcol = -1;
+        return;
+    }
     if (!g_dropin_ctx) {
         int rc = uwshcu_ctx_create(nullptr, nullptr, -1, &g_dropin_ctx);
         if (rc) {
@@ -1129,9 +1140,14 @@ void bxcompute_uwshcu_inv_(int* pcols, int* pver, int* ncol, int* pcnst, double*
 }
 
 int uwshcu_set_dropin_params(const uwshcu_params* par) {
-    if (!par) return UWSHCU_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (!par) {
+        g_dropin_par_set = false;
+        return UWSHCU_OK;
+    }
+    if (!(par->rpen == par->rpen) || !(par->rle == par->rle) || !(par->rkm == par->rkm)) return UWSHCU_ERR_ARG;
     g_dropin_par = *par;
+    g_dropin_par_set = true;
     return UWSHCU_OK;
 }
 

@@ -19,37 +19,48 @@ namespace uwshcu {
 
 UWSHCU_HD inline bool finite_value(double x) { return x - x == 0.0; }   // false for NaN and +-Inf
 
-UWSHCU_HD inline bool column_fails_check(const FwdArgs& a, const uwshcu_consts& C, int i, double tol) {
+// Budget residuals of column i: r[0] water (kg/m2/s), r[1] energy (W/m2), r[2] the
+// largest |qtflx|, r[3] the largest |slflx|; returns false if an output is not finite.
+UWSHCU_HD inline bool column_budget(const FwdArgs& a, const uwshcu_consts& C, int i, double* r) {
     const int mkx = a.mkx;
     const size_t ldc = (size_t)a.ldc;
     auto at = [&](const double* p, int row) { return p[(size_t)row * ldc + i]; };
     const double dt = a.dt;
     double w = 0.0, te = 0.0, acc = 0.0, fq = 0.0, fs = 0.0;
-    for (int r = 0; r < mkx; ++r) {
-        const double dpg = at(a.dp0, r) / C.g;
-        const double qv = at(a.qvten, r), ql = at(a.qlten, r), qi = at(a.qiten, r);
-        const double u0 = at(a.u0, r), v0 = at(a.v0, r);
-        const double uf = u0 + at(a.uten, r) * dt, vf = v0 + at(a.vten, r) * dt;
+    for (int r0 = 0; r0 < mkx; ++r0) {
+        const double dpg = at(a.dp0, r0) / C.g;
+        const double qv = at(a.qvten, r0), ql = at(a.qlten, r0), qi = at(a.qiten, r0);
+        const double u0 = at(a.u0, r0), v0 = at(a.v0, r0);
+        const double uf = u0 + at(a.uten, r0) * dt, vf = v0 + at(a.vten, r0) * dt;
         // (uf-u0)(uf+u0): exactly zero where the wind tendency is, whether or not the
         // compiler fuses the products
         const double dke = 0.5 * ((uf - u0) * (uf + u0) + (vf - v0) * (vf + v0)) / dt;
         w += (qv + ql + qi) * dpg;
-        te += (at(a.sten, r) + dke + (C.xlv + C.xlf) * qv + C.xlf * ql) * dpg;
-        acc += fabs(at(a.qrten, r)) + fabs(at(a.qsten, r)) + fabs(at(a.evapc, r)) + fabs(at(a.cufrc, r)) +
-               fabs(at(a.qcu, r)) + fabs(at(a.qlu, r)) + fabs(at(a.qiu, r)) + fabs(at(a.qc, r));
-        for (int m = 0; m < a.ncnst; ++m) acc += fabs(a.trten[((size_t)m * mkx + r) * ldc + i]);
+        te += (at(a.sten, r0) + dke + (C.xlv + C.xlf) * qv + C.xlf * ql) * dpg;
+        acc += fabs(at(a.qrten, r0)) + fabs(at(a.qsten, r0)) + fabs(at(a.evapc, r0)) + fabs(at(a.cufrc, r0)) +
+               fabs(at(a.qcu, r0)) + fabs(at(a.qlu, r0)) + fabs(at(a.qiu, r0)) + fabs(at(a.qc, r0));
+        for (int m = 0; m < a.ncnst; ++m) acc += fabs(a.trten[((size_t)m * mkx + r0) * ldc + i]);
     }
-    for (int r = 0; r <= mkx; ++r) {
-        fq = fmax(fq, fabs(at(a.qtflx, r)));
-        fs = fmax(fs, fabs(at(a.slflx, r)));
-        acc += fabs(at(a.umf, r)) + fabs(at(a.wu, r)) + fabs(at(a.flxprc1, r)) + fabs(at(a.flxsnow1, r));
+    for (int r0 = 0; r0 <= mkx; ++r0) {
+        fq = fmax(fq, fabs(at(a.qtflx, r0)));
+        fs = fmax(fs, fabs(at(a.slflx, r0)));
+        acc += fabs(at(a.umf, r0)) + fabs(at(a.wu, r0)) + fabs(at(a.flxprc1, r0)) + fabs(at(a.flxsnow1, r0));
     }
     const double precip = a.precip[i], rliq = a.rliq[i], snow = a.snow[i];
     w += (precip + rliq) * 1000.0;
     const double eres = te + (precip + rliq - snow) * 1000.0 * C.xlf;
     acc += fabs(a.cbmf[i]) + fabs(a.cush[i]) + fs + fq;
-    if (!(finite_value(w) && finite_value(eres) && finite_value(acc))) return true;
-    return (fq > 0.0 && fabs(w) > tol * fq) || (fs > 0.0 && fabs(eres) > tol * fs);
+    r[0] = w;
+    r[1] = eres;
+    r[2] = fq;
+    r[3] = fs;
+    return finite_value(w) && finite_value(eres) && finite_value(acc);
+}
+
+UWSHCU_HD inline bool column_fails_check(const FwdArgs& a, const uwshcu_consts& C, int i, double tol) {
+    double r[4];
+    if (!column_budget(a, C, i, r)) return true;
+    return (r[2] > 0.0 && fabs(r[0]) > tol * r[2]) || (r[3] > 0.0 && fabs(r[1]) > tol * r[3]);
 }
 
 }  // namespace uwshcu

@@ -158,13 +158,14 @@ enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4,
 // R: bulk (plume) arithmetic type; D: the FP64 type (double, or AD in the adjoint).
 // ZW: the modified scheme compute_zwshcu (zwshcu.F90:630-4969), FP64 only; its
 // differences from compute_uwshcu are marked where they apply.
-template <class R, class D = double, bool ZW = false>
+// FAST: the fast path's saturation adjustment (ThermoT<D, true>), FP64 only.
+template <class R, class D = double, bool ZW = false, bool FAST = false>
 struct Column {
     static_assert(!ZW || (std::is_same<R, double>::value && std::is_same<D, double>::value),
                   "the modified scheme runs in FP64");
     static constexpr bool kAD = is_ad<D>::value;
     const FwdArgs& a;
-    const ThermoT<D>& T;
+    const ThermoT<D, FAST>& T;
     const uwshcu_consts& C;
     const int i, iw, mkx, KS, ncnst;
     const size_t ldc, wld;
@@ -172,7 +173,7 @@ struct Column {
     R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
     TapeCtx* tape;              // adjoint kernel: this column's tape
 
-    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D>& th, int col, int slot, TapeCtx* own_tape = nullptr)
+    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D, FAST>& th, int col, int slot, TapeCtx* own_tape = nullptr)
         : a(args), T(th), C(*th.c), i(col), iw(slot), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
           ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<D*>(args.ws)),
           ws(reinterpret_cast<R*>(reinterpret_cast<D*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
@@ -483,9 +484,9 @@ struct Column {
     }
 };
 
-template <class R, class D, bool ZW>
+template <class R, class D, bool ZW, bool FAST>
 template <bool kSplit>
-UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const {
+UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body() const {
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
@@ -497,15 +498,18 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW>::run_body() const
     const bool has_nl = ixl >= 1 && ixl <= ncnst, has_ni = ixi >= 1 && ixi <= ncnst;
     auto trz = [&](int m, int k) -> R { return (m >= 1 && m <= ncnst) ? R(trin(m, k)) : R(0); };
 
-    // FP32 branch guard: a decision the reference takes on a quantity computed here
-    // in FP32 is flagged when the quantity lies within a relative 1e-5 of the branch
-    // point, well above the FP32 rounding it carries.  Flagged columns are recomputed
-    // in FP64 by the mixed-precision driver.  No effect in FP64.
+    // Branch guard: a decision the reference takes on a quantity computed here in FP32,
+    // or through the fast path's converged saturation adjustment, is flagged when the
+    // quantity lies within a relative 1e-5 (FP32) or 1e-8 (fast path, whose results
+    // differ from the reference's by about 1e-10) of the branch point, well above the
+    // perturbation it carries.  Flagged columns are recomputed in FP64 by the
+    // mixed-precision driver.  No effect on the FP64 path.
     constexpr bool kFp32 = sizeof(R) == sizeof(float);
+    constexpr double kGuard = kFp32 ? 1.e-5 : 1.e-8;
     int amb = 0;
     auto near = [&](auto dist, auto scale) {
-        if constexpr (kFp32) {
-            if (fabs(dist) <= 1.e-5 * fabs(scale)) amb = 1;
+        if constexpr (kFp32 || FAST) {
+            if (fabs(dist) <= kGuard * fabs(scale)) amb = 1;
         }
     };
 

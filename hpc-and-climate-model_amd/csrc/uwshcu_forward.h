@@ -9,7 +9,7 @@ namespace uwshcu {
 // One thread per column; the saturation table is staged into LDS once per workgroup.
 // Thread j of the launch works in workspace slot j on column col0 + j, or, with a
 // column map, on column colmap[col0 + j] while col0 + j < *colcount.
-template <class R, bool ZW, bool SPLIT>
+template <class R, bool ZW, bool SPLIT, bool FAST>
 __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const uwshcu_consts* __restrict__ consts,
                                              const double* __restrict__ estbl) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
@@ -25,8 +25,8 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
     if (j >= nvalid) return;
     const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
     const int slot = args.resume ? col - args.slot0 : j;
-    const Thermo T{consts, s_estbl};
-    const Column<R, double, ZW> c(args, T, col, slot);
+    const ThermoT<double, FAST> T{consts, s_estbl};
+    const Column<R, double, ZW, FAST> c(args, T, col, slot);
     c.template run_body<SPLIT>();
 }
 
@@ -35,21 +35,21 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
 // wavefronts than SIMDs, where nothing could run beside it.  forward_kernel_occ2
 // allows half that (some spilling) so that two wavefronts share a SIMD and hide each
 // other's FP64 latency: 1.37x faster at 1 M columns.
-template <class R, bool ZW = false>
+template <class R, bool ZW = false, bool FAST = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1)))
 void forward_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts, const double* __restrict__ estbl) {
-    forward_body<R, ZW, false>(args, n, consts, estbl);   // never the first two launches of a split
+    forward_body<R, ZW, false, FAST>(args, n, consts, estbl);   // never the first two launches of a split
 }
-template <class R, bool ZW = false>
+template <class R, bool ZW = false, bool FAST = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
 void forward_kernel_occ2(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                          const double* __restrict__ estbl) {
-    forward_body<R, ZW, true>(args, n, consts, estbl);
+    forward_body<R, ZW, true, FAST>(args, n, consts, estbl);
 }
 
 // Level-parallel set-up: thread t of the launch handles workspace slot t % n and
 // level t / n (+1 for stage 2), so that a wavefront covers consecutive columns.
-template <class R, int STAGE, bool ZW = false>
+template <class R, int STAGE, bool ZW = false, bool FAST = false>
 __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                                                     const double* __restrict__ estbl) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
@@ -65,8 +65,8 @@ __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const u
     const int k = (int)(t / n) + (STAGE == 2 ? 1 : 0);
     if (k > args.mkx || j >= nvalid) return;
     const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
-    const Thermo T{consts, s_estbl};
-    const Column<R, double, ZW> c(args, T, col, j);
+    const ThermoT<double, FAST> T{consts, s_estbl};
+    const Column<R, double, ZW, FAST> c(args, T, col, j);
     if (STAGE == 1) c.setup_level1(k);
     else c.setup_level2(k);
 }
@@ -154,7 +154,7 @@ static __global__ __launch_bounds__(256) void mark_scatter_kernel(const int* mar
 // args0.defer_list / defer_count / defer_state (nullable) give the lists' storage:
 // defer_count[0..1]; defer_list[0..3n + n/256 + 1): list 1, list 2, the marks of list 1
 // and the compaction's block sums; and NRESUME x wld doubles.
-template <class R, bool ZW = false>
+template <class R, bool ZW = false, bool FAST = false>
 hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_consts* dconsts, const double* destbl,
                                   hipStream_t stream) {
     int* const defer_list = args0.defer_list;
@@ -165,9 +165,9 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     args.defer_list = nullptr;
     args.defer_count = nullptr;
     const long long t1 = (long long)n * (args.mkx + 1), t2 = (long long)n * args.mkx;
-    hipLaunchKernelGGL((setup_kernel<R, 1, ZW>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, stream, args, n,
+    hipLaunchKernelGGL((setup_kernel<R, 1, ZW, FAST>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
-    hipLaunchKernelGGL((setup_kernel<R, 2, ZW>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
+    hipLaunchKernelGGL((setup_kernel<R, 2, ZW, FAST>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream, args, n,
                        dconsts, destbl);
     const bool large = n >= 2 * 1024 * 64;
     const bool split = large && defer_list && defer_count && args0.defer_state && !args.colmap;
@@ -190,9 +190,9 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
     }
     // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build
     if (large)
-        hipLaunchKernelGGL((forward_kernel_occ2<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+        hipLaunchKernelGGL((forward_kernel_occ2<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     else
-        hipLaunchKernelGGL((forward_kernel<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
+        hipLaunchKernelGGL((forward_kernel<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     if (split) {
         auto compact = [&](int* list, int* count) {   // marks -> list in column order
             hipLaunchKernelGGL(mark_block_sum_kernel, dim3(nblk), dim3(256), 0, stream, (const int*)marks, n, bsum);
@@ -216,7 +216,7 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
             b.colmap = defer_list;
             b.colcount = defer_count;
             b.col0 = 0;
-            hipLaunchKernelGGL((forward_kernel_occ2<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts,
+            hipLaunchKernelGGL((forward_kernel_occ2<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts,
                                destbl);
         } else {
             // the first launch marked the columns that need a second convection pass
@@ -234,7 +234,7 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
         c.defer_at_closure = 0;
         c.colmap = defer_list + n;
         c.colcount = defer_count + 1;
-        hipLaunchKernelGGL((forward_kernel<R, ZW>), dim3((n + 127) / 128), dim3(128), 0, stream, c, n, dconsts, destbl);
+        hipLaunchKernelGGL((forward_kernel<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, c, n, dconsts, destbl);
     }
     return hipGetLastError();
 }

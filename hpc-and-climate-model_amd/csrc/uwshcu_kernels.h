@@ -13,7 +13,10 @@ hipError_t launch_forward_fp64(const FwdArgs& args, int n, const uwshcu_consts* 
 // The modified scheme compute_zwshcu (FP64; args.zw, args.omega, args.tpert, args.qpert).
 hipError_t launch_forward_zw(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                              hipStream_t stream);
-// Same for the mixed-precision kernel (FP32 bulk, FP64 saturation/CIN/fix-ups).
+// The fast FP64 pass of UWSHCU_MIXED: saturation adjustment solved to convergence.
+hipError_t launch_forward_fast(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                               hipStream_t stream);
+// The FP32-plume kernel of UWSHCU_MIXED_UNCHECKED (FP64 saturation/CIN/fix-ups).
 hipError_t launch_forward_mixed(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                 hipStream_t stream);
 // Zero *count, then append to list every column of [0, args.ncol) whose amb_flag is

@@ -34,7 +34,9 @@ struct SatState {
     double tc, nu, leff, temps, tprev, es, qs;
 };
 
-template <class D>
+// FAST selects the saturation adjustment of the fast path (conden_d below): the same
+// equation solved to convergence instead of by the reference's ten fixed steps.
+template <class D, bool FAST = false>
 struct ThermoT {
     const uwshcu_consts* c;   // kernel-argument copy
     const double* estbl;      // LDS copy of the table (FP64 values)
@@ -177,14 +179,35 @@ struct ThermoT {
         const double rleff = div_fast(1.0, leff), rr = div_fast(1.0, c->r);
         const double cpl = div_rcp(c->cp, leff, rleff);
         double tprev = temps;
-        for (int it = 0; it < 10; ++it) {
-            tprev = temps;
-            const double rt = div_fast(1.0, temps);
-            const double num = div_rcp((tc - temps) * c->cp, leff, rleff) + qt - rvls;
-            const double d3 = div_rcp(div_rcp(div_rcp(c->ep2 * leff * rvls, c->r, rr), temps, rt), temps, rt);
-            temps = temps + div_fast(num, cpl + d3);
-            qsat_c(temps, es, qs);
+        if constexpr (FAST) {
+            // The fixed point of the reference's iteration is the root of
+            //     F(T) = (tc - T) cp/leff + qt - qs(T, p),
+            // with qs from the piecewise-linear table.  Newton's method with F's own
+            // derivative, -cp/leff - dqs/dT, where dqs/dT = epsqs p (des/dT) / (p - (1-epsqs) es)^2
+            // and des/dT the table segment's slope (zero where T is clamped or qs capped),
+            // converges quadratically inside a segment; the loop stops once a step moves
+            // T by at most 1e-10 of itself (at most ten steps, as the reference).
+            for (int it = 0; it < 10; ++it) {
+                tprev = temps;
+                const double dedt = (temps > c->tmin && temps < c->tmax) ? thi - tlo : 0.0;
+                const double den = p - omeps * es;
+                const double dqs = (qs < 1.0) ? div_fast(c->epsqs * p * dedt, den * den) : 0.0;
+                const double f = div_rcp((tc - temps) * c->cp, leff, rleff) + qt - qs;
+                temps = temps + div_fast(f, cpl + dqs);
+                qsat_c(temps, es, qs);
+                if (fabs(temps - tprev) <= 1.e-10 * temps) break;
+            }
             rvls = qs;
+        } else {
+            for (int it = 0; it < 10; ++it) {
+                tprev = temps;
+                const double rt = div_fast(1.0, temps);
+                const double num = div_rcp((tc - temps) * c->cp, leff, rleff) + qt - rvls;
+                const double d3 = div_rcp(div_rcp(div_rcp(c->ep2 * leff * rvls, c->r, rr), temps, rt), temps, rt);
+                temps = temps + div_fast(num, cpl + d3);
+                qsat_c(temps, es, qs);
+                rvls = qs;
+            }
         }
         const double qc = fmax(qt - qs, 0.0);
         qv = qt - qc;
@@ -329,6 +352,7 @@ struct ThermoT {
     }
 };
 using Thermo = ThermoT<double>;
+using ThermoFast = ThermoT<double, true>;
 
 // uwshcu.F90:4787-4829
 template <class R>
@@ -407,8 +431,8 @@ UWSHCU_HD inline R compute_ppen(R wtwb, R D, R bogbot, R bogtop, R rho0j, R dpen
 // early exit fires only when no unconverged point remains), so each (column,level)
 // is solved independently.  Returns the doit code: 0 not converged, 1 skipped,
 // 2 converged, 4 too cold; enerr is the relative enthalpy error of converged points.
-template <class D>
-UWSHCU_HD inline int findsp_point(const ThermoT<D>& T, D q, D t, D p, D& qsp_out, D& enerr) {
+template <class D, bool F>
+UWSHCU_HD inline int findsp_point(const ThermoT<D, F>& T, D q, D t, D p, D& qsp_out, D& enerr) {
     const uwshcu_consts* c = T.c;
     const double omeps = 1.0 - c->epsqs, trinv = 1.0 / c->ttrice;
     const double a1 = 7.5 * ulog(10.0), rair = 287.04, c3 = rair * a1 / c->cpws;

@@ -16,8 +16,9 @@ ESTBL_LEN = 250
 NDIAG = 12
 
 FP64 = 0
-MIXED = 1             # FP32 plume pass + FP64 conservation/branch check + FP64 recompute
-MIXED_UNCHECKED = 2   # the FP32 pass alone
+MIXED = 1                  # fast FP64 pass (converged saturation adjustment) + check + FP64 recompute
+MIXED_UNCHECKED = 2        # experimental FP32-plume pass alone (tendencies off by 1-2 %)
+MIXED_FAST_UNCHECKED = 3   # the fast FP64 pass alone
 
 ERRORS = {
     0: "ok",
@@ -30,6 +31,7 @@ ERRORS = {
     -13: "no usable GPU",
     -14: "adjoint tape capacity exceeded",
     -15: "chunk needs more than the static scratch of the allocation-free entry point",
+    -16: "drop-in called before uwshcu_set_dropin_params (uwshcu_rpen unset)",
 }
 
 NOALLOC_SCRATCH_BYTES = 256 << 20   # UWSHCU_NOALLOC_SCRATCH_BYTES

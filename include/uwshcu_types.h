@@ -52,13 +52,21 @@ typedef struct uwshcu_zw_params {
 
 // Precision modes.
 enum {
-    UWSHCU_FP64  = 0,   // all arithmetic in FP64
-    // FP32 plume arithmetic; FP64 saturation adjustment, LCL, CIN, cloud-base closure
-    // and moisture fix-up; then every column that fails the FP64 water/energy
-    // conservation check (relative tolerance 1e-4 by default) or took an FP32 branch
-    // decision within rounding of its branch point is recomputed in FP64 on the GPU.
+    UWSHCU_FP64  = 0,   // the reference's arithmetic in FP64, bit for bit
+    // The fast path: every state in FP64, the saturation adjustment (conden) solved by
+    // Newton's method with the table's own derivative to 1e-10 of T instead of by the
+    // reference's ten fixed steps; then every column that fails the FP64 water/energy
+    // conservation check (relative tolerance 1e-4 by default) or took a branch decision
+    // within 1e-8 of its branch point is recomputed by the FP64 path on the GPU.
+    // Tendencies stay within about 1e-6 of UWSHCU_FP64 (relative L2 over a chunk).
     UWSHCU_MIXED = 1,
-    UWSHCU_MIXED_UNCHECKED = 2   // the FP32 pass alone (no check, no recompute)
+    // Experimental: FP32 plume arithmetic (FP64 saturation adjustment, LCL, CIN,
+    // closure and fix-ups), no check and no recompute.  Its tendencies differ from
+    // UWSHCU_FP64 by 1-2 % (relative L2): the buoyancy differences of ~300 K virtual
+    // temperatures carry 1e-4 relative FP32 error, which the plume amplifies.
+    UWSHCU_MIXED_UNCHECKED = 2,
+    // The fast pass of UWSHCU_MIXED alone: no check, no recompute.
+    UWSHCU_MIXED_FAST_UNCHECKED = 3
 };
 
 // Cotangents of the outputs for the adjoint: device pointers in the caller's layout
@@ -126,7 +134,8 @@ enum {
     UWSHCU_ERR_HIP            = -12,  // a HIP runtime call failed
     UWSHCU_ERR_NO_DEVICE      = -13,  // no usable GPU
     UWSHCU_ERR_TAPE           = -14,  // adjoint: a column's tape exceeded its capacity
-    UWSHCU_ERR_SCRATCH        = -15   // allocation-free entry point: the chunk needs more than its static scratch
+    UWSHCU_ERR_SCRATCH        = -15,  // allocation-free entry point: the chunk needs more than its static scratch
+    UWSHCU_ERR_PARAMS         = -16   // drop-in called before uwshcu_set_dropin_params (uwshcu.F90:241-242)
 };
 
 #ifdef __cplusplus

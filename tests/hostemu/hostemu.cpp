@@ -36,9 +36,9 @@ extern "C" void hostemu_set_zw(const uwshcu_zw_params* zw, const double* omega, 
     g_zw = zw; g_omega = omega; g_tpert = tpert; g_qpert = qpert;
 }
 
-template <class R, bool ZW = false>
-static void run_columns(FwdArgs& a, const Thermo& T, int ncol, int pver) {
-    using Col = Column<R, double, ZW>;
+template <class R, bool ZW = false, bool FAST = false>
+static void run_columns(FwdArgs& a, const ThermoT<double, FAST>& T, int ncol, int pver) {
+    using Col = Column<R, double, ZW, FAST>;
     if (g_staged) {
         a.env_ready = 1;
 #pragma omp parallel for schedule(static)
@@ -112,22 +112,26 @@ extern "C" int hostemu_forward(const uwshcu_consts* c, const double* estbl, int 
     a.cnt = cnt; a.cnb = cnb; a.wu = wu; a.diag = diag;
     int err[2] = {0, -1};
     a.err = err;
-    // precision: UWSHCU_FP64, UWSHCU_MIXED (FP32 pass, conservation/branch check, FP64
-    // recompute of the listed columns -- the product's mixed pipeline, run serially
-    // here), or UWSHCU_MIXED_UNCHECKED (the FP32 pass alone).
-    const bool fp32 = precision != UWSHCU_FP64;
+    // precision: UWSHCU_FP64, UWSHCU_MIXED (the fast FP64 pass, conservation/branch
+    // check, FP64 recompute of the listed columns -- the product's mixed pipeline, run
+    // serially here), or UWSHCU_MIXED_UNCHECKED (the FP32 pass alone).
+    const bool fp32 = precision == UWSHCU_MIXED_UNCHECKED;
+    const bool fast = precision == UWSHCU_MIXED || precision == UWSHCU_MIXED_FAST_UNCHECKED;
     a.wld = ncol;
     std::vector<char> ws(ws_bytes_per_column(pver, ncnst, sizeof(double)) * (size_t)ncol + 64);
     std::memset(ws.data(), 0xff, ws.size());   // NaN pattern: unwritten workspace reads show up
     a.ws = ws.data();
     std::vector<int> amb(ncol, 0);
     std::vector<double> cush_in(cush, cush + ncol);
-    a.amb_flag = fp32 ? amb.data() : nullptr;
+    a.amb_flag = (fp32 || fast) ? amb.data() : nullptr;
     const Thermo T{c, estbl};
+    const ThermoFast TF{c, estbl};
     if (g_zw) {
-        if (fp32) return UWSHCU_ERR_ARG;
+        if (fp32 || fast) return UWSHCU_ERR_ARG;
         a.zw = *g_zw; a.omega = g_omega; a.tpert = g_tpert; a.qpert = g_qpert;
         run_columns<double, true>(a, T, ncol, pver);
+    } else if (fast) {
+        run_columns<double, false, true>(a, TF, ncol, pver);
     } else if (!fp32) {
         run_columns<double>(a, T, ncol, pver);
     } else {

@@ -71,11 +71,12 @@ def test_table_builder_equals_the_oracle_table():
 
 def test_workspace_size():
     # an FP64 environment block (23 arrays) and a plume block of 29 + 4*ncnst arrays,
-    # each of pver+1 levels; the plume block is FP32 in the mixed precision
+    # each of pver+1 levels; the plume block is FP32 in the FP32-plume pass only
     fp64 = _capi.workspace_bytes_per_column(35, 3, _capi.FP64)
-    mixed = _capi.workspace_bytes_per_column(35, 3, _capi.MIXED)
     assert fp64 == 36 * 8 * (23 + 29 + 12)
-    assert mixed == 36 * (8 * 23 + 4 * (29 + 12))
+    assert _capi.workspace_bytes_per_column(35, 3, _capi.MIXED) == fp64
+    assert _capi.workspace_bytes_per_column(35, 3, _capi.MIXED_FAST_UNCHECKED) == fp64
+    assert _capi.workspace_bytes_per_column(35, 3, _capi.MIXED_UNCHECKED) == 36 * (8 * 23 + 4 * (29 + 12))
 
 
 def test_device_entry_points_reject_bad_arguments_without_a_gpu():

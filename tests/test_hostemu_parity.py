@@ -86,21 +86,51 @@ def test_dry_constituents_use_dry_pressure_thickness():
     np.testing.assert_array_equal(got["trten"], ref["trten"])
 
 
-def test_mixed_precision_error_and_budgets():
-    ch = make_chunk(2048, 35, 3, seed=23)
-    ref, rdiag, _, _ = oracle.run(ch, nthreads=8)
-    got, gdiag, _, _ = hostemu.run(ch, precision=1)
-    same = (gdiag[:, 7] == rdiag[:, 7])
-    assert same.mean() > 0.97                          # branch pattern preserved almost everywhere
-    conv = (rdiag[:, 7] > 0) & same & (gdiag[:, 6] == rdiag[:, 6])
-    errs = {k: np.median(parity.column_rel_err(got[k][..., conv], ref[k][..., conv]))
-            for k in ("qvten", "sten", "umf", "qlten", "precip")}
-    assert max(errs.values()) < 1e-3, errs
+# Per-output error of a precision mode against the FP64 path: relative L2 over the chunk
+# and the worst column (parity.column_rel_err), for the tendencies, fluxes and column
+# outputs the north star's bound applies to.
+ERR_FIELDS = ("sten", "qvten", "qlten", "qiten", "uten", "vten", "trten", "umf", "slflx", "qtflx", "qrten", "qsten",
+              "evapc", "precip", "snow", "cufrc")
+
+
+def mode_errors(got, ref):
+    l2 = {k: float(np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)) for k in ERR_FIELDS}
+    col = {k: parity.column_rel_err(got[k], ref[k]) for k in ERR_FIELDS}
+    return l2, col
+
+
+@pytest.mark.parametrize("inv", [0.896, 0.48, 0.0625])
+def test_mixed_mode_meets_the_1e4_tendency_bound(inv):
+    """UWSHCU_MIXED (fast FP64 pass, check, FP64 recompute) against the FP64 oracle at
+    10 / 50 / 90 % convecting: integer diagnostics identical, every output within 1e-5
+    relative L2 over the chunk (the bound is 1e-4), no column above 1e-4 on these
+    seeds, budgets closed to FP64 rounding."""
+    ch = make_chunk(4096, 35, 3, seed=23, inversion_frac=inv)
+    ref, rdiag, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    st = {}
+    got, gdiag, rc, _ = hostemu.run(ch, precision=1, stats=st)
+    assert rc == 0
+    np.testing.assert_array_equal(gdiag, rdiag)
+    l2, col = mode_errors(got, ref)
+    assert max(l2.values()) <= 1e-5, l2
+    worst = {k: float(v.max()) for k, v in col.items()}
+    assert max(worst.values()) <= 1e-4, worst
+    assert st["recomputed"] < 0.05 * ch.ncol
     dp = ch["dp0"]
     w = ((got["qvten"] + got["qlten"] + got["qiten"]) * dp).sum(0) / GRAV + (got["precip"] + got["rliq"]) * 1000.0
     scale = np.abs(got["qtflx"]).max(0) + 1e-30
-    c = gdiag[:, 7] > 0
-    assert np.median(np.abs(w[c]) / scale[c]) < 1e-5
+    assert np.max(np.abs(w) / scale) < 1e-11
+
+
+def test_fp32_plume_pass_misses_the_bound():
+    """The FP32-plume pass (UWSHCU_MIXED_UNCHECKED) is kept as an experiment: its
+    tendencies differ from FP64 by far more than 1e-4, which is why UWSHCU_MIXED does
+    not use it."""
+    ch = make_chunk(2048, 35, 3, seed=23, inversion_frac=0.48)
+    ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
+    got, _, _, _ = hostemu.run(ch, precision=2)
+    l2, _ = mode_errors(got, ref)
+    assert l2["qvten"] > 1e-3 and l2["sten"] > 1e-3
 
 
 def test_libm_sensitivity_is_confined_to_few_columns():
@@ -148,15 +178,15 @@ def test_column_code_stays_in_bounds(ncol, pver, ncnst, inv, precision, staged):
 
 
 def test_mixed_pipeline_recomputes_listed_columns_in_fp64():
-    """UWSHCU_MIXED: an FP32 plume pass, then an FP64 recompute of every column that
-    fails the conservation check or took an FP32 branch decision within rounding.
+    """UWSHCU_MIXED: the fast pass, then an FP64 recompute of every column that fails
+    the conservation check or took a branch decision within 1e-8 of its branch point.
     Columns it changed must equal the FP64 result bit for bit; the others must equal
-    the FP32 pass."""
+    the fast pass (UWSHCU_MIXED_FAST_UNCHECKED)."""
     ch = make_chunk(2048, 35, 3, seed=31, inversion_frac=0.3)
     ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
     st = {}
     mix, _, _, _ = hostemu.run(ch, precision=1, stats=st)
-    raw, _, _, _ = hostemu.run(ch, precision=2)
+    raw, _, _, _ = hostemu.run(ch, precision=3)
     changed = np.zeros(ch.ncol, bool)
     for k in ref:
         changed |= (np.asarray(mix[k]) != np.asarray(raw[k])).reshape(-1, ch.ncol).any(axis=0)
@@ -164,4 +194,3 @@ def test_mixed_pipeline_recomputes_listed_columns_in_fp64():
     for k in ref:
         np.testing.assert_array_equal(mix[k][..., changed], ref[k][..., changed], err_msg=k)
         np.testing.assert_array_equal(mix[k][..., ~changed], raw[k][..., ~changed], err_msg=k)
-
