@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 
 
-def build(arch: str = "gfx950", jobs: int = 3, force: bool = False) -> str:
+def build(arch: str = "gfx950", jobs: int = 8, force: bool = False) -> str:
     """Compile every HIP source for ``arch``; returns the path of the shared library."""
     env = dict(os.environ)
     if force:
