@@ -123,3 +123,71 @@ def test_autograd_rejects_inputs_changed_in_place():
             inp[name].add_(0.5)
         with pytest.raises(RuntimeError, match="inplace"):
             loss.backward()
+
+
+def test_c4_gradients_at_65536_columns():
+    """The 65 536 x 35 x 3 calibration workload: the loss sum (sten/cp - tau_T)^2 +
+    (qvten - tau_q)^2 with tau_T ~ N(0, 1e-5) K/s and tau_q ~ N(0, 1e-8) 1/s, its
+    gradient by the GPU adjoint with respect to t0, qv0, u0, v0, tke and (rpen, rle, rkm).
+    On 512 sampled columns, along a random direction v: <grad, v> equals <cot, J v> from
+    the dual-number oracle (median 1e-12, 95th percentile 1e-9), and equals centred
+    differences of the FP64 oracle's loss to 1e-6 on the columns away from branch
+    switches (differences at two steps agree, one-sided ones are consistent with a smooth loss)."""
+    import torch
+    from uwshcu_amd import device
+    cp = 1.00464e3
+    n, L = 65536, 35
+    ch = make_chunk(n, L, 3, seed=70, inversion_frac=0.3)
+    rng = np.random.default_rng(71)
+    tau_t, tau_q = rng.normal(0, 1e-5, (L, n)), rng.normal(0, 1e-8, (L, n))
+    ctx = device.default_context()
+    inp = device.chunk_to_device(ch, "cuda:0")
+    out = ctx.compute_uwshcu_inv(inp, dt=ch.dt)
+    sten, qvten = out["sten"].cpu().numpy(), out["qvten"].cpu().numpy()
+    cot = {"sten": 2.0 * (sten / cp - tau_t) / cp, "qvten": 2.0 * (qvten - tau_q)}
+    print(f"\n[c4] GPU adjoint over {n}x{L}x3", flush=True)
+    rc, _, g = _gpu_adjoint(ch, cot)
+    assert rc == 0
+    idx = np.sort(rng.choice(n, 512, replace=False))
+    sub = oracle.column_chunk(ch, 0)
+    sub.ncol = len(idx)
+    sub.fields = {k: np.ascontiguousarray(v[..., idx]) for k, v in ch.fields.items()}
+    m = len(idx)
+    v = {"t0": rng.normal(0, 1, (L, m)), "qv0": rng.normal(0, 1e-4, (L, m)), "u0": rng.normal(0, 1, (L, m)),
+         "v0": rng.normal(0, 1, (L, m)), "tke": rng.normal(0, 0.1, (L + 1, m))}
+    dpar = np.array([1.0, 0.01, 1.0])
+    lhs = sum((g[k][:, idx] * v[k]).sum(axis=0) for k in v) + (g["par_col"][:, idx] * dpar[:, None]).sum(axis=0)
+    _, tout, diag = oracle.jvp(sub, v, dpar=tuple(dpar))
+    subcot = {k: c[:, idx] for k, c in cot.items()}
+    rhs = sum((subcot[k] * tout[k]).sum(axis=0) for k in subcot)
+    conv = diag[:, 7] > 0
+    rel = np.abs(lhs - rhs) / np.maximum(np.abs(lhs) + np.abs(rhs), 1e-300)
+    print(f"[c4] adjoint vs dual numbers on {conv.sum()} convecting columns: median {np.median(rel[conv]):.1e} "
+          f"p95 {np.quantile(rel[conv], 0.95):.1e} max {rel[conv].max():.1e}", flush=True)
+    assert np.median(rel[conv]) <= 1e-12 and np.quantile(rel[conv], 0.95) <= 1e-9
+    assert not lhs[~conv].any()
+
+    def loss(s):
+        c2 = sub.copy()
+        for k in v:
+            c2.fields[k] = sub[k] + s * v[k]
+        r = oracle.run(c2, nthreads=16, portable=True,
+                       consts=oracle.default_consts(rpen=10.0 + s * dpar[0], rle=0.1 + s * dpar[1],
+                                                    rkm=14.0 + s * dpar[2]))[0]
+        return (((r["sten"] / cp - tau_t[:, idx]) ** 2).sum(axis=0) + ((r["qvten"] - tau_q[:, idx]) ** 2).sum(axis=0))
+
+    h = 1e-6
+    l0, lp, lm, lp2, lm2 = loss(0.0), loss(h), loss(-h), loss(h / 2), loss(-h / 2)
+    c1, c2 = (lp - lm) / (2 * h), (lp2 - lm2) / h
+    # one-sided differences at h and h/2: fw - bw = 2 ((fw - fw/2) + (bw/2 - bw)) + O(h^2)
+    # for a smooth loss; a kink at the point breaks it by its slope jump (the centred
+    # differences at both steps then agree with each other, not with the derivative)
+    fw, bw, fw2, bw2 = (lp - l0) / h, (l0 - lm) / h, (lp2 - l0) / (h / 2), (l0 - lm2) / (h / 2)
+    kink = np.abs((fw - bw) - 2.0 * ((fw - fw2) + (bw2 - bw)))
+    scale = np.abs(lhs) + 1e-300
+    smooth = conv & (np.abs(c1 - c2) <= 1e-7 * scale) & (kink <= 1e-6 * scale)
+    err = np.abs(lhs - c2) / scale
+    print(f"[c4] adjoint vs centred differences on {smooth.sum()} smooth columns: median {np.median(err[smooth]):.1e} "
+          f"max {err[smooth].max():.1e}", flush=True)
+    assert smooth.sum() > 100
+    assert np.all(err[smooth] <= 1e-6)

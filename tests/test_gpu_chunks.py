@@ -1,8 +1,10 @@
-"""One launch over many chunks (uwshcu_forward_chunks_device) against the same
-columns run as one chunk: bit-identical outputs, untouched padding."""
+"""One launch over many chunks (uwshcu_forward_chunks_device): against the portable-math
+oracle on the same columns and against the same columns run as one chunk, bit for bit,
+with untouched padding."""
 import numpy as np
 import pytest
 
+import oracle
 from uwshcu_amd.generator import make_chunk
 
 pytestmark = pytest.mark.gpu
@@ -53,17 +55,21 @@ def test_chunks_equal_one_chunk(sizes, pad):
     ref = ctx.compute_uwshcu_inv(inp, dt=ch.dt)
     torch.cuda.synchronize()
     parts, calls = _run_chunks(ctx, inp, sizes, pad, ch.dt)
+    orc, _, orc_rc, _ = oracle.run(ch, nthreads=8, portable=True)
+    assert orc_rc == 0
     for (sub, n, c0), (_, out, cush, _) in zip(parts, calls):
         got = dict(out, cush=cush)
         for k, r in ref.items():
             g = got[k].cpu().numpy()
+            np.testing.assert_array_equal(g[..., :n], orc[k][..., c0:c0 + n], err_msg=k)   # the oracle
             np.testing.assert_array_equal(g[..., :n], r[..., c0:c0 + n].cpu().numpy(), err_msg=k)
             assert np.all(g[..., n:] == SENTINEL), k
 
 
 def test_chunks_large_enough_for_the_deferring_pipeline():
     """40 chunks of 4096 columns: the packed range (163840 columns) runs the
-    two-wavefront column kernel with deferred second passes."""
+    two-wavefront column kernel with deferred second passes.  Every chunk is a copy of
+    the same 4096 columns, so every chunk must equal the oracle on them."""
     import torch
     from uwshcu_amd import device
     base = make_chunk(4096, 35, 3, seed=43)
@@ -73,7 +79,10 @@ def test_chunks_large_enough_for_the_deferring_pipeline():
     ref = ctx.compute_uwshcu_inv(inp, dt=base.dt)
     torch.cuda.synchronize()
     parts, calls = _run_chunks(ctx, inp, [4096] * 40, 0, base.dt)
+    orc, _, orc_rc, _ = oracle.run(base, nthreads=8, portable=True)
+    assert orc_rc == 0
     for (sub, n, c0), (_, out, cush, _) in zip(parts, calls):
         got = dict(out, cush=cush)
         for k, r in ref.items():
             assert torch.equal(got[k], r[..., c0:c0 + n]), k
+            np.testing.assert_array_equal(got[k].cpu().numpy(), orc[k], err_msg=k)
