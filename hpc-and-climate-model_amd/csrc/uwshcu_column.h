@@ -113,6 +113,32 @@ struct FwdArgs {
     const double *omega, *tpert, *qpert;
 };
 
+// Level loops whose first level differs between columns (the plume from krel, the
+// penetrative entrainment down from kpen-1) start every lane of a wavefront at the
+// wavefront's extreme first level; a lane runs the loop body only from its own first
+// level on, so its results are those of its own loop.  The lanes then address the same
+// level of the level-major arrays in each iteration, one 64-column row for the
+// wavefront, where lanes at different levels would each read a row of their own.
+// Returns the smallest k among the wavefront's active lanes (the lane's own k on the host).
+UWSHCU_HD inline int wave_min_level(int k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int kw = 0;
+    while (__ballot(k <= kw) == 0ull) ++kw;   // ends at this lane's k at the latest
+    return kw;
+#else
+    return k;
+#endif
+}
+UWSHCU_HD inline int wave_max_level(int k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int kw = k;
+    while (__ballot(k > kw) != 0ull) ++kw;    // ends at the largest k of the active lanes
+    return kw;
+#else
+    return k;
+#endif
+}
+
 // zmh_ramp (mzfunctions_mod.F90:222-248): 0 below x0, 1 above x0 + xint, linear in
 // between; the floor on xint is the single-precision literal 1.e-20 there.
 UWSHCU_HD inline double zmh_ramp(double x, double x0, double xint) {
@@ -142,7 +168,23 @@ UWSHCU_HD inline int ad_inputs(int mkx) { return ADI_PARAMS * (mkx + 1) + 3; }
 
 // Exit reasons reported in diag row 0 (same numbering as the oracle).
 // Scalars of a column deferred at its second convection pass (FwdArgs::defer_state).
-enum { NRESUME = 56 };
+// Field RS_CONT (after the 56 scalars) is set by the buoyancy-sorting and the
+// first-pass-rest parts of a split launch: 1 where the column continues in the next
+// part, 0 where it ended there.
+enum { NRESUME = 57, RS_CONT = 56 };
+
+// Parts of the column pass that the split pipeline runs as launches of their own.
+// PART_ALL runs whatever FwdArgs::resume asks for; the others fix it at compile time,
+// so that each kernel holds only its own part's live state.
+enum : int {
+    PART_ALL = 0,
+    PART_SORT = 1,      // a listed column from its closure through the buoyancy sorting (S11)
+    PART_REST = 2,      // the same from the forcedCu filter (S12) through the in-cloud
+                        // diagnostics (S21)
+    PART_CLOSURE = 3,   // every column up to the cloud-base closure of its first pass
+    PART_CIN2 = 4       // the same, after the level-parallel set-B environment (S22), through
+                        // the implicit-CIN iteration's CIN (S1-S7), deferring a second pass
+};
 
 enum { EX_NONE = 0, EX_CONDEN = 1, EX_KINV1 = 2, EX_KLCLMKX = 3, EX_KLFCMKX = 4, EX_MU3 = 5, EX_WTW = 6,
        EX_UFRC = 7, EX_WU = 8, EX_CUFILTER = 9, EX_SUBCONDEN = 10, EX_KBUPCONDEN = 11 };
@@ -406,12 +448,15 @@ struct Column {
     }
 
     UWSHCU_HD void run() const { run_body<true>(); }
+    template <int PART>
+    UWSHCU_HD void run_part() const { run_body<true, PART>(); }
     // the column pass itself, expanded into each forward kernel so that each gets its
     // own register allocation (the two builds differ in their register budget).
-    // kSplit = false leaves out what only the first and second launches of a split
-    // large launch do (stop at, and continue from, the buoyancy sorting), which the
-    // one-wavefront-per-SIMD build never runs.
-    template <bool kSplit>
+    // kSplit = false leaves out what only the launches of a split large launch do (stop
+    // at, and continue from, the closure, the buoyancy sorting or the second pass),
+    // which the one-wavefront-per-SIMD build never runs.  PART (PART_*) fixes which part
+    // of a split launch runs.
+    template <bool kSplit, int PART = PART_ALL>
     UWSHCU_HD __attribute__((always_inline)) void run_body() const;
 
     // Level-parallel set-up (one thread per column and level), the part of S0 and of
@@ -459,6 +504,33 @@ struct Column {
         env_mid(k, thv0j, qs0);
     }
 
+    // S22 of a column's first pass, level-parallel (uwshcu.F90:4169-4329), after the
+    // first pass's tendencies are in place: stage 1, level k = 1..mkx, the provisional
+    // state's thl and qt (set B); stage 2, level k = 1..mkx, after stage 1 of the
+    // neighbouring levels, their slopes and the set-B interface environment (a failed
+    // saturation adjustment marks B_THV0BOT(k) = -1, which ends the column as the
+    // sequential S22 would).  Every value equals the one run() computes in S22.
+    UWSHCU_HD void setup_level_b1(int k) const {
+        const D qv0s = in_qv0(k) + out(a.qvten, k) * a.dt;
+        const D ql0s = in(a.ql0, k) + out(a.qlten, k) * a.dt;
+        const D qi0s = in(a.qi0, k) + out(a.qiten, k) * a.dt;
+        const D t0s = in_t0(k) + out(a.sten, k) * a.dt / C.cp;
+        const D qt0 = (qv0s + ql0s + qi0s);
+        E(B_QT0, k) = qt0;
+        E(B_THL0, k) = (t0s - C.xlv * ql0s / C.cp - C.xls * qi0s / C.cp) / E(A_EXN0, k);
+    }
+    UWSHCU_HD void setup_level_b2(int k) const {
+        const int j = k < mkx ? k : mkx - 1;
+        auto mm = [&](int arr) {
+            const D above = (E(arr, j + 1) - E(arr, j)) / (in(a.p0, j + 1) - in(a.p0, j));
+            const D below = (j == 1) ? above : (E(arr, j) - E(arr, j - 1)) / (in(a.p0, j) - in(a.p0, j - 1));
+            return (above > 0.0) ? fmax(0.0, fmin(above, below)) : fmin(0.0, fmax(above, below));
+        };
+        E(B_SSTHL0, k) = mm(B_THL0);
+        E(B_SSQT0, k) = mm(B_QT0);
+        if (!interface_env_level(B_THL0, k)) E(B_THV0BOT, k) = -1.0;
+    }
+
     // Adjoint kernel, after run(): copy the recorded outputs' values to the caller's
     // arrays and their tape nodes to nodes[(f * (mkx+1) + k) * wld + slot].
     UWSHCU_HD void ad_export(int* nodes) const {
@@ -485,8 +557,17 @@ struct Column {
 };
 
 template <class R, class D, bool ZW, bool FAST>
-template <bool kSplit>
+template <bool kSplit, int PART>
 UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body() const {
+    static_assert(PART == PART_ALL || kSplit, "the parts of a split launch need the split build");
+    // what this launch continues from: 0 the start, 1 the closure of the first pass,
+    // 2 the second convection pass, 3 the forcedCu filter of the first pass, 4 the
+    // implicit-CIN iteration
+    const int resume = PART == PART_SORT ? 1 : PART == PART_REST ? 3 : PART == PART_CLOSURE ? 0
+                     : PART == PART_CIN2 ? 4 : a.resume;
+    if constexpr (PART == PART_REST || PART == PART_CIN2) {
+        if (rstate(RS_CONT) == 0.0) return;      // the column ended in the previous part
+    }
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
@@ -546,7 +627,10 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
     for (int m = 1; m <= ncnst; ++m) slope_tr([&](int k) { return R(trin(m, k)); }, m);
     }
 
-    if (!kSplit || !a.resume) {   // a column continued at its first pass keeps what launch 1 wrote
+    // a column continued at its first pass keeps what launch 1 wrote; launch 1 of a split
+    // clears a column's outputs and plume arrays only when it reaches the release level
+    // (S1-S7 read neither, and every exit rewrites all outputs)
+    if (PART == PART_ALL && (!kSplit || !resume)) {
         zero_pass_state();
         for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
     }
@@ -594,8 +678,15 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         if (mark) a.defer_mark[iw] = 1;
         else a.defer_list[__atomic_fetch_add(a.defer_count, 1, __ATOMIC_RELAXED)] = i;
     };
-    if (a.resume) {
+    if (resume) {
         resume_io([&](int f, auto& x) { x = decltype(+x)(rstate(f)); });
+    }
+    if constexpr (PART == PART_CIN2) {
+        // the first pass ended with this cush; a failed saturation adjustment of the
+        // set-B environment ends the column as the first pass's S22 would
+        cush_s = cush;
+        for (int k = 1; k <= mkx; ++k)
+            if (E(B_THV0BOT, k) < 0.0) { exit_iter = 1; exit_code = EX_CONDEN; goto L333; }
     }
 
     if (a.env_ready) {
@@ -606,16 +697,16 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         goto L333;
     }
 
-    for (int iter = a.resume ? (kSplit ? a.resume : 2) : 1; iter <= 2; ++iter) {
+    for (int iter = resume ? (kSplit ? (resume == 3 ? 1 : resume == 4 ? 2 : resume) : 2) : 1; iter <= 2; ++iter) {
         exit_iter = iter;
         const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
         const int e_thvb = (iter == 1) ? A_THV0BOT : B_THV0BOT, e_thvt = e_thvb + 1;
         const int e_thvlb = (iter == 1) ? A_THVL0BOT : B_THVL0BOT, e_thvlt = e_thvlb + 1;
         const D tscaleh = cush;
-        cush = -1.0;
+        if (!(resume == 3 && iter == 1)) cush = -1.0;   // continued after S11: its cush stands
 
-        if (kSplit ? a.resume == iter : a.resume != 0) {
+        if (kSplit ? (resume == iter || (resume == 3 && iter == 1)) : resume != 0) {
             // a deferred column continues with the scalars it saved: at the second pass
             // those of its implicit-CIN restore, whose convection starts afresh
             if (!kSplit || iter == 2) zero_pass_state();
@@ -675,7 +766,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         klfc = mkx;
         if (klcl >= kinv) {
             D thvubot = 0.0, thvutop = 0.0;
-            for (int k = kinv; k <= mkx - 1; ++k) {
+            for (int k = wave_min_level(kinv); k <= mkx - 1; ++k) {
+                if (k < kinv) continue;
                 if (k < klcl) {
                     thvubot = thvlsrc;
                     thvutop = thvlsrc;
@@ -701,7 +793,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             }
         } else {
             cinlcl = 0.0;
-            for (int k = kinv; k <= mkx - 1; ++k) {
+            for (int k = wave_min_level(kinv); k <= mkx - 1; ++k) {
+                if (k < kinv) continue;
                 if (T.conden_x(in(a.ps0, k - 1), E(A_EXNS0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 const D thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
                 if (T.conden_x(in(a.ps0, k), E(A_EXNS0, k), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
@@ -731,7 +824,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
                 kinv = kinv_o; klcl = klcl_o; klfc = klfc_o; plcl = plcl_o; plfc = plfc_o; tkeavg = tkeavg_o;
                 thvlmin = thvlmin_o; qtsrc = qtsrc_o; thvlsrc = thvlsrc_o; thlsrc = thlsrc_o; usrc = usrc_o;
                 vsrc = vsrc_o; thv0lcl = thv0lcl_o;
-                if (kSplit && (a.defer_list || a.defer_mark) && !a.defer_at_closure) {
+                if (PART == PART_CIN2 ||
+                    (PART == PART_ALL && kSplit && (a.defer_list || a.defer_mark) && !a.defer_at_closure)) {
                     // the second pass runs in a launch of its own: save, list, stop
                     cush = tscaleh;
                     defer_here(a.defer_mark != nullptr);
@@ -752,7 +846,11 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         {
         R wtw;   // set by the closure, or at the start of the buoyancy sorting
         // a column continued at its first pass (resume 1) resumes at the buoyancy sorting
-        if (kSplit ? !(a.resume == 1 && iter == 1) : true) {
+        if (kSplit ? !((resume == 1 || resume == 3) && iter == 1) : true) {
+        if constexpr (PART == PART_CLOSURE) {
+            zero_pass_state();
+            for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
+        }
         // ---- S8 release level (uwshcu.F90:2035-2043) ----
         if (klcl < kinv) {
             krel = kinv;
@@ -851,7 +949,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         }
         for (int m = 1; m <= ncnst; ++m) WT(1, m, krel - 1) = R(trin(m, 1));   // trsrc = tr0(1,m)
 
-        if (kSplit && iter == 1 && a.defer_at_closure && (a.defer_mark || a.defer_list)) {
+        if (PART == PART_CLOSURE ||
+            (PART == PART_ALL && kSplit && iter == 1 && a.defer_at_closure && (a.defer_mark || a.defer_list))) {
             // it convects: the plume stages run in a launch of their own, packed with the
             // other columns that got this far
             cush = tscaleh;
@@ -861,6 +960,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         }   // S8-S10
 
         // ---- S11 buoyancy sorting (uwshcu.F90:2296-2952) ----
+        if (!(kSplit && resume == 3 && iter == 1)) {   // a column continued after S11 skips it
         R scaleh = R(tscaleh);
         if (R(tscaleh) < R(0)) scaleh = R(1000);
         for (int iter_scaleh = 1; iter_scaleh <= 3; ++iter_scaleh) {
@@ -886,7 +986,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             R c_vu = W(W_VU, krel - 1), c_thvu = W(W_THVU, krel - 1);
             R c_wu = R(out(a.wu, krel - 1)), c_umf = R(out(a.umf, krel - 1));
 
-            for (int k = krel; k <= mkx - 1; ++k) {
+            for (int k = wave_min_level(krel); k <= mkx - 1; ++k) {
+                if (k < krel) continue;    // level-aligned lanes (wave_min_level)
                 const int km1 = k - 1;
                 // Read once for both iter_xc passes: the environment of layer k, which the loop
                 // body does not write, and (issued early) that of layer k+1 for the next level.
@@ -1187,11 +1288,29 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
                 scaleh = R(cush);
             }
         }
+        if constexpr (PART == PART_SORT) {
+            // the rest of the pass runs in the next launch: save what S11 set (the other
+            // saved scalars still hold the closure's values) and stop
+            resume_io([&](int f, auto& x) {
+                const bool set_here = f == 0 || (f >= 18 && f <= 29) || f == 34 || f == 35 || f == 36 ||
+                                      f == 37 || f == 39;
+                if (!set_here) return;
+                if constexpr (std::is_integral<std::remove_reference_t<decltype(x)>>::value) rstate(f) = (double)x;
+                else rstate(f) = (double)val(x);
+            });
+            rstate(RS_CONT) = 1.0;
+            return;
+        }
+        }   // S11
 
         // ---- S12 forcedCu filter (uwshcu.F90:2964-2992) ----
         if (kbup == krel) { exit_code = EX_CUFILTER; goto L333; }
-        for (int k = kpen; k <= mkx; ++k) { out(a.umf, k) = 0.0; W(W_EMF, k) = R(0); W(W_UFRC, k) = R(0); }
-        for (int k = kpen + 1; k <= mkx; ++k) { W(W_DWTEN, k) = R(0); W(W_DITEN, k) = R(0); W(W_FER, k) = R(0); W(W_FDR, k) = R(0); }
+        for (int k = wave_min_level(kpen); k <= mkx; ++k) {
+            if (k < kpen) continue;
+            out(a.umf, k) = 0.0; W(W_EMF, k) = R(0); W(W_UFRC, k) = R(0);
+            if (k == kpen) continue;
+            W(W_DWTEN, k) = R(0); W(W_DITEN, k) = R(0); W(W_FER, k) = R(0); W(W_FDR, k) = R(0);
+        }
 
         // ---- S13 penetrative entrainment (uwshcu.F90:3055-3183) ----
         for (int k = 0; k <= mkx; ++k) {
@@ -1201,7 +1320,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             W(W_VUE, k) = W(W_VU, k);
             for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = WT(1, m, k);
         }
-        for (int k = kpen - 1; k >= kbup; --k) {
+        for (int k = wave_max_level(kpen - 1); k >= kbup; --k) {
+            if (k > kpen - 1) continue;
             const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
             const R umfk = R(out(a.umf, k));
             if (k == kpen - 1) {
@@ -1286,7 +1406,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         }
         {
             R uplus = R(0), vplus = R(0);
-            for (int k = kinv; k <= krel - 1; ++k) {
+            for (int k = wave_min_level(kinv); k <= krel - 1; ++k) {
+                if (k < kinv) continue;
                 const int kp1 = k + 1;
                 const R d = R(in(a.ps0, k) - in(a.p0, kp1));
                 out(a.qtflx, k) = D(cbmf * (R(qtsrc) - (Er(A_QT0, kp1) + Er(A_SSQT0, kp1) * d)));
@@ -1299,7 +1420,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
                     WT(3, m, k) = cbmf * (R(trin(m, 1)) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
             }
         }
-        for (int k = krel; k <= kbup - 1; ++k) {
+        for (int k = wave_min_level(krel); k <= kbup - 1; ++k) {
+            if (k < krel) continue;
             const int kp1 = k + 1;
             const R d = R(in(a.ps0, k) - in(a.p0, kp1));
             const R umfk = R(out(a.umf, k));
@@ -1310,7 +1432,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             for (int m = 1; m <= ncnst; ++m)
                 WT(3, m, k) = umfk * (WT(1, m, k) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
         }
-        for (int k = kbup; k <= kpen - 1; ++k) {
+        for (int k = wave_min_level(kbup); k <= kpen - 1; ++k) {
+            if (k < kbup) continue;
             const R d = R(in(a.ps0, k) - in(a.p0, k));
             const R emfk = W(W_EMF, k);
             out(a.slflx, k) = D(cp * Er(A_EXNS0, k) * emfk * (W(W_THLUE, k) - (Er(A_THL0, k) + Er(A_SSTHL0, k) * d)));
@@ -1648,7 +1771,8 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         {
             // the tendency stage adjusted the same updraft states: reuse its condensate
             R qcubelow = qlr_base + qir_base, qlubelow = qlr_base, qiubelow = qir_base;
-            for (int k = krel; k <= kpen; ++k) {
+            for (int k = wave_min_level(krel); k <= kpen; ++k) {
+                if (k < krel) continue;
                 const R qlr = W(W_QLR, k), qir = W(W_QIR, k);
                 R qcuk = R(0.5) * (qcubelow + qlr + qir);
                 R qluk = R(0.5) * (qlubelow + qlr);
@@ -1696,6 +1820,18 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         cout(a.cnt) = D(mkx + 1 - kpen);
         cout(a.cnb) = D(mkx + 1 - (krel - 1));
 
+        if constexpr (PART == PART_REST) {
+            // S22 runs level-parallel (setup_level_b1/b2) and the implicit-CIN iteration in
+            // a launch of its own: save what S12-S21 set and stop
+            resume_io([&](int f, auto& x) {
+                if (!((f >= 26 && f <= 29) || f == 38 || f == 39)) return;
+                if constexpr (std::is_integral<std::remove_reference_t<decltype(x)>>::value) rstate(f) = (double)x;
+                else rstate(f) = (double)val(x);
+            });
+            rstate(RS_CONT) = 1.0;
+            return;
+        }
+
         // ---- S22 provisional state and environment set B (uwshcu.F90:4169-4329) ----
         if (iter == 1) {
             cush_s = cush;
@@ -1723,6 +1859,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
 L333:
     write_exit(mkx);
     path = 0;
+    if constexpr (PART == PART_SORT || PART == PART_REST) rstate(RS_CONT) = 0.0;
 
 Ldone:
     if (a.amb_flag) a.amb_flag[i] = amb;

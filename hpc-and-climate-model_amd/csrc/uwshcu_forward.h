@@ -9,7 +9,7 @@ namespace uwshcu {
 // One thread per column; the saturation table is staged into LDS once per workgroup.
 // Thread j of the launch works in workspace slot j on column col0 + j, or, with a
 // column map, on column colmap[col0 + j] while col0 + j < *colcount.
-template <class R, bool ZW, bool SPLIT, bool FAST>
+template <class R, bool ZW, bool SPLIT, bool FAST, int PART = PART_ALL>
 __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const uwshcu_consts* __restrict__ consts,
                                              const double* __restrict__ estbl) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
@@ -27,7 +27,7 @@ __device__ __forceinline__ void forward_body(const FwdArgs& args, int n, const u
     const int slot = args.resume ? col - args.slot0 : j;
     const ThermoT<double, FAST> T{consts, s_estbl};
     const Column<R, double, ZW, FAST> c(args, T, col, slot);
-    c.template run_body<SPLIT>();
+    c.template run_body<SPLIT, PART>();
 }
 
 // Two register budgets for the same column code.  forward_kernel lets it use all 512
@@ -47,8 +47,33 @@ void forward_kernel_occ2(FwdArgs args, int n, const uwshcu_consts* __restrict__ 
     forward_body<R, ZW, true, FAST>(args, n, consts, estbl);
 }
 
+// Register budgets (wavefronts per SIMD) of the parts of a split launch.
+#ifndef UWSHCU_OCC_CLOSURE
+#define UWSHCU_OCC_CLOSURE 2
+#endif
+#ifndef UWSHCU_OCC_SORT
+#define UWSHCU_OCC_SORT 2
+#endif
+#ifndef UWSHCU_OCC_REST
+#define UWSHCU_OCC_REST 2
+#endif
+#ifndef UWSHCU_OCC_CIN2
+#define UWSHCU_OCC_CIN2 2
+#endif
+
+// One part (PART_*) of a split launch, with OCC wavefronts per SIMD as its register budget.
+template <class R, bool ZW, bool FAST, int PART, int OCC>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(OCC)))
+void forward_kernel_part(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                         const double* __restrict__ estbl) {
+    forward_body<R, ZW, true, FAST, PART>(args, n, consts, estbl);
+}
+
 // Level-parallel set-up: thread t of the launch handles workspace slot t % n and
-// level t / n (+1 for stage 2), so that a wavefront covers consecutive columns.
+// level t / n (+1 for stages 2-4), so that a wavefront covers consecutive columns.
+// Stages 1 and 2: the set-A environment and the wet bulb, before the column pass.
+// Stages 3 and 4: a listed column's set-B environment (S22) after the first pass of a
+// split launch, in the column's own slot (column - slot0), where it continues.
 template <class R, int STAGE, bool ZW = false, bool FAST = false>
 __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
                                                     const double* __restrict__ estbl) {
@@ -62,13 +87,16 @@ __global__ __launch_bounds__(256) void setup_kernel(FwdArgs args, int n, const u
     __syncthreads();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = (int)(t % n);
-    const int k = (int)(t / n) + (STAGE == 2 ? 1 : 0);
+    const int k = (int)(t / n) + (STAGE == 1 ? 0 : 1);
     if (k > args.mkx || j >= nvalid) return;
     const int col = args.colmap ? args.colmap[args.col0 + j] : args.col0 + j;
     const ThermoT<double, FAST> T{consts, s_estbl};
-    const Column<R, double, ZW, FAST> c(args, T, col, j);
+    const Column<R, double, ZW, FAST> c(args, T, col, STAGE >= 3 ? col - args.slot0 : j);
+    if (STAGE >= 3 && c.rstate(RS_CONT) == 0.0) return;   // it ended in its first pass
     if (STAGE == 1) c.setup_level1(k);
-    else c.setup_level2(k);
+    else if (STAGE == 2) c.setup_level2(k);
+    else if (STAGE == 3) c.setup_level_b1(k);
+    else c.setup_level_b2(k);
 }
 
 static __global__ void clear_count_kernel(int* count) {
@@ -138,18 +166,25 @@ static __global__ __launch_bounds__(256) void mark_scatter_kernel(const int* mar
 // The forward pass of columns [col0, col0 + n) (or of the mapped columns): the two
 // level-parallel set-up stages, then the column kernel.
 //
-// Large launches (at least two wavefronts per SIMD) run the column pass in three
-// launches (two where a column's working set is large, without step 1's list), so
-// that wavefronts are not held by a few of their lanes:
-//  1. every column runs the environment, CIN and trigger stages; a column that
-//     reaches the cloud-base closure saves its scalars and lists itself (list 1);
-//  2. the listed (triggered) columns, packed densely, run the convection stages of
-//     the first pass; a column whose implicit-CIN pass convects a second time (a few
-//     per cent) saves its scalars there and lists itself again (list 2);
+// Large launches (at least two wavefronts per SIMD) split the column pass, so that
+// wavefronts are not held by a few of their lanes and each kernel holds only the live
+// state of its own stages (the column code as one kernel needs about 500 registers per
+// lane and spills 1.7 KB of them at two wavefronts per SIMD):
+//  1. every column runs the environment, CIN and trigger stages (PART_CLOSURE); a
+//     column that reaches the cloud-base closure saves its scalars and marks itself,
+//     and the marks are compacted into list 1 in column order;
+//  2. the listed (triggered) columns, packed densely, run the rest of the first pass
+//     in four launches: the buoyancy sorting (PART_SORT), the stages from the forcedCu
+//     filter through the in-cloud diagnostics (PART_REST), the set-B environment level
+//     by level (setup stages 3 and 4), and the implicit-CIN iteration up to its CIN
+//     (PART_CIN2), where a column that convects a second time (a few per cent) saves
+//     its scalars and marks itself for list 2;
 //  3. the columns of list 2 run their second convection pass.
-// Each column keeps its workspace slot throughout.  Without the split, a wavefront
-// of a launch with 10 % convecting columns still runs the whole plume for the few
-// that convect, and nearly every wavefront carries one or two second passes.
+// Each column keeps its workspace slot throughout.  Where a column's working set is
+// large (72 levels x 10 constituents), step 1 runs the whole first pass instead and
+// step 2 is left out.  Without the split, a wavefront of a launch with 10 % convecting
+// columns still runs the whole plume for the few that convect, and nearly every
+// wavefront carries one or two second passes.
 //
 // args0.defer_list / defer_count / defer_state (nullable) give the lists' storage:
 // defer_count[0..1]; defer_list[0..3n + n/256 + 1): list 1, list 2, the marks of list 1
@@ -188,8 +223,12 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
         args.defer_mark = marks;
         args.defer_at_closure = closure_split ? 1 : 0;
     }
-    // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build
-    if (large)
+    // at least two wavefronts per SIMD of the 256-CU part: the occupancy-2 build, or the
+    // closure part alone
+    if (closure_split)
+        hipLaunchKernelGGL((forward_kernel_part<R, ZW, FAST, PART_CLOSURE, UWSHCU_OCC_CLOSURE>), dim3((n + 127) / 128),
+                           dim3(128), 0, stream, args, n, dconsts, destbl);
+    else if (large)
         hipLaunchKernelGGL((forward_kernel_occ2<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
     else
         hipLaunchKernelGGL((forward_kernel<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, args, n, dconsts, destbl);
@@ -204,20 +243,34 @@ hipError_t launch_column_pipeline(const FwdArgs& args0, int n, const uwshcu_cons
         if (closure_split) {
             compact(defer_list, defer_count);
             hipLaunchKernelGGL(fill_zero_kernel, dim3(nblk), dim3(256), 0, stream, marks, n);
-            // 2. the triggered columns from the closure of their first pass; up to all n
-            // of them, so the occupancy-2 build
+            // 2. the triggered columns, packed: (a) from the closure through the buoyancy
+            // sorting, (b) from there through the in-cloud diagnostics, (c) the set-B
+            // environment, level-parallel, (d) the implicit-CIN iteration's CIN, marking
+            // the columns that convect a second time.  Each part holds only its own live
+            // state (the saved scalars carry the rest).
             b.env_ready = 1;
             b.resume = 1;
             b.slot0 = args0.col0;
             b.defer_at_closure = 0;
-            b.defer_mark = marks;
+            b.defer_mark = nullptr;
             b.defer_list = nullptr;
             b.defer_count = nullptr;
             b.colmap = defer_list;
             b.colcount = defer_count;
             b.col0 = 0;
-            hipLaunchKernelGGL((forward_kernel_occ2<R, ZW, FAST>), dim3((n + 127) / 128), dim3(128), 0, stream, b, n, dconsts,
-                               destbl);
+            hipLaunchKernelGGL((forward_kernel_part<R, ZW, FAST, PART_SORT, UWSHCU_OCC_SORT>), dim3((n + 127) / 128),
+                               dim3(128), 0, stream, b, n, dconsts, destbl);
+            b.resume = 3;
+            hipLaunchKernelGGL((forward_kernel_part<R, ZW, FAST, PART_REST, UWSHCU_OCC_REST>), dim3((n + 127) / 128),
+                               dim3(128), 0, stream, b, n, dconsts, destbl);
+            hipLaunchKernelGGL((setup_kernel<R, 3, ZW, FAST>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream,
+                               b, n, dconsts, destbl);
+            hipLaunchKernelGGL((setup_kernel<R, 4, ZW, FAST>), dim3((unsigned)((t2 + 255) / 256)), dim3(256), 0, stream,
+                               b, n, dconsts, destbl);
+            b.resume = 4;
+            b.defer_mark = marks;
+            hipLaunchKernelGGL((forward_kernel_part<R, ZW, FAST, PART_CIN2, UWSHCU_OCC_CIN2>), dim3((n + 127) / 128),
+                               dim3(128), 0, stream, b, n, dconsts, destbl);
         } else {
             // the first launch marked the columns that need a second convection pass
             b.env_ready = 1;

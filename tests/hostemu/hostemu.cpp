@@ -60,22 +60,43 @@ static void run_columns(FwdArgs& a, const ThermoT<double, FAST>& T, int ncol, in
         a.defer_state = state.data();
         a.defer_at_closure = 1;
     }
+    if (g_defer) {
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < ncol; ++i) Col(a, T, i, i).run();
+        for (int i = 0; i < ncol; ++i) Col(a, T, i, i).template run_part<PART_CLOSURE>();
+    } else {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int i = 0; i < ncol; ++i) Col(a, T, i, i).run();
+    }
     a.env_ready = 0;
     a.defer_list = nullptr;
     a.defer_count = nullptr;
     a.defer_at_closure = 0;
     if (g_defer) {
         // continue the listed columns in their own slots, whose set-A environment the
-        // first launch (or the level-parallel set-up) left in place
+        // first launch (or the level-parallel set-up) left in place: the buoyancy
+        // sorting, then the rest of the first pass, listing second passes
         a.env_ready = 1;
         a.slot0 = 0;
         a.resume = 1;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < nlisted; ++j) Col(a, T, listed[j], listed[j]).template run_part<PART_SORT>();
+        a.resume = 3;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < nlisted; ++j) Col(a, T, listed[j], listed[j]).template run_part<PART_REST>();
+        // the set-B environment level by level, then the implicit-CIN iteration, listing
+        // second passes
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < nlisted; ++j) {
+            const Col c(a, T, listed[j], listed[j]);
+            if (c.rstate(RS_CONT) == 0.0) continue;
+            for (int k = 1; k <= pver; ++k) c.setup_level_b1(k);
+            for (int k = 1; k <= pver; ++k) c.setup_level_b2(k);
+        }
+        a.resume = 4;
         a.defer_list = second.data();
         a.defer_count = &nsecond;
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int j = 0; j < nlisted; ++j) Col(a, T, listed[j], listed[j]).run();
+        for (int j = 0; j < nlisted; ++j) Col(a, T, listed[j], listed[j]).template run_part<PART_CIN2>();
         a.defer_list = nullptr;
         a.defer_count = nullptr;
         a.resume = 2;
