@@ -225,7 +225,7 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
             ctx->d_defer_state = nullptr;
             ctx->defer_cols = 0;
         }
-        if (!hip_ok(hipMalloc((void**)&ctx->d_defer, (3 * (size_t)nb + (size_t)nb / 256 + 8) * sizeof(int))))
+        if (!hip_ok(hipMalloc((void**)&ctx->d_defer, (2 + uwshcu::defer_list_ints((size_t)nb)) * sizeof(int))))
             return UWSHCU_ERR_NOMEM;
         if (!hip_ok(hipMalloc((void**)&ctx->d_defer_state, (size_t)uwshcu::NRESUME * ((size_t)nb + 1) * sizeof(double)))) {
             (void)hipFree(ctx->d_defer);
@@ -292,6 +292,7 @@ int run_device(uwshcu_ctx* ctx, int precision, const uwshcu_params* par, int pco
     a.cush_in = ctx->d_cush_in;
     a.colmap = ctx->d_list;
     a.colcount = ctx->d_count;
+    a.sparse_map = 1;
     a.wld = nb64;
     for (long long c0 = 0; c0 < ncol; c0 += nb64) {
         const int n = (int)((ncol - c0) < nb64 ? (ncol - c0) : nb64);

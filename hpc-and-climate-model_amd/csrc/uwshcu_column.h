@@ -106,6 +106,9 @@ struct FwdArgs {
     // launch (column - slot0), whose set-A environment is intact.
     int resume;
     int slot0;
+    // 1: colmap lists a few columns of a large range (the mixed mode's FP64 recompute):
+    // run them with the one-wavefront build, unsplit
+    int sparse_map;
     AdArgs ad;               // adjoint kernel only
     // the modified scheme (Column<..., true>): its options, layer omega (caller layout)
     // and the per-column perturbations tpert and qpert
@@ -138,6 +141,17 @@ UWSHCU_HD inline int wave_max_level(int k) {
     return k;
 #endif
 }
+
+// Parts (bit PART_*) whose level loops align their lanes (wave_min_level).  Alignment
+// makes a wavefront's accesses coalesce, but the lanes idle until their own first level.
+// Measured at 1 036 800 x 35 x 3: aligned, the buoyancy sorting moves 29 GB instead of
+// 49 GB but takes 0.7 ms longer, and the one-wavefront-per-SIMD build loses 10 % at
+// 32768 columns; the stages after the buoyancy sorting, which wait on memory most of the
+// time, gain.  Fastest: those alone (51.4 ms against 52.0 ms unaligned, 52.1 ms aligned
+// in every split part).
+#ifndef UWSHCU_ALIGN_PARTS
+#define UWSHCU_ALIGN_PARTS (1 << PART_REST)
+#endif
 
 // zmh_ramp (mzfunctions_mod.F90:222-248): 0 below x0, 1 above x0 + xint, linear in
 // between; the floor on xint is the single-precision literal 1.e-20 there.
@@ -172,6 +186,9 @@ UWSHCU_HD inline int ad_inputs(int mkx) { return ADI_PARAMS * (mkx + 1) + 3; }
 // first-pass-rest parts of a split launch: 1 where the column continues in the next
 // part, 0 where it ended there.
 enum { NRESUME = 57, RS_CONT = 56 };
+
+// Ints of a split launch's list storage for n columns (launch_column_pipeline).
+UWSHCU_HD inline size_t defer_list_ints(size_t n) { return 3 * n + (n + 255) / 256 + 8; }
 
 // Parts of the column pass that the split pipeline runs as launches of their own.
 // PART_ALL runs whatever FwdArgs::resume asks for; the others fix it at compile time,
@@ -568,6 +585,11 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
     if constexpr (PART == PART_REST || PART == PART_CIN2) {
         if (rstate(RS_CONT) == 0.0) return;      // the column ended in the previous part
     }
+    // level loops with per-column first levels: level-aligned lanes (wave_min_level) in
+    // the parts of UWSHCU_ALIGN_PARTS (bit PART), each lane's own first level elsewhere
+    constexpr bool kAlign = ((UWSHCU_ALIGN_PARTS) >> PART) & 1;
+    auto lvl_lo = [](int k) { return kAlign ? wave_min_level(k) : k; };
+    auto lvl_hi = [](int k) { return kAlign ? wave_max_level(k) : k; };
     // uwshcu.F90:1040-1068 parameters
     const R rmaxfrac = R(0.10), rbuoy = R(1), rdrag = R(1);
     const R PGFc = R(0.7), criqc = R(0.7e-3), frc_rasn = R(1), kevp = R(2.e-6);
@@ -698,15 +720,17 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
     }
 
     for (int iter = resume ? (kSplit ? (resume == 3 ? 1 : resume == 4 ? 2 : resume) : 2) : 1; iter <= 2; ++iter) {
+        // this pass continues after its buoyancy sorting
+        const bool after_sort = resume == 3 && iter == 1;
         exit_iter = iter;
         const int eb = (iter == 1) ? A_THL0 : B_THL0;        // environment set for S1-S6
         const int e_thl = eb, e_qt = eb + 1, e_ssthl = eb + 2, e_ssqt = eb + 3;
         const int e_thvb = (iter == 1) ? A_THV0BOT : B_THV0BOT, e_thvt = e_thvb + 1;
         const int e_thvlb = (iter == 1) ? A_THVL0BOT : B_THVL0BOT, e_thvlt = e_thvlb + 1;
         const D tscaleh = cush;
-        if (!(resume == 3 && iter == 1)) cush = -1.0;   // continued after S11: its cush stands
+        if (!after_sort) cush = -1.0;   // continued after S11: its cush stands
 
-        if (kSplit ? (resume == iter || (resume == 3 && iter == 1)) : resume != 0) {
+        if (kSplit ? (resume == iter || after_sort) : resume != 0) {
             // a deferred column continues with the scalars it saved: at the second pass
             // those of its implicit-CIN restore, whose convection starts afresh
             if (!kSplit || iter == 2) zero_pass_state();
@@ -766,7 +790,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         klfc = mkx;
         if (klcl >= kinv) {
             D thvubot = 0.0, thvutop = 0.0;
-            for (int k = wave_min_level(kinv); k <= mkx - 1; ++k) {
+            for (int k = lvl_lo(kinv); k <= mkx - 1; ++k) {
                 if (k < kinv) continue;
                 if (k < klcl) {
                     thvubot = thvlsrc;
@@ -793,7 +817,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             }
         } else {
             cinlcl = 0.0;
-            for (int k = wave_min_level(kinv); k <= mkx - 1; ++k) {
+            for (int k = lvl_lo(kinv); k <= mkx - 1; ++k) {
                 if (k < kinv) continue;
                 if (T.conden_x(in(a.ps0, k - 1), E(A_EXNS0, k - 1), thlsrc, qtsrc, th, qvj, qlj, qij)) { exit_code = EX_CONDEN; goto L333; }
                 const D thvubot = th * (1.0 + C.zvir * qvj - qlj - qij);
@@ -846,7 +870,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         {
         R wtw;   // set by the closure, or at the start of the buoyancy sorting
         // a column continued at its first pass (resume 1) resumes at the buoyancy sorting
-        if (kSplit ? !((resume == 1 || resume == 3) && iter == 1) : true) {
+        if (kSplit ? !((resume == 1 && iter == 1) || after_sort) : true) {
         if constexpr (PART == PART_CLOSURE) {
             zero_pass_state();
             for (int k = 0; k <= mkx; ++k) { out(a.flxprc1, k) = 0.0; out(a.flxsnow1, k) = 0.0; }
@@ -960,7 +984,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         }   // S8-S10
 
         // ---- S11 buoyancy sorting (uwshcu.F90:2296-2952) ----
-        if (!(kSplit && resume == 3 && iter == 1)) {   // a column continued after S11 skips it
+        if (!(kSplit && after_sort)) {   // a column continued after S11 skips it
         R scaleh = R(tscaleh);
         if (R(tscaleh) < R(0)) scaleh = R(1000);
         for (int iter_scaleh = 1; iter_scaleh <= 3; ++iter_scaleh) {
@@ -986,7 +1010,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             R c_vu = W(W_VU, krel - 1), c_thvu = W(W_THVU, krel - 1);
             R c_wu = R(out(a.wu, krel - 1)), c_umf = R(out(a.umf, krel - 1));
 
-            for (int k = wave_min_level(krel); k <= mkx - 1; ++k) {
+            for (int k = lvl_lo(krel); k <= mkx - 1; ++k) {
                 if (k < krel) continue;    // level-aligned lanes (wave_min_level)
                 const int km1 = k - 1;
                 // Read once for both iter_xc passes: the environment of layer k, which the loop
@@ -1305,7 +1329,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
 
         // ---- S12 forcedCu filter (uwshcu.F90:2964-2992) ----
         if (kbup == krel) { exit_code = EX_CUFILTER; goto L333; }
-        for (int k = wave_min_level(kpen); k <= mkx; ++k) {
+        for (int k = lvl_lo(kpen); k <= mkx; ++k) {
             if (k < kpen) continue;
             out(a.umf, k) = 0.0; W(W_EMF, k) = R(0); W(W_UFRC, k) = R(0);
             if (k == kpen) continue;
@@ -1320,7 +1344,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             W(W_VUE, k) = W(W_VU, k);
             for (int m = 1; m <= ncnst; ++m) WT(2, m, k) = WT(1, m, k);
         }
-        for (int k = wave_max_level(kpen - 1); k >= kbup; --k) {
+        for (int k = lvl_hi(kpen - 1); k >= kbup; --k) {
             if (k > kpen - 1) continue;
             const R rhos0j = R(in(a.ps0, k)) / (r * R(0.5) * (Er(A_THV0BOT, k + 1) + Er(A_THV0TOP, k)) * Er(A_EXNS0, k));
             const R umfk = R(out(a.umf, k));
@@ -1406,7 +1430,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         }
         {
             R uplus = R(0), vplus = R(0);
-            for (int k = wave_min_level(kinv); k <= krel - 1; ++k) {
+            for (int k = lvl_lo(kinv); k <= krel - 1; ++k) {
                 if (k < kinv) continue;
                 const int kp1 = k + 1;
                 const R d = R(in(a.ps0, k) - in(a.p0, kp1));
@@ -1420,7 +1444,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
                     WT(3, m, k) = cbmf * (R(trin(m, 1)) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
             }
         }
-        for (int k = wave_min_level(krel); k <= kbup - 1; ++k) {
+        for (int k = lvl_lo(krel); k <= kbup - 1; ++k) {
             if (k < krel) continue;
             const int kp1 = k + 1;
             const R d = R(in(a.ps0, k) - in(a.p0, kp1));
@@ -1432,7 +1456,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
             for (int m = 1; m <= ncnst; ++m)
                 WT(3, m, k) = umfk * (WT(1, m, k) - (R(trin(m, kp1)) + WT(0, m, kp1) * d));
         }
-        for (int k = wave_min_level(kbup); k <= kpen - 1; ++k) {
+        for (int k = lvl_lo(kbup); k <= kpen - 1; ++k) {
             if (k < kbup) continue;
             const R d = R(in(a.ps0, k) - in(a.p0, k));
             const R emfk = W(W_EMF, k);
@@ -1771,7 +1795,7 @@ UWSHCU_HD __attribute__((always_inline)) void Column<R, D, ZW, FAST>::run_body()
         {
             // the tendency stage adjusted the same updraft states: reuse its condensate
             R qcubelow = qlr_base + qir_base, qlubelow = qlr_base, qiubelow = qir_base;
-            for (int k = wave_min_level(krel); k <= kpen; ++k) {
+            for (int k = lvl_lo(krel); k <= kpen; ++k) {
                 if (k < krel) continue;
                 const R qlr = W(W_QLR, k), qir = W(W_QIR, k);
                 R qcuk = R(0.5) * (qcubelow + qlr + qir);
