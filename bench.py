@@ -1,32 +1,44 @@
 """Benchmark: UW shallow-cumulus columns/s on one MI355X.
 
-Headline workload: the FP64 forward of the column physics on a 32 768-column chunk
-(a 256x128 grid) of 35 levels and 3 constituents, inputs resident in HBM.  One
-step = one call of the forward path over the whole chunk.  Timing: W untimed
-warm-up steps, then K steps between two synchronisations, with HIP events
-recorded on the stream the kernels run on.
+Headline workload: the forward of the column physics on a 1 036 800-column chunk (a
+1440x720 grid) of 35 levels and 3 constituents with half the columns convecting,
+inputs resident in HBM, timed in FP64 (UWSHCU_FP64, the reference's arithmetic bit for
+bit) and, in the same run, in the mixed-precision mode (UWSHCU_MIXED: the fast FP64
+pass whose saturation adjustment is solved to convergence, the conservation/branch
+check and the FP64 recompute of the columns it lists).  `value` is the FP64 forward's
+columns/s: it is exact and, on this chip, the faster of the two.  The line also
+carries the mixed mode's columns/s, its speed relative to FP64 and its error per
+output against the FP64 path (all columns) and against the oracle (sample).
+One step = one call of the forward over the whole chunk.  Timing: W untimed warm-up
+steps, then K steps between two synchronisations, with HIP events recorded on the
+stream the kernels run on.
 
 Other modes (same JSON contract; metric and config name the mode):
-  --precision mixed   the mixed-precision forward (FP32 plume arithmetic, FP64
-                      saturation/CIN/closure/fix-ups, FP64 recompute of columns that
-                      fail the conservation or branch check); reports its error against
-                      the FP64 path on the same chunk and the recomputed fraction.
-                      The 1 036 800-column workload: --ncol 1036800.
+  --no-mixed          the FP64 forward alone (the 32 768-column chunk: --ncol 32768
+                      --inversion-frac 0.25 --no-mixed).
+  --precision mixed   time the mixed mode as the line's mode (and FP64 beside it
+                      unless --no-mixed).
   --adjoint           forward + adjoint (gradients of the calibration loss
                       sum (sten/cp - tau_T)^2 + (qvten - tau_q)^2 with respect to t0,
                       qv0, u0, v0, tke, pblh, rpen, rle, rkm); one step = FP64 forward
                       plus the adjoint.  The 65 536-column workload: --ncol 65536.
 
 The JSON line also carries:
-  roofline      HBM view of the forward kernel: algorithmic bytes per launch (the
-                FP64 inputs the physics reads plus the full output set, per column
-                8*[9*(pver+1) + 24*pver + 2*ncnst*pver + 9] bytes) over the
-                kernel's average duration, against 8 TB/s;
+  roofline      the binding axis: FP64 vector ALU.  `achieved` = the reference
+                algorithm's FP64 operations (add, mul, div, sqrt; the transcendental
+                functions as the operations of their portable evaluation), counted per
+                column by the instrumented oracle on a random sample of the chunk, times
+                ncol, over the timed step; peak 78.6 TFLOP/s (MI355X FP64 vector, which
+                counts a fused multiply-add as two).  `hbm` beside it: the algorithmic
+                bytes per column (the FP64 inputs the physics reads plus the full output
+                set, 8*[9*(pver+1) + 24*pver + 2*ncnst*pver + 9]) over the same time,
+                against 8 TB/s, and the measured traffic (rocprofv3, profiles/);
   cpu_baseline  the CPU oracle (FP64 C restatement of the reference with the host
                 libm, OpenMP over columns) on a bounded sample of the same chunk;
-  error_vs_oracle  on that sample: whether the GPU outputs are bit-identical to the
-                portable-math oracle, and the per-column relative error against the
-                host-libm oracle (the tendency error the metric is stated at).
+  error_vs_oracle  on that sample: the FP64 path's bit-identity to the portable-math
+                oracle and the mixed mode's error against it;
+  transfers     host-to-device copy of the inputs and device-to-host copy of the
+                outputs, timed separately (they are not part of `value`).
 """
 from __future__ import annotations
 
@@ -41,22 +53,28 @@ sys.path.insert(0, os.path.join(ROOT, "hpc-and-climate-model_amd"))
 
 METRIC = "UWshcu columns/s (FP64 vs mixed; fwd & fwd+adjoint) at stated tendency error"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--ncol", type=int, default=32768)
+    ap.add_argument("--ncol", type=int, default=1036800)
     ap.add_argument("--pver", type=int, default=35)
     ap.add_argument("--ncnst", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="the mode `value` times; the other is timed beside it unless --no-mixed")
+    ap.add_argument("--no-mixed", action="store_true", help="time the --precision mode alone")
     ap.add_argument("--scheme", choices=["uw", "zw"], default="uw",
                     help="uw: compute_uwshcu_inv; zw: the modified scheme compute_zwshcu_inv (FP64, default options)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--inversion-frac", type=float, default=0.25)
-    ap.add_argument("--cpu-sample", type=int, default=8192, help="columns timed on the CPU oracle")
+    ap.add_argument("--inversion-frac", type=float, default=0.48,
+                    help="fraction of columns capped by an inversion (0.48: half the columns convect)")
+    ap.add_argument("--cpu-sample", type=int, default=262144, help="columns timed on the CPU oracle")
+    ap.add_argument("--op-sample", type=int, default=4096,
+                    help="columns whose FP64 operations the instrumented oracle counts")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", action="store_true", help="time forward + adjoint")
@@ -220,30 +238,50 @@ def main():
     cols_per_s = args.ncol / (ms * 1e-3)
     diag = out["diag"].cpu().numpy()
     conv_frac = float((diag[7] > 0).mean())
-    if prec == _capi.MIXED and not args.adjoint:
-        nfix = ctx.mixed_recomputed(stream)
-        mix = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
-        mix["cush"] = cush.cpu().numpy()
-        f64_ms, _ = timed(lambda: fwd(_capi.FP64), args.steps, 1, stream)
-        extra["fp64_forward_ms_same_chunk"] = float(np.mean(f64_ms))
-        extra["speedup_vs_fp64"] = float(np.mean(f64_ms)) / ms
-        f64 = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
-        f64["cush"] = cush.cpu().numpy()
-        raw_ms, _ = timed(lambda: fwd(_capi.MIXED_UNCHECKED), args.steps, 1, stream)
-        raw = {k: v.cpu().numpy() for k, v in out.items() if k != "diag"}
-        raw["cush"] = cush.cpu().numpy()
-        extra["fp32_pass_alone_ms"] = float(np.mean(raw_ms))
-        extra["fp32_pass_alone"] = mixed_error(raw, f64, 0, args.ncol)
-        fwd(_capi.MIXED)          # leave the mixed result in place
-        extra["mixed"] = mixed_error(mix, f64, nfix, args.ncol)
+    mix_out = None
+    compare = not args.adjoint and args.chunks == 1 and not zw and not args.no_mixed
+    if compare:
+        # the other precision mode on the same chunk, then the mixed mode's error against FP64
+        main_out = {k: v.clone() for k, v in out.items()}
+        main_out["cush"] = cush.clone()
+        nfix = ctx.mixed_recomputed(stream) if prec == _capi.MIXED else None
+        other = _capi.FP64 if prec == _capi.MIXED else _capi.MIXED
+        o_ms, _ = timed(lambda: fwd(other), args.steps, args.warmup, stream)
+        rc, bad = ctx.status(stream)
+        if rc:
+            raise _capi.UwshcuError(rc, bad)
+        other_out = {k: v.clone() for k, v in out.items()}
+        other_out["cush"] = cush.clone()
+        if nfix is None:
+            nfix = ctx.mixed_recomputed(stream)
+        oms = float(np.mean(o_ms))
+        f64_out, mix_out = (main_out, other_out) if prec == _capi.FP64 else (other_out, main_out)
+        f64_ms, mix_ms = (ms, oms) if prec == _capi.FP64 else (oms, ms)
+        extra["fp64_forward"] = {"ms_per_step": f64_ms, "columns_per_s": args.ncol / (f64_ms * 1e-3)}
+        extra["mixed_forward"] = {"ms_per_step": mix_ms, "columns_per_s": args.ncol / (mix_ms * 1e-3),
+                                  "step_ms_median": float(np.median(o_ms if prec == _capi.FP64 else per_step_ms))}
+        extra["mixed_speedup_vs_fp64"] = f64_ms / mix_ms
+        extra["mixed"] = mixed_error(mix_out, f64_out, nfix, args.ncol)
+        # the FP32-plume pass, kept as an experiment: its speed and error on the same chunk
+        raw_ms, _ = timed(lambda: fwd(_capi.MIXED_UNCHECKED), max(2, args.steps // 2), 1, stream)
+        raw = dict(out)
+        raw["cush"] = cush
+        e = mixed_error(raw, mix_out, 0, args.ncol, brief=True)
+        extra["fp32_plume_pass_experimental"] = {"ms_per_step": float(np.mean(raw_ms)),
+                                                 "tendency_rel_l2_vs_mixed": e["tendency_rel_l2"]}
+        # leave the FP64 result in place for the oracle comparison
+        for k, v in f64_out.items():
+            (cush if k == "cush" else out[k]).copy_(v)
+        torch.cuda.synchronize()
     bpc = algorithmic_bytes_per_column(args.pver, args.ncnst)
-    achieved = bpc * args.ncol / (ms * 1e-3) / 1e9
+    achieved_gbs = bpc * args.ncol / (ms * 1e-3) / 1e9
     mode = "fp64 forward" if prec == _capi.FP64 else "mixed forward"
     if args.adjoint:
         mode = "fp64 forward + adjoint"
     if args.chunks > 1:
         mode += f" over {args.chunks} chunks in one call"
     scheme = "compute_zwshcu_inv (default options, rpen 15, rkm 10)" if zw else "compute_uwshcu_inv"
+    extra["transfers"] = transfers(ch, out, args.tile)
     res = {
         "metric": METRIC,
         "value": cols_per_s,
@@ -255,7 +293,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if (prec == _capi.FP64 or args.adjoint) else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic" if args.tile == 1 else f"synthetic, {args.ncol // args.tile} distinct columns tiled {args.tile}x",
         "config": {
             "workload": f"{mode} of {scheme}, {args.ncol} columns x {args.pver} levels x "
@@ -264,23 +302,23 @@ def main():
             "convecting_fraction": conv_frac, "inversion_frac": args.inversion_frac, "seed": args.seed,
             "workspace_gib": ws_gib if ws_gib else 24.0,
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(mode, args),
-            "algorithmic_bytes_per_launch": bpc * args.ncol, "algorithmic_bytes_per_column": bpc,
-            "timed": ("the forward pipeline (two level-parallel set-up kernels + the column kernel, which is 97 % "
-                      "of it in rocprofv3), HIP events on its stream") if args.chunks == 1 else
-                     "one batched call: table upload, gather, the forward pipeline and scatter, HIP events on its stream",
-            "note": "latency-bound FP64 VALU kernel; HBM view of the algorithmic I/O",
-        },
+        "roofline": None,
         "step_ms_min": float(np.min(per_step_ms)),
         "step_ms_median": float(np.median(per_step_ms)),
         "wall_s_timed": wall,
     }
     res.update(extra)
+    hbm = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(mode, args),
+           "algorithmic_bytes_per_launch": bpc * args.ncol, "algorithmic_bytes_per_column": bpc}
+    ops = None
     if not args.no_cpu:
-        res.update(cpu_leg(args, ch, out, cush))
+        leg = cpu_leg(args, ch, out, cush, mix_out)
+        ops = leg.pop("op_count", None)
+        res.update(leg)
+    res["roofline"] = roofline(ops, args.ncol, ms, hbm, args.chunks > 1)
     print(json.dumps(res), flush=True)
+
 
 
 def measured_traffic(mode, args):
@@ -296,23 +334,84 @@ def measured_traffic(mode, args):
     return ent["traffic_bytes"] if ok else None
 
 
-def mixed_error(mix, f64, nfix, ncol):
-    """Error of the mixed-precision outputs against the FP64 path on the same chunk."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import parity
-    worst = np.zeros(ncol)
-    for k in f64:
-        worst = np.maximum(worst, parity.column_rel_err(mix[k], f64[k]))
-    rel_l2 = {}
-    for k in ("qvten", "qlten", "qiten", "sten", "uten", "vten", "umf", "precip"):
-        den = float((f64[k] ** 2).sum())
-        rel_l2[k] = float(np.sqrt(((mix[k] - f64[k]) ** 2).sum() / den)) if den > 0 else 0.0
+ERR_FIELDS = ("sten", "qvten", "qlten", "qiten", "uten", "vten", "trten", "umf", "slflx", "qtflx", "qrten", "qsten",
+              "evapc", "precip", "snow", "cufrc")
+
+
+def _col_rel_err(a, b):
+    """Per-column max |a - b| / max |b| (floor 1e-12 of the field maximum), on the device."""
+    import torch
+    a2, b2 = a.reshape(-1, a.shape[-1]), b.reshape(-1, b.shape[-1])
+    floor = torch.clamp(b2.abs().max() * 1e-12, min=1e-300)
+    return (a2 - b2).abs().max(dim=0).values / torch.maximum(b2.abs().max(dim=0).values, floor)
+
+
+def mixed_error(mix, f64, nfix, ncol, brief=False):
+    """Error of the mixed-mode outputs against the FP64 path on the same chunk, computed on
+    the GPU over every column: relative L2 per output over the chunk (the bound is 1e-4),
+    the per-column error (worst output) distribution and the integer-diagnostic matches."""
+    import torch
+    l2 = {}
+    for k in ERR_FIELDS:
+        den = torch.linalg.vector_norm(f64[k])
+        l2[k] = float(torch.linalg.vector_norm(mix[k] - f64[k]) / den) if float(den) > 0 else 0.0
+    if brief:
+        return {"tendency_rel_l2": l2}
+    colw = torch.zeros(ncol, dtype=torch.float64, device=f64["sten"].device)
+    for k in ERR_FIELDS:
+        colw = torch.maximum(colw, _col_rel_err(mix[k], f64[k]))
+    samp = colw[:: max(1, ncol // 1000000)]
+    q = torch.quantile(samp[: 1 << 24], torch.tensor([0.5, 0.99, 0.999], dtype=torch.float64, device=samp.device))
     return {"recomputed_fp64_columns": int(nfix), "recomputed_fraction": nfix / ncol,
-            "tendency_rel_l2_vs_fp64": rel_l2,
-            "column_rel_err_vs_fp64": {"median": float(np.median(worst)), "p90": float(np.quantile(worst, 0.9)),
-                                       "p99": float(np.quantile(worst, 0.99)), "max": float(worst.max())},
-            "reference_harness_rms_vs_fp64": parity.reference_rms_errors(mix, f64)}
+            "tendency_rel_l2_vs_fp64": l2, "bound": 1e-4,
+            "max_rel_l2": max(l2.values()),
+            "column_rel_err_vs_fp64": {"median": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+                                       "max": float(colw.max()), "columns_above_1e-4": int((colw > 1e-4).sum())},
+            "integer_diag_mismatch_columns": int((mix["diag"] != f64["diag"]).any(dim=0).sum())}
+
+
+def transfers(ch, out, tile):
+    """Host-to-device copy of the chunk's inputs (from pageable numpy arrays) and
+    device-to-host copy of the outputs, each timed once after a warm copy."""
+    import torch
+    from uwshcu_amd import device
+    nbytes_in = sum(v.nbytes for v in ch.fields.values())
+    device.chunk_to_device(ch, "cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    device.chunk_to_device(ch, "cuda:0")
+    torch.cuda.synchronize()
+    h2d = time.perf_counter() - t0
+    nbytes_out = sum(v.numel() * v.element_size() for k, v in out.items() if k != "diag")
+    t0 = time.perf_counter()
+    for k, v in out.items():
+        if k != "diag":
+            v.cpu()
+    d2h = time.perf_counter() - t0
+    return {"h2d_ms": h2d * 1e3, "h2d_bytes": int(nbytes_in), "h2d_gbs": nbytes_in / h2d / 1e9,
+            "d2h_ms": d2h * 1e3, "d2h_bytes": int(nbytes_out), "d2h_gbs": nbytes_out / d2h / 1e9,
+            "note": "pageable host memory, not part of the timed step" +
+                    (f"; inputs are the {ch.ncol} distinct columns" if tile > 1 else "")}
+
+
+def roofline(ops, ncol, ms, hbm, chunked):
+    """The binding axis (FP64 vector ALU) from the counted operations, HBM beside it."""
+    r = {"bound": "valu", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
+         "traffic": hbm["traffic"],
+         "note": ("FP64 vector ALU, not MFMA: the column physics is scalar per column.  achieved = the reference "
+                  "algorithm's FP64 operations (add, mul, div, sqrt; transcendentals as the operations of their "
+                  "portable evaluation), counted by the instrumented oracle on a random sample of the chunk, times "
+                  "ncol, over the timed step.  The peak counts a fused multiply-add as two operations; divisions "
+                  "and square roots cost several instructions each.  traffic = measured HBM bytes per step "
+                  "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated), against the algorithmic bytes of hbm."),
+         "hbm": hbm}
+    if ops:
+        total = ops["mean_per_column"] * ncol
+        achieved = total / (ms * 1e-3) / 1e12
+        r.update(achieved=achieved, frac=achieved / FP64_PEAK_TFLOPS, ops_per_step=total, op_count=ops)
+    if chunked:
+        r["timed"] = "one batched call: table upload, gather, the forward pipeline and scatter"
+    return r
 
 
 def host_cpu():
@@ -329,8 +428,9 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count()}
 
 
-def cpu_leg(args, ch, out, cush):
-    """CPU baseline (the oracle on a bounded sample) and the GPU-vs-oracle error on it."""
+def cpu_leg(args, ch, out, cush, mix_out=None):
+    """The oracle leg: CPU baseline (the oracle on a bounded sample), the GPU results
+    against the oracle on that sample, and the FP64 operation count per column."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -355,35 +455,67 @@ def cpu_leg(args, ch, out, cush):
     t0 = time.perf_counter()
     ref, rdiag, rc, _ = orun(sub, nthr)                   # host libm: the faithful reading of the reference
     dt_cpu = time.perf_counter() - t0
-    n1 = min(2048, n)                                     # one thread, on a smaller prefix
+    n1 = min(4096, n)                                     # one thread, on a smaller prefix
     one = Chunk(n1, ch.pver, ch.ncnst, ch.dt, {k: np.ascontiguousarray(v[..., :n1]) for k, v in ch.fields.items()})
     t1 = time.perf_counter()
     orun(one, 1)
     dt_one = time.perf_counter() - t1
     pm, pmdiag, _, _ = orun(sub, nthr, portable=True)
-    got = {k: v.cpu().numpy()[..., :n] for k, v in out.items() if k != "diag"}
-    got["cush"] = cush.cpu().numpy()[:n]
-    diag = out["diag"].cpu().numpy()[:, :n].T
+    got = {k: v[..., :n].cpu().numpy() for k, v in out.items() if k != "diag"}
+    got["cush"] = cush[:n].cpu().numpy()
+    diag = out["diag"][:, :n].cpu().numpy().T
     errs = parity.compare(got, ref)
     worst_col = np.zeros(n)
     for k in ref:
         worst_col = np.maximum(worst_col, parity.column_rel_err(got[k], ref[k]))
     identical = all(np.array_equal(got[k], pm[k]) for k in pm) and np.array_equal(diag, pmdiag)
-    return {
+    res = {
         "cpu_baseline": {"value": n / dt_cpu, "unit": "columns/s", "cores": nthr, "kind": "port",
                          "sample": f"first {n} columns of the same chunk, FP64 C oracle (host libm), "
-                                   f"{nthr} threads over columns",
+                                   f"{nthr} threads over columns ({dt_cpu:.1f} s)",
                          "single_thread_value": n1 / dt_one, "single_thread_sample": f"first {n1} columns, 1 thread",
                          "host_cpu": host_cpu()},
         "error_vs_oracle": {
-            "bit_identical_to_portable_math_oracle": bool(identical),
-            "host_libm_oracle_max_col_rel_err": float(worst_col.max()),
-            "host_libm_oracle_cols_above_1e-10": int((worst_col > 1e-10).sum()),
-            "host_libm_oracle_worst_field": max(errs, key=errs.get),
+            "fp64_bit_identical_to_portable_math_oracle": bool(identical),
+            "fp64_vs_host_libm_oracle_max_col_rel_err": float(worst_col.max()),
+            "fp64_vs_host_libm_oracle_cols_above_1e-10": int((worst_col > 1e-10).sum()),
+            "fp64_vs_host_libm_oracle_worst_field": max(errs, key=errs.get),
             "integer_diag_mismatch_columns": int((diag != rdiag).any(axis=1).sum()),
             "sample_columns": n,
         },
     }
+    if mix_out is not None:
+        mx = {k: v[..., :n].cpu().numpy() for k, v in mix_out.items() if k not in ("diag",)}
+        l2 = {k: float(np.linalg.norm(mx[k] - pm[k]) / max(np.linalg.norm(pm[k]), 1e-300)) for k in ERR_FIELDS}
+        colw = np.zeros(n)
+        for k in ERR_FIELDS:
+            colw = np.maximum(colw, parity.column_rel_err(mx[k], pm[k]))
+        res["error_vs_oracle"]["mixed_vs_portable_oracle"] = {
+            "tendency_rel_l2": l2, "max_rel_l2": max(l2.values()),
+            "columns_above_1e-4": int((colw > 1e-4).sum()), "max_col_rel_err": float(colw.max()),
+            "integer_diag_mismatch_columns": int((mix_out["diag"][:, :n].cpu().numpy().T != pmdiag).any(axis=1).sum())}
+        # the FP64 reference's own sensitivity, for scale: host libm against portable math
+        hl = np.zeros(n)
+        for k in ERR_FIELDS:
+            hl = np.maximum(hl, parity.column_rel_err(ref[k], pm[k]))
+        res["error_vs_oracle"]["reference_libm_sensitivity"] = {
+            "note": "the FP64 oracle with the host libm against the portable-math oracle on the same sample",
+            "max_col_rel_err": float(hl.max()), "columns_above_1e-4": int((hl > 1e-4).sum())}
+    if args.scheme == "uw" and args.op_sample > 0:
+        rng = np.random.default_rng(args.seed + 17)
+        idx = np.sort(rng.choice(ch.ncol, min(args.op_sample, ch.ncol), replace=False))
+        ops, _, odiag = oracle.count_ops(Chunk(len(idx), ch.pver, ch.ncnst, ch.dt,
+                                               {k: np.ascontiguousarray(v[..., idx]) for k, v in ch.fields.items()}))
+        flops = ops[:, :4].sum(axis=1).astype(np.float64)
+        conv = odiag[:, 7] > 0
+        hist, edges = np.histogram(flops, bins=[0, 2e4, 5e4, 1e5, 2e5, 3e5, 4e5, 5e5, 7e5, 1e6, 1e9])
+        res["op_count"] = {
+            "mean_per_column": float(flops.mean()), "sample_columns": int(len(idx)),
+            "mean_convecting": float(flops[conv].mean()) if conv.any() else 0.0,
+            "mean_not_convecting": float(flops[~conv].mean()) if (~conv).any() else 0.0,
+            "by_kind_mean": {k: float(v) for k, v in zip(oracle.OP_KINDS, ops.mean(axis=0))},
+            "histogram": {"edges": [float(e) for e in edges], "counts": [int(c) for c in hist]}}
+    return res
 
 
 if __name__ == "__main__":
