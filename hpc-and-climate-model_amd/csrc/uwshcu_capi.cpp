@@ -316,13 +316,14 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     const int KS = pver + 1;
     const long long user_cap = ctx->tape_cap > 0 ? ctx->tape_cap : 0x7fffffffLL;
     // per-column fixed pieces: AD workspace, AD outputs, output nodes, tape header,
-    // offset and capacity; the tape and the sweep's adjoints take 32 bytes per node
+    // offset and capacity; a tape node (operands, partials and its adjoint) takes 32 bytes
     const size_t b_ws = ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD));
     const size_t b_out = (size_t)ad_fields(ncnst) * KS * sizeof(AD);
     const size_t b_nodes = (size_t)ad_fields(ncnst) * KS * sizeof(int);
     const size_t b_fix = b_ws + b_out + b_nodes + sizeof(TapeCtx) + sizeof(long long) + sizeof(int);
-    const size_t b_node = sizeof(TapeNode) + sizeof(double);
-    auto layout = [&](long long nb, size_t nodes, size_t* o) {   // o: out, nodes, tapes, off, cap, tape, adj, end
+    const size_t b_node = sizeof(TapeNode);
+    static_assert(sizeof(TapeNode) == 32, "tape node layout");
+    auto layout = [&](long long nb, size_t nodes, size_t* o) {   // o: out, nodes, tapes, off, cap, tape, end
         o[0] = align256(b_ws * nb);
         o[1] = o[0] + align256(b_out * nb);
         o[2] = o[1] + align256(b_nodes * nb);
@@ -330,7 +331,6 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
         o[4] = o[3] + align256(sizeof(long long) * nb);
         o[5] = o[4] + align256(sizeof(int) * nb);
         o[6] = o[5] + align256(sizeof(TapeNode) * nodes);
-        o[7] = o[6] + align256(sizeof(double) * nodes);
     };
     if ((size_t)ncol > ctx->count_cols) {
         if (ctx->d_counts) {
@@ -380,9 +380,9 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     long long nbc = (long long)(ctx->ws_limit / (b_fix + 64));
     if (nbc > ncol) nbc = ncol;
     if (nbc < 1) return UWSHCU_ERR_NOMEM;
-    size_t o[8];
+    size_t o[7];
     layout(nbc, 0, o);
-    int rc = ensure_ws(ctx, o[7]);
+    int rc = ensure_ws(ctx, o[6]);
     if (rc) return rc;
     char* base = (char*)ctx->d_ws;
     a.ws = base;
@@ -430,7 +430,7 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
         }
         batch_nodes.push_back(acc);
         layout(starts[b + 1] - starts[b], acc, o);
-        need = std::max(need, o[7]);
+        need = std::max(need, o[6]);
     }
     if ((rc = ensure_ws(ctx, need))) return rc;
     base = (char*)ctx->d_ws;
@@ -448,8 +448,8 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
                                    stream)))
             return UWSHCU_ERR_HIP;
         if (!hip_ok(launch_adjoint(a, (int)n, ctx->d_consts, ctx->d_estbl, (TapeNode*)(base + o[5]),
-                                   (double*)(base + o[6]), (const long long*)(base + o[3]),
-                                   (const int*)(base + o[4]), (int*)(base + o[1]), ct, g, stream)))
+                                   (const long long*)(base + o[3]), (const int*)(base + o[4]), (int*)(base + o[1]),
+                                   ct, g, stream)))
             return UWSHCU_ERR_HIP;
         // the host copies above read off/cap, which live until the next batch's copies
         if (b + 2 < starts.size() && !hip_ok(hipStreamSynchronize(stream))) return UWSHCU_ERR_HIP;

@@ -232,7 +232,7 @@ struct Column {
     R* __restrict__ ws;         // ws_arrays(ncnst) * KS rows of wld R values
     TapeCtx* tape;              // adjoint kernel: this column's tape
 
-    UWSHCU_HD Column(const FwdArgs& args, const ThermoT<D, FAST>& th, int col, int slot, TapeCtx* own_tape = nullptr)
+    UWSHCU_HD UW_INLINE Column(const FwdArgs& args, const ThermoT<D, FAST>& th, int col, int slot, TapeCtx* own_tape = nullptr)
         : a(args), T(th), C(*th.c), i(col), iw(slot), mkx(args.mkx), KS(args.mkx + 1), ncnst(args.ncnst),
           ldc((size_t)args.ldc), wld((size_t)args.wld), env(reinterpret_cast<D*>(args.ws)),
           ws(reinterpret_cast<R*>(reinterpret_cast<D*>(args.ws) + (size_t)ENV_ARRAYS * (size_t)(args.mkx + 1) *
@@ -550,7 +550,7 @@ struct Column {
 
     // Adjoint kernel, after run(): copy the recorded outputs' values to the caller's
     // arrays and their tape nodes to nodes[(f * (mkx+1) + k) * wld + slot].
-    UWSHCU_HD void ad_export(int* nodes) const {
+    UWSHCU_HD UW_INLINE void ad_export(int* nodes) const {
         if constexpr (kAD) {
             auto put = [&](int f, int k, double* dst) {
                 const D& v = adout(f, k);

@@ -28,11 +28,11 @@ hipError_t launch_check_columns(const FwdArgs& args, const uwshcu_consts* dconst
 hipError_t launch_adjoint_count(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                 int* counts, hipStream_t stream);
 // Adjoint over columns [args.col0, args.col0 + n): record each column's FP64 forward
-// pass on its tape (slot j: cap[j] nodes at tape_base + off[j], adjoint scratch at
-// adj_base + off[j]; args.ad holds the AD output values and the tape headers), write
-// the forward outputs, then sweep the tapes from the cotangents to the gradients.
+// pass on its tape (slot j: cap[j] nodes at tape_base + off[j], each node carrying its
+// adjoint; args.ad holds the AD output values and the tape headers), write the forward
+// outputs, then sweep the tapes from the cotangents to the gradients.
 hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                          TapeNode* tape_base, double* adj_base, const long long* off, const int* cap, int* node_out,
+                          TapeNode* tape_base, const long long* off, const int* cap, int* node_out,
                           const AdCotangents& cot, const AdGradients& g, hipStream_t stream);
 // sum[p] = sum over columns c < ncol of par[p*ldc + c], p = 0..2, in FP64.
 hipError_t launch_param_sum(const double* par, long long ldc, int ncol, double* sum, hipStream_t stream);

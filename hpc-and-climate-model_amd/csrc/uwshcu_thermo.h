@@ -294,7 +294,7 @@ struct ThermoT {
     }
 
     // uwshcu.F90:4863-4929
-    UWSHCU_HD D qsinvert(D qt, D thl, D psfc) const {
+    UWSHCU_HD __attribute__((always_inline)) D qsinvert(D qt, D thl, D psfc) const {
         const double psmin = 100.0 * 100.0, dpsmax = 1.0;
         D es, qs, gam;
         const D Ti = thl * upow(psfc / c->p00, c->rovcp);
@@ -377,7 +377,7 @@ UWSHCU_HD inline void roots(R a, R b, R c, R& r1, R& r2, int& status) {
 
 // uwshcu.F90:4931-4950
 template <class D>
-UWSHCU_HD inline D compute_alpha(D del_CIN, D ke) {
+UWSHCU_HD inline __attribute__((always_inline)) D compute_alpha(D del_CIN, D ke) {
     D x0 = 0.0;
     for (int it = 0; it < 10; ++it) {
         const D e = uexp(-x0 * ke * del_CIN);
@@ -388,7 +388,7 @@ UWSHCU_HD inline D compute_alpha(D del_CIN, D ke) {
 
 // uwshcu.F90:4952-4979 (erfc is the device libm's)
 template <class R>
-UWSHCU_HD inline R compute_mumin2(R mulcl, R rmaxfrac, R mulow) {
+UWSHCU_HD inline __attribute__((always_inline)) R compute_mumin2(R mulcl, R rmaxfrac, R mulow) {
     R x0 = mulow;
     for (int it = 0; it < 10; ++it) {
         const R ex = uexp(-(x0 * x0));

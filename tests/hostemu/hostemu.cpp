@@ -219,10 +219,10 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
     AdGradients g{grad[0], grad[1], grad[2], grad[3], grad[4], grad[5], grad[6]};
     const ThermoT<AD> T{c, estbl};
     const int nin = ad_inputs(pver);
+    if (g_window > 0 && g_window < kSweepBatch) return UWSHCU_ERR_ARG;   // the window holds a batch
 #pragma omp parallel
     {
         std::vector<TapeNode> buf((size_t)cap);
-        std::vector<double> adj((size_t)cap);
 #pragma omp for schedule(dynamic, 4)
         for (int i = 0; i < ncol; ++i) {
             ad_begin(tapes[i], buf.data(), cap, nin);
@@ -230,12 +230,12 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
             col.run();
             col.ad_export(nodes.data());
             if (g_window > 0) {
-                std::vector<double> wv((size_t)g_window);
+                std::vector<double> wv((size_t)g_window + 1);
                 auto win = [&](int q) -> double& { return wv[(size_t)q]; };
-                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g,
+                ad_reverse(tapes[i], nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g,
                            win, g_window);
             } else {
-                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
+                ad_reverse(tapes[i], nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
             }
             tape_len[i] = tapes[i].overflow ? -1 : tapes[i].n;
         }

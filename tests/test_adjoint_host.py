@@ -114,10 +114,11 @@ def test_tape_overflow_is_reported():
     assert (tlen[diag[:, 7] > 0] == -1).all()
 
 
-@pytest.mark.parametrize("window", [1, 7, 64])
+@pytest.mark.parametrize("window", [16, 23, 64])
 def test_windowed_sweep_is_bit_identical(window):
-    """The device sweep keeps recent adjoints in a window; each adjoint still sums its
-    contributions in decreasing node order, so the gradients equal the plain sweep's."""
+    """The device sweep keeps recent adjoints in a window (at least one batch of 16 nodes
+    long); each adjoint still sums its contributions in decreasing node order, so the
+    gradients equal the plain sweep's."""
     ch = make_chunk(256, 35, 3, seed=47)
     ref, _, _, _ = oracle.run(ch, nthreads=8, portable=True)
     rng = np.random.default_rng(5)
