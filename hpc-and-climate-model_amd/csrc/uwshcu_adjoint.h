@@ -13,21 +13,39 @@
 
 namespace uwshcu {
 
-UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int cap, int nin, int stride = 1) {
-    t.nodes = nodes;
+// Start a column's tape: contiguous in nodes (cap nodes; nullptr = count only), or,
+// with pages != nullptr, paged from shard `shard` of pool with the column's page table
+// in pages.  The first nin nodes are the leaves (the independent variables).
+UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int* pages, TapePool* pool, int shard, int cap, int nin) {
+    t.pages = pages;
+    t.pool = pool;
+    t.adj = 0;
     t.cap = cap;
-    t.stride = stride;
-    t.overflow = nin > cap ? 1 : 0;
-    t.n = nin < cap ? nin : cap;
-    if (!nodes) return;
-    for (int q = 0; q < t.n; ++q) {
-        TapeNode nd;
-        nd.da = 0.0;
-        nd.db = 0.0;
-        nd.a = -1;
-        nd.b = -1;
-        nd.adj = 0.0;
-        ((UWSHCU_GMEM TapeNode*)nodes)[tape_slot(t, q)] = nd;
+    t.overflow = 0;
+    t.pad = 0;
+    t.n = 0;
+    if (pages) {
+        t.nodes = nullptr;
+        t.limit = 0;
+        t.ctr = &pool->next[shard * 32];
+        t.shard_base = shard * pool->per_shard;
+    } else {
+        t.nodes = nodes ? nodes + (cap - 1) : nullptr;
+        t.limit = cap;
+        t.ctr = nullptr;
+        t.shard_base = 0;
+    }
+    for (int q = 0; q < nin; ++q) {
+        if (q >= t.limit && !tape_grow(t, q)) break;
+        t.n = q + 1;
+        if (TapeNode* top = t.nodes) {
+            TapeNode nd;
+            nd.da = 0.0;
+            nd.db = 0.0;
+            nd.a = -1;
+            nd.b = -1;
+            *(UWSHCU_GMEM TapeNode*)(top - q) = nd;
+        }
     }
 }
 
@@ -66,39 +84,50 @@ struct AdGradients {
 };
 
 // Reverse sweep of one column: seed the outputs' adjoints from the cotangents,
-// propagate back through the tape, write the input gradients.  Node q's adjoint is the
-// adj field of its tape node (zero when recorded).
+// propagate back through the tape, write the input gradients.  adjv[q] is node q's
+// adjoint (n entries, zero on entry).
 //
 // With a window of S >= kSweepBatch entries (win(q % S), and a spare entry win(S)), the
 // adjoints of recent nodes are kept in it: 82 % of the operands a node refers to lie
 // within 64 nodes of it (35-level columns).  The sweep works kSweepBatch (K) nodes at a
-// time; when a batch starts at node q0 the window holds the adjoints of nodes
-// q0 - S + 1 .. q0.  A contribution to one of those goes to the window, one to an
-// older node to the tape (a far contribution).  When the batch ends, its K nodes leave
-// the window and nodes q0 - S - K + 1 .. q0 - S take their entries, carrying what far
-// contributions they received.  Every adjoint still receives its contributions in
-// decreasing node order, so the result is the same, bit for bit, for every S, and as
-// without a window.
+// time, in batches aligned to K (so that a batch never straddles a tape page); when a
+// batch starts at node q0 the window holds the adjoints of nodes q0 - S + 1 .. q0.  A
+// contribution to one of those goes to the window, one to an older node to adjv (a far
+// contribution).  When the batch ends, its K nodes leave the window and nodes
+// q0 - S - K + 1 .. q0 - S take their entries, carrying what far contributions they
+// received.  Every adjoint still receives its contributions in decreasing node order,
+// so the result is the same, bit for bit, for every S, and as without a window.
 //
 // The memory operations of a batch are issued together at its end: its far
 // contributions (no-return atomic adds, in step order), then the loads of the adjoints
 // that enter the window and of the next batch's tape nodes (indices clamped into the
-// tape), so that one memory latency is paid per batch and no step waits on memory.
-// (A far contribution can wait to the batch's end: its node enters the window at a
-// later batch's end.)  Within a step, the node's
-// adjoint and its operands' window entries are read together and written back
-// unconditionally (an operand outside the window uses the spare entry), so that a
-// step waits for the window once.
+// batch's page), so that one memory latency is paid per batch and no step waits on
+// memory.  (A far contribution can wait to the batch's end: its node enters the window
+// at a later batch's end.)  The page-table entry of a batch is loaded at the end of the
+// batch two before it, so that it has arrived when its nodes are loaded.  Within a
+// step, the node's adjoint and its operands' window entries are
+// read together and written back unconditionally (an operand outside the window uses
+// the spare entry), so that a step waits for the window once.
 constexpr int kSweepBatch = 16;
+static_assert(kTapePage % kSweepBatch == 0, "a sweep batch lies within one tape page");
 
 template <class Win>
-UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld, int slot, int mkx, int ncnst,
-                                 size_t ldc, int col, const AdCotangents& cot, const AdGradients& g, Win win, int S) {
+UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nodes, size_t wld, int slot, int mkx,
+                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g, Win win,
+                                 int S) {
     const int KS = mkx + 1;
     // the header's fields in registers: the sweep's stores could alias the header
     const TapeCtx th = t;
-    UWSHCU_GMEM TapeNode* const tp = (UWSHCU_GMEM TapeNode*)th.nodes;
-    auto P = [&](int q) { return &tp[tape_slot(th, q)].adj; };
+    UWSHCU_GMEM double* const adjv = (UWSHCU_GMEM double*)adjv_;
+    auto P = [&](int q) { return adjv + q; };
+    const bool paged = th.pages != nullptr;
+    const UWSHCU_GMEM int* const pages = (const UWSHCU_GMEM int*)th.pages;
+    const UWSHCU_GMEM TapeNode* const pool = paged ? (const UWSHCU_GMEM TapeNode*)th.pool->base : nullptr;
+    const UWSHCU_GMEM TapeNode* const contig = (const UWSHCU_GMEM TapeNode*)th.nodes;   // biased: node q at contig - q
+    // node q of a batch whose page (paged tapes) is pg lives at top(pg) - (q & qmask)
+    const int qmask = paged ? kTapePage - 1 : 0x7fffffff;
+    auto top = [&](int pg) { return paged ? pool + (size_t)pg * kTapePage + (kTapePage - 1) : contig; };
+    auto page_of = [&](int q) { return paged ? pages[q >> kTapePageLog2] : 0; };
     auto node = [&](int f, int k) { return nodes[((size_t)f * KS + k) * wld + slot]; };
     auto row = [&](int k) { return (size_t)(mkx - k) * ldc + col; };
     for (int f = 0; f < ADF_TR; ++f) {
@@ -125,28 +154,34 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld,
         constexpr int K = kSweepBatch;
         double da[K], db[K];
         int ia[K], ib[K];
-        auto load_batch = [&](int q0) {
+        // the nodes of the batch at q0, clamped into [nin, n - 1] (the same page)
+        auto load_batch = [&](int q0, const UWSHCU_GMEM TapeNode* tb) {
 #pragma unroll
             for (int u = 0; u < K; ++u) {
-                const UWSHCU_GMEM TapeNode* p = &tp[tape_slot(th, q0 - u > nin ? q0 - u : nin)];
+                int q = q0 - u < n - 1 ? q0 - u : n - 1;
+                q = q > nin ? q : nin;
+                const UWSHCU_GMEM TapeNode* p = tb - (q & qmask);
                 da[u] = p->da;
                 db[u] = p->db;
                 ia[u] = p->a;
                 ib[u] = p->b;
             }
         };
-        for (int q = n - 1; q >= 0 && q > n - 1 - S; --q) win(q % S) = adj_load(P(q));
-        if (n > nin) load_batch(n - 1);
-        for (int q0 = n - 1; q0 >= nin; q0 -= K) {
+        int q0 = (n - 1) | (K - 1);
+        for (int q = n - 1; q >= 0 && q > q0 - S; --q) win(q % S) = adj_load(P(q));
+        if (n > nin) load_batch(q0, top(page_of(n - 1)));
+        // page of the next batch, loaded a batch ahead of its use (clamped to node 0)
+        int pg_next = page_of(q0 - K >= 0 ? q0 - K : 0);
+        for (; q0 >= nin; q0 -= K) {
             const int lo = q0 - S;     // the window holds the nodes above lo
             int fx[2 * K];             // far contributions of the batch: node (or -1), value
             double fc[2 * K];
 #pragma unroll
             for (int u = 0; u < K; ++u) {
-                // past the last node (q < nin) a step still runs, on the spare entry, so that
-                // every path consumes the batch's loads
+                // a step outside [nin, n - 1] still runs, on the spare entry, so that every
+                // path consumes the batch's loads
                 const int q = q0 - u;
-                const bool act = q >= nin;
+                const bool act = q >= nin && q < n;
                 const int a = ia[u], b = ib[u];
                 const bool nearA = act && a >= 0 && a > lo, nearB = act && b >= 0 && b > lo;
                 const int sa = nearA ? a % S : S, sb = nearB ? b % S : S;
@@ -171,7 +206,9 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld,
             double enter[K];
 #pragma unroll
             for (int u = 0; u < K; ++u) enter[u] = adj_load(P(lo - u > 0 ? lo - u : 0));
-            load_batch(q0 - K);
+            const int pg_after = page_of(q0 - 2 * K >= 0 ? q0 - 2 * K : 0);
+            if (q0 - K >= nin) load_batch(q0 - K, top(pg_next));
+            pg_next = pg_after;
             // written unconditionally (to the spare entry when nothing enters), so that no
             // load is left pending on some paths into the next batch
 #pragma unroll
@@ -181,12 +218,12 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld,
         for (int q = n - 1; q >= nin; --q) {
             const double w = adj_load(P(q));
             if (w == 0.0) continue;
-            const UWSHCU_GMEM TapeNode& nd = tp[tape_slot(th, q)];
+            const UWSHCU_GMEM TapeNode& nd = *(top(page_of(q)) - (q & qmask));
             if (nd.a >= 0) adj_add(P(nd.a), w * nd.da);
             if (nd.b >= 0) adj_add(P(nd.b), w * nd.db);
         }
     }
-    // the inputs' adjoints: the window holds nodes nin - S .. nin - 1, the tape the rest
+    // the inputs' adjoints: the window holds nodes nin - S .. nin - 1, adjv the rest
     auto A = [&](int q) { return (S > 0 && q >= nin - S) ? win(q % S) : adj_load(P(q)); };
     for (int k = 1; k <= mkx; ++k) {
         if (g.t0) g.t0[row(k)] = A(ADI_T0 * KS + k);
@@ -204,9 +241,9 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld,
 struct NoWindow {
     UWSHCU_HD double& operator()(int) const { return *(double*)nullptr; }
 };
-UWSHCU_HD inline void ad_reverse(const TapeCtx& t, const int* nodes, size_t wld, int slot, int mkx, int ncnst,
-                                 size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
-    ad_reverse(t, nodes, wld, slot, mkx, ncnst, ldc, col, cot, g, NoWindow{}, 0);
+UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv, const int* nodes, size_t wld, int slot, int mkx,
+                                 int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {
+    ad_reverse(t, adjv, nodes, wld, slot, mkx, ncnst, ldc, col, cot, g, NoWindow{}, 0);
 }
 
 }  // namespace uwshcu

@@ -8,16 +8,16 @@
 
 namespace uwshcu {
 
-// Recording pass: slot j's tape holds cap[j] nodes at tape_base + off[j].  With
-// counts != nullptr it is the sizing pass instead: the recording stores nothing and
-// counts[col] receives the number of nodes the column needs.  One kernel serves both,
-// so that the column code, expanded in place (the compiler then sees every array
-// pointer come from the kernel arguments and issues global, not flat, accesses), is
-// compiled once.
+// Recording pass.  Slot j's tape takes its nodes page by page from the pool (its page
+// table at pages + j * maxp), so no sizing pass is needed; a column that finds the
+// pool empty goes on counting and is counted in pool->retry (the host records it again
+// in a launch the pool can hold).  At its end a column takes its n adjoint entries
+// from the pool's adjoint array.  The column code is expanded in place: the compiler
+// then sees every array pointer come from the kernel arguments and issues global, not
+// flat, accesses.
 __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
-                                                           const double* __restrict__ estbl, TapeNode* tape_base,
-                                                           const long long* off, const int* cap, int* node_out,
-                                                           int* counts) {
+                                                           const double* __restrict__ estbl, TapePool* pool, int* pages,
+                                                           int maxp, int cap, int* node_out) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
     for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
     __syncthreads();
@@ -25,33 +25,43 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     if (j >= n) return;
     // the tape header lives in LDS: every recorded operation updates it
     TapeCtx& tape = *ad_tape();
-    if (counts) ad_begin(tape, nullptr, 0x7fffffff, ad_inputs(args.mkx));
-    else ad_begin(tape, tape_base + off[j], cap[j], ad_inputs(args.mkx));
+    int* const pt = pages + (size_t)j * maxp;
+    const int nsh = pool->nshards;
+    ad_begin(tape, nullptr, pt, pool, nsh > 1 ? (int)(blockIdx.x % (unsigned)nsh) : 0, cap, ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
     col.template run_body<false>();
-    if (counts) {
-        counts[args.col0 + j] = tape.n;
-        return;
-    }
     col.ad_export(node_out);
+    if (tape.overflow == 0) tape.adj = (long long)atomicAdd(&pool->adj_next, (unsigned long long)tape.n);
+    else if (tape.overflow == 2) atomicAdd(&pool->retry, 1);
+    else record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
+    tape.pages = pt;
     args.ad.tapes[j] = tape;     // the sweep reads the header from HBM
-    if (tape.overflow) record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
 }
 
 // Backward sweep, one column per thread, with a 64-entry window of recent adjoints
 // (and its spare entry) per thread in LDS: 33 KB per workgroup of 64, so that the four
-// workgroups a CU needs to give each SIMD one wave fit its LDS.
+// workgroups a CU needs to give each SIMD one wave fit its LDS.  Columns whose
+// recording did not complete are skipped.
 constexpr int kSweepWindow = 64;
 __global__ __launch_bounds__(64) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
                                                           long long ldc, const TapeCtx* tapes, const int* node_out,
-                                                          AdCotangents cot, AdGradients g) {
+                                                          double* adj, AdCotangents cot, AdGradients g) {
     __shared__ double s_win[kSweepWindow + 1][64];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    const TapeCtx& t = tapes[j];
+    if (t.overflow) return;
     const int lane = threadIdx.x;
     auto win = [&](int s) -> double& { return s_win[s][lane]; };
-    ad_reverse(tapes[j], node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc, col0 + j, cot, g, win, kSweepWindow);
+    ad_reverse(t, adj + t.adj, node_out, (size_t)wld, j, mkx, ncnst, (size_t)ldc, col0 + j, cot, g, win,
+               kSweepWindow);
+}
+
+// Zero n doubles (the sweep's adjoints; a kernel, not a memset, so that a captured
+// graph records no memset node).
+__global__ __launch_bounds__(256) void zero_kernel(double* p, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0.0;
 }
 
 // Sum of the per-column parameter gradients par[3][ldc] over ncol columns, in FP64:
@@ -70,22 +80,22 @@ __global__ __launch_bounds__(256) void param_sum_kernel(const double* par, long 
     if (threadIdx.x == 0) sum[p] = s[0];
 }
 
-hipError_t launch_adjoint_count(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                                int* counts, hipStream_t stream) {
-    hipLaunchKernelGGL(adjoint_record_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, args, n, dconsts, destbl,
-                       (TapeNode*)nullptr, (const long long*)nullptr, (const int*)nullptr, (int*)nullptr, counts);
+hipError_t launch_adjoint_record(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                                 TapePool* pool, int* pages, int maxp, int cap, int* node_out, hipStream_t stream) {
+    hipLaunchKernelGGL(adjoint_record_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, args, n, dconsts, destbl, pool,
+                       pages, maxp, cap, node_out);
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                          TapeNode* tape_base, const long long* off, const int* cap, int* node_out,
-                          const AdCotangents& cot, const AdGradients& g, hipStream_t stream) {
-    const int block = 64;
-    const int grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(adjoint_record_kernel, dim3(grid), dim3(block), 0, stream, args, n, dconsts, destbl, tape_base,
-                       off, cap, node_out, (int*)nullptr);
-    hipLaunchKernelGGL(adjoint_sweep_kernel, dim3(grid), dim3(block), 0, stream, n, args.col0, args.mkx, args.ncnst,
-                       args.wld, (long long)args.ldc, (const TapeCtx*)args.ad.tapes, (const int*)node_out, cot, g);
+hipError_t launch_adjoint_sweep(const FwdArgs& args, int n, double* adj, long long nadj, const int* node_out,
+                                const AdCotangents& cot, const AdGradients& g, hipStream_t stream) {
+    if (nadj > 0) {
+        long long blocks = (nadj + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, adj, nadj);
+    }
+    hipLaunchKernelGGL(adjoint_sweep_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, n, args.col0, args.mkx,
+                       args.ncnst, args.wld, (long long)args.ldc, (const TapeCtx*)args.ad.tapes, node_out, adj, cot, g);
     return hipGetLastError();
 }
 

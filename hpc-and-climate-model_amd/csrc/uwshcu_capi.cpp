@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -46,8 +47,8 @@ struct uwshcu_ctx {
     long long tape_cap = 0;
     double* d_parcol = nullptr;
     size_t parcol_cols = 0;
-    int* d_counts = nullptr;
-    size_t count_cols = 0;
+    uwshcu::TapePool h_pool{};   // host copy of the recording pass's page-pool counters
+    double tape_est = 0;         // tape nodes per column the pool is sized for (0: the default)
     // columns whose second convection pass runs in a launch of its own
     int* d_defer = nullptr;
     double* d_defer_state = nullptr;   // NRESUME x defer_cols doubles: their saved scalars
@@ -314,34 +315,29 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     if (ncol == 0) return UWSHCU_OK;
     if (!hip_ok(hipSetDevice(ctx->device))) return UWSHCU_ERR_HIP;
     const int KS = pver + 1;
-    const long long user_cap = ctx->tape_cap > 0 ? ctx->tape_cap : 0x7fffffffLL;
-    // per-column fixed pieces: AD workspace, AD outputs, output nodes, tape header,
-    // offset and capacity; a tape node (operands, partials and its adjoint) takes 32 bytes
+    // tape capacity per column: the caller's, else four times the default bound
+    // (2048 + 512 ncnst)(pver + 1), about ten times the longest tapes measured
+    long long capn = ctx->tape_cap > 0 ? ctx->tape_cap : 4LL * (2048 + 512LL * ncnst) * KS;
+    if (capn > 0x7fffffffLL - kTapePage) capn = 0x7fffffffLL - kTapePage;
+    const int cap = (int)capn;
+    const int maxp = (cap + kTapePage - 1) / kTapePage;
+    // per-column fixed pieces: AD workspace, AD outputs, output nodes, tape header, page
+    // table; per pool page: its nodes and as many adjoint entries
     const size_t b_ws = ws_bytes_per_column(pver, ncnst, sizeof(AD), sizeof(AD));
     const size_t b_out = (size_t)ad_fields(ncnst) * KS * sizeof(AD);
     const size_t b_nodes = (size_t)ad_fields(ncnst) * KS * sizeof(int);
-    const size_t b_fix = b_ws + b_out + b_nodes + sizeof(TapeCtx) + sizeof(long long) + sizeof(int);
-    const size_t b_node = sizeof(TapeNode);
-    static_assert(sizeof(TapeNode) == 32, "tape node layout");
-    auto layout = [&](long long nb, size_t nodes, size_t* o) {   // o: out, nodes, tapes, off, cap, tape, end
+    const size_t b_fix = b_ws + b_out + b_nodes + sizeof(TapeCtx) + (size_t)maxp * sizeof(int);
+    const size_t b_page = (size_t)kTapePage * (sizeof(TapeNode) + sizeof(double));
+    auto layout = [&](long long nb, size_t npages, size_t* o) {   // o: out, nodes, tapes, pages, pool, tape, adj, end
         o[0] = align256(b_ws * nb);
         o[1] = o[0] + align256(b_out * nb);
         o[2] = o[1] + align256(b_nodes * nb);
         o[3] = o[2] + align256(sizeof(TapeCtx) * nb);
-        o[4] = o[3] + align256(sizeof(long long) * nb);
-        o[5] = o[4] + align256(sizeof(int) * nb);
-        o[6] = o[5] + align256(sizeof(TapeNode) * nodes);
+        o[4] = o[3] + align256((size_t)maxp * sizeof(int) * nb);
+        o[5] = o[4] + align256(sizeof(TapePool));
+        o[6] = o[5] + align256(sizeof(TapeNode) * kTapePage * npages);
+        o[7] = o[6] + align256(sizeof(double) * kTapePage * npages);
     };
-    if ((size_t)ncol > ctx->count_cols) {
-        if (ctx->d_counts) {
-            if (!hip_ok(hipDeviceSynchronize())) return UWSHCU_ERR_HIP;
-            (void)hipFree(ctx->d_counts);
-            ctx->d_counts = nullptr;
-            ctx->count_cols = 0;
-        }
-        if (!hip_ok(hipMalloc((void**)&ctx->d_counts, (size_t)ncol * sizeof(int)))) return UWSHCU_ERR_NOMEM;
-        ctx->count_cols = (size_t)ncol;
-    }
     double* parcol = grad->par_col;
     if (!parcol && grad->par_sum) {
         if ((size_t)pcols > ctx->parcol_cols) {
@@ -375,88 +371,123 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     a.cbmf = ch.cbmf; a.qc = ch.qc; a.rliq = ch.rliq; a.cnt = ch.cnt; a.cnb = ch.cnb; a.wu = ch.wu;
     a.diag = diag;
     a.err = ctx->d_err;
+    // cush is read and written by the column: a launch recorded again (below) must read
+    // the caller's input, so every launch reads a copy of it
+    if (int e = ensure_mixed(ctx, (size_t)ncol)) return e;
+    if (!hip_ok(hipMemcpyAsync(ctx->d_cush_in, ch.cush, (size_t)ncol * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream)))
+        return UWSHCU_ERR_HIP;
+    a.cush_in = ctx->d_cush_in;
 
-    // 1. sizing pass in batches of the fixed per-column pieces
-    long long nbc = (long long)(ctx->ws_limit / (b_fix + 64));
-    if (nbc > ncol) nbc = ncol;
-    if (nbc < 1) return UWSHCU_ERR_NOMEM;
-    size_t o[7];
-    layout(nbc, 0, o);
-    int rc = ensure_ws(ctx, o[6]);
+    // Columns per launch and pool size: the pool is sized for a per-column estimate of
+    // the tape length, plus half a page per column for their last pages.  The pool is
+    // split into shards with a page counter each, one per four workgroups of 64 columns.  The estimate
+    // is 1.25 times the mean length of the previous call's tapes, or on a context's
+    // first call (600 + 150 ncnst)(pver + 1) nodes (1.7 times the mean tape of a
+    // 35-level chunk that is 72 % convecting).  A launch whose columns need more finds
+    // the pool empty; it is then recorded again in parts that fit, from the lengths its
+    // columns counted.
+    const double est = ctx->tape_est > 0 ? ctx->tape_est : (double)(600 + 150 * ncnst) * KS;
+    const double pages_per_col = est / kTapePage + 0.5;
+    double recorded = 0;   // tape nodes of the columns recorded
+    long long nb = (long long)((double)ctx->ws_limit / ((double)b_fix + pages_per_col * (double)b_page + 512.0));
+    if (nb > ncol) nb = ncol;
+    if (nb < 1) return UWSHCU_ERR_NOMEM;
+    // pages the workspace limit leaves for a launch of `cols` columns
+    auto room = [&](long long cols) {
+        size_t o[8];
+        layout(cols, 0, o);
+        return ctx->ws_limit > o[7] + 1024 ? (ctx->ws_limit - o[7] - 1024) / b_page : (size_t)0;
+    };
+    auto pages_for = [&](long long cols) {
+        const size_t np = (size_t)std::ceil((double)cols * pages_per_col) + 1;
+        return np < room(cols) ? np : room(cols);
+    };
+    size_t o[8];
+    layout(nb, pages_for(nb), o);
+    int rc = ensure_ws(ctx, o[7]);
     if (rc) return rc;
     char* base = (char*)ctx->d_ws;
-    a.ws = base;
-    a.wld = nbc;
-    a.ad.out = (AD*)(base + o[0]);
-    a.ad.tapes = (TapeCtx*)(base + o[2]);
-    for (long long c0 = 0; c0 < ncol; c0 += nbc) {
-        const int n = (int)((ncol - c0) < nbc ? (ncol - c0) : nbc);
-        a.col0 = (int)c0;
-        if (!hip_ok(launch_adjoint_count(a, n, ctx->d_consts, ctx->d_estbl, ctx->d_counts, stream)))
-            return UWSHCU_ERR_HIP;
-    }
-    std::vector<int> counts((size_t)ncol);
-    if (!hip_ok(hipMemcpyAsync(counts.data(), ctx->d_counts, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost,
-                               stream)) ||
-        !hip_ok(hipStreamSynchronize(stream)))
-        return UWSHCU_ERR_HIP;
-    // 2. batches whose exact tapes fit the workspace; column c of a batch starts at
-    // the sum of the earlier columns' tape lengths
-    std::vector<int> cap((size_t)ncol);
-    for (int c = 0; c < ncol; ++c) cap[c] = (int)std::min<long long>(counts[c], user_cap);
-    std::vector<long long> starts{0};
-    {
-        size_t nodes = 0;
-        long long n = 0;
-        for (int c = 0; c < ncol; ++c) {
-            if (n > 0 && (size_t)(n + 1) * (b_fix + 2048) + (nodes + cap[c]) * b_node > ctx->ws_limit) {
-                starts.push_back(c);
-                nodes = 0;
-                n = 0;
-            }
-            nodes += (size_t)cap[c];
-            ++n;
+    TapePool* const hp = &ctx->h_pool;
+
+    // record [c0, c0 + n) with a pool of npages pages; *retry receives the number of
+    // columns that found the pool empty, *nadj the adjoint entries taken
+    auto record = [&](long long c0, long long n, size_t npages, int nshards, int* retry, long long* nadj) -> int {
+        size_t oo[8];
+        layout(n, npages, oo);
+        if (oo[7] > ctx->ws_bytes) {   // a part of a launch that ran out of pages (after a synchronise)
+            if (int e = ensure_ws(ctx, oo[7])) return e;
+            base = (char*)ctx->d_ws;
         }
-        starts.push_back(ncol);
-    }
-    std::vector<long long> off((size_t)ncol);
-    std::vector<size_t> batch_nodes;
-    size_t need = 0;
-    for (size_t b = 0; b + 1 < starts.size(); ++b) {
-        size_t acc = 0;
-        for (long long c = starts[b]; c < starts[b + 1]; ++c) {
-            off[c] = (long long)acc;
-            acc += (size_t)cap[c];
-        }
-        batch_nodes.push_back(acc);
-        layout(starts[b + 1] - starts[b], acc, o);
-        need = std::max(need, o[6]);
-    }
-    if ((rc = ensure_ws(ctx, need))) return rc;
-    base = (char*)ctx->d_ws;
-    for (size_t b = 0; b + 1 < starts.size(); ++b) {
-        const long long c0 = starts[b], n = starts[b + 1] - c0;
-        layout(n, batch_nodes[b], o);
         a.ws = base;
         a.wld = n;
         a.col0 = (int)c0;
-        a.ad.out = (AD*)(base + o[0]);
-        a.ad.tapes = (TapeCtx*)(base + o[2]);
-        if (!hip_ok(hipMemcpyAsync(base + o[3], off.data() + c0, (size_t)n * sizeof(long long), hipMemcpyHostToDevice,
+        a.ad.out = (AD*)(base + oo[0]);
+        a.ad.tapes = (TapeCtx*)(base + oo[2]);
+        std::memset(hp, 0, sizeof(TapePool));
+        hp->base = (TapeNode*)(base + oo[5]);
+        hp->nshards = nshards;
+        hp->per_shard = (int)(npages / (size_t)nshards);
+        TapePool* dpool = (TapePool*)(base + oo[4]);
+        if (!hip_ok(hipMemcpyAsync(dpool, hp, sizeof(TapePool), hipMemcpyHostToDevice, stream)) ||
+            !hip_ok(launch_adjoint_record(a, (int)n, ctx->d_consts, ctx->d_estbl, dpool, (int*)(base + oo[3]), maxp,
+                                          cap, (int*)(base + oo[1]), stream)) ||
+            !hip_ok(hipMemcpyAsync(hp, dpool, offsetof(TapePool, next), hipMemcpyDeviceToHost, stream)) ||
+            !hip_ok(hipStreamSynchronize(stream)))
+            return UWSHCU_ERR_HIP;
+        *retry = hp->retry;
+        *nadj = (long long)hp->adj_next;
+        return UWSHCU_OK;
+    };
+    auto sweep = [&](long long n, size_t npages, long long nadj) -> int {
+        size_t oo[8];
+        layout(n, npages, oo);
+        if (!hip_ok(launch_adjoint_sweep(a, (int)n, (double*)(base + oo[6]), nadj, (const int*)(base + oo[1]), ct, g,
+                                         stream)))
+            return UWSHCU_ERR_HIP;
+        return UWSHCU_OK;
+    };
+    for (long long c0 = 0; c0 < ncol; c0 += nb) {
+        const long long n = (ncol - c0) < nb ? (ncol - c0) : nb;
+        const size_t npages = pages_for(n);
+        // a page counter per four workgroups (at most kTapeShards)
+        const long long groups = (n + 63) / 64;
+        const int nshards = (int)std::max(1LL, std::min<long long>(kTapeShards, groups / 4));
+        int retry = 0;
+        long long nadj = 0;
+        if ((rc = record(c0, n, npages, nshards, &retry, &nadj))) return rc;
+        if (retry == 0) {
+            if ((rc = sweep(n, npages, nadj))) return rc;
+            recorded += (double)nadj;
+            continue;
+        }
+        // the pool ran out: every column counted its length; record again in parts
+        std::vector<TapeCtx> hdr((size_t)n);
+        layout(n, npages, o);
+        if (!hip_ok(hipMemcpyAsync(hdr.data(), base + o[2], (size_t)n * sizeof(TapeCtx), hipMemcpyDeviceToHost,
                                    stream)) ||
-            !hip_ok(hipMemcpyAsync(base + o[4], cap.data() + c0, (size_t)n * sizeof(int), hipMemcpyHostToDevice,
-                                   stream)))
+            !hip_ok(hipStreamSynchronize(stream)))
             return UWSHCU_ERR_HIP;
-        if (!hip_ok(launch_adjoint(a, (int)n, ctx->d_consts, ctx->d_estbl, (TapeNode*)(base + o[5]),
-                                   (const long long*)(base + o[3]), (const int*)(base + o[4]), (int*)(base + o[1]),
-                                   ct, g, stream)))
-            return UWSHCU_ERR_HIP;
-        // the host copies above read off/cap, which live until the next batch's copies
-        if (b + 2 < starts.size() && !hip_ok(hipStreamSynchronize(stream))) return UWSHCU_ERR_HIP;
+        long long s0 = 0;
+        while (s0 < n) {
+            size_t need = 0;
+            long long s1 = s0;
+            while (s1 < n) {
+                const size_t pc = ((size_t)(hdr[s1].n < cap ? hdr[s1].n : cap) + kTapePage - 1) / kTapePage;
+                if (s1 > s0 && need + pc > room(s1 + 1 - s0)) break;
+                need += pc;
+                ++s1;
+            }
+            if (need > room(s1 - s0)) return UWSHCU_ERR_NOMEM;   // one column needs more than the limit holds
+            if ((rc = record(c0 + s0, s1 - s0, need, 1, &retry, &nadj))) return rc;   // one counter: exact fit
+            if (retry) return UWSHCU_ERR_HIP;                    // the same columns counted a different length
+            if ((rc = sweep(s1 - s0, need, nadj))) return rc;
+            recorded += (double)nadj;
+            s0 = s1;
+        }
     }
+    ctx->tape_est = 1.25 * recorded / (double)ncol;
     if (grad->par_sum && !hip_ok(launch_param_sum(parcol, pcols, ncol, grad->par_sum, stream))) return UWSHCU_ERR_HIP;
-    // off/cap are host vectors: make sure the last copies have completed before they go
-    if (!hip_ok(hipStreamSynchronize(stream))) return UWSHCU_ERR_HIP;
     return UWSHCU_OK;
 }
 
@@ -608,7 +639,6 @@ int uwshcu_ctx_destroy(uwshcu_ctx* ctx) {
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_parcol) (void)hipFree(ctx->d_parcol);
-    if (ctx->d_counts) (void)hipFree(ctx->d_counts);
     if (ctx->d_defer) (void)hipFree(ctx->d_defer);
     if (ctx->d_defer_state) (void)hipFree(ctx->d_defer_state);
     if (ctx->d_packed) (void)hipFree(ctx->d_packed);

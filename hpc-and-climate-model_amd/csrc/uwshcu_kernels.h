@@ -23,17 +23,17 @@ hipError_t launch_forward_mixed(const FwdArgs& args, int n, const uwshcu_consts*
 // set or whose outputs fail the FP64 conservation check at relative tolerance tol.
 hipError_t launch_check_columns(const FwdArgs& args, const uwshcu_consts* dconsts, double tol, int* list, int* count,
                                 hipStream_t stream);
-// Adjoint sizing pass over columns [args.col0, args.col0 + n): counts[col] = the
-// number of tape nodes the column's recording needs.
-hipError_t launch_adjoint_count(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                                int* counts, hipStream_t stream);
-// Adjoint over columns [args.col0, args.col0 + n): record each column's FP64 forward
-// pass on its tape (slot j: cap[j] nodes at tape_base + off[j], each node carrying its
-// adjoint; args.ad holds the AD output values and the tape headers), write the forward
-// outputs, then sweep the tapes from the cotangents to the gradients.
-hipError_t launch_adjoint(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
-                          TapeNode* tape_base, const long long* off, const int* cap, int* node_out,
-                          const AdCotangents& cot, const AdGradients& g, hipStream_t stream);
+// Adjoint over columns [args.col0, args.col0 + n), in two launches.  Recording: each
+// column records its FP64 forward pass on a tape paged from pool (page table of slot
+// j at pages + j * maxp; at most cap nodes), writes the forward outputs and its output
+// nodes (node_out), and takes its adjoint entries from the pool's adjoint array
+// (args.ad holds the AD output values and the tape headers).  Sweep: zero adj[0, nadj)
+// (nadj = the pool's adj_next after the recording), then sweep each recorded tape from
+// the cotangents to the gradients.
+hipError_t launch_adjoint_record(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
+                                 TapePool* pool, int* pages, int maxp, int cap, int* node_out, hipStream_t stream);
+hipError_t launch_adjoint_sweep(const FwdArgs& args, int n, double* adj, long long nadj, const int* node_out,
+                                const AdCotangents& cot, const AdGradients& g, hipStream_t stream);
 // sum[p] = sum over columns c < ncol of par[p*ldc + c], p = 0..2, in FP64.
 hipError_t launch_param_sum(const double* par, long long ldc, int ncol, double* sum, hipStream_t stream);
 // Set the two status words to zero (a fill kernel, so that graph capture records no memset node).

@@ -27,6 +27,10 @@ extern "C" int hostemu_last_triggered() { return g_triggered; }
 static int g_window = 0;
 extern "C" void hostemu_set_window(int w) { g_window = w; }
 extern "C" void hostemu_set_staged(int on) { g_staged = on; }
+// hostemu_adjoint records paged tapes (the GPU's layout: pages of kTapePage nodes from a
+// pool, here one pool per thread) instead of one contiguous tape per column
+static int g_paged = 0;
+extern "C" void hostemu_set_paged(int on) { g_paged = on; }
 // the modified scheme (Column<double, double, true>) for the next hostemu_forward
 // calls while set; zw == nullptr returns to the UW scheme
 static const uwshcu_zw_params* g_zw = nullptr;
@@ -222,20 +226,35 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
     if (g_window > 0 && g_window < kSweepBatch) return UWSHCU_ERR_ARG;   // the window holds a batch
 #pragma omp parallel
     {
-        std::vector<TapeNode> buf((size_t)cap);
+        const int maxp = (cap + kTapePage - 1) / kTapePage;
+        std::vector<TapeNode> buf(g_paged ? (size_t)maxp * kTapePage : (size_t)cap);
+        std::vector<int> pages((size_t)maxp);
+        std::vector<double> adj((size_t)cap);
+        TapePool pool{};
+        pool.base = buf.data();
+        pool.per_shard = maxp;
+        pool.nshards = 1;
 #pragma omp for schedule(dynamic, 4)
         for (int i = 0; i < ncol; ++i) {
-            ad_begin(tapes[i], buf.data(), cap, nin);
+            pool.next[0] = 0;
+            if (g_paged) {
+                ad_begin(tapes[i], nullptr, pages.data(), &pool, 0, cap, nin);
+            } else {
+                ad_begin(tapes[i], buf.data(), nullptr, nullptr, 0, cap, nin);
+            }
+            std::fill(adj.begin(), adj.end(), 0.0);
             const Column<AD, AD> col(a, T, i, i);
             col.run();
             col.ad_export(nodes.data());
-            if (g_window > 0) {
+            if (tapes[i].overflow) {
+                // as on the GPU: a tape that did not fit gets no sweep (and no gradient)
+            } else if (g_window > 0) {
                 std::vector<double> wv((size_t)g_window + 1);
                 auto win = [&](int q) -> double& { return wv[(size_t)q]; };
-                ad_reverse(tapes[i], nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g,
+                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g,
                            win, g_window);
             } else {
-                ad_reverse(tapes[i], nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
+                ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
             }
             tape_len[i] = tapes[i].overflow ? -1 : tapes[i].n;
         }

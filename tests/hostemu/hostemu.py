@@ -124,7 +124,7 @@ OUT_ORDER = ("umf", "slflx", "qtflx", "flxprc1", "flxsnow1", "qvten", "qlten", "
              "cnt", "cnb")
 
 
-def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False, window=0):
+def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False, window=0, paged=False):
     """Host adjoint (tape) of the FP64 column code.
 
     cot: dict output-name -> cotangent array (same shape as the output; missing = 0).
@@ -161,8 +161,10 @@ def adjoint(chunk, cot, params=None, consts=None, cap=1 << 20, checked=False, wi
     L_ = lib(checked)
     L_.hostemu_adjoint.restype = C.c_int
     L_.hostemu_set_window(int(window))
+    L_.hostemu_set_paged(int(paged))
     rc = L_.hostemu_adjoint(*args)
     L_.hostemu_set_window(0)
+    L_.hostemu_set_paged(0)
     assert rc == 0, rc
     out["cush"] = cush
     if ncnst == 0:

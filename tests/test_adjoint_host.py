@@ -126,6 +126,9 @@ def test_windowed_sweep_is_bit_identical(window):
     _, g0, _, _ = hostemu.adjoint(ch, cot)
     _, g1, _, _ = hostemu.adjoint(ch, cot, window=window)
     _, g2, _, _ = hostemu.adjoint(ch, cot, window=window, checked=True)
+    _, g3, t3, _ = hostemu.adjoint(ch, cot, window=window, paged=True)     # the GPU's tape layout
+    assert t3.max() > 4 * 2048                                            # several tape pages
     for k in g0:
         np.testing.assert_array_equal(g1[k], g0[k], err_msg=k)
         np.testing.assert_array_equal(g2[k], g0[k], err_msg=k)
+        np.testing.assert_array_equal(g3[k], g0[k], err_msg=k)

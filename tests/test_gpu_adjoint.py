@@ -75,6 +75,39 @@ def test_batched_adjoint_gives_the_same_gradients():
         np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
 
 
+def test_adjoint_records_again_when_the_tape_pool_runs_out():
+    """The recording takes tape pages from a pool sized for 1.25 times the previous
+    call's mean tape.  After a chunk that hardly convects, a convecting chunk under a
+    small workspace limit overruns the pool; its columns, which counted their lengths,
+    are recorded again in parts that fit, with the same gradients."""
+    import torch
+    from uwshcu_amd import device
+    quiet = make_chunk(512, 35, 3, seed=64, inversion_frac=0.95)
+    ch = make_chunk(700, 35, 3, seed=62, inversion_frac=0.1)
+    cot = _cot(ch, 2)
+    _, _, g1 = _gpu_adjoint(ch, cot)
+    ctx = device.Context(0, workspace_limit=600 << 20)
+    try:
+        dev = torch.device("cuda:0")
+        for c, cc in ((quiet, _cot(quiet, 4)), (ch, cot)):
+            inp = device.chunk_to_device(c, "cuda:0")
+            out = ctx.alloc_outputs(c.ncol, c.pver, c.ncnst, diag=True)
+            grads = {k: torch.zeros((c.pver, c.ncol), dtype=torch.float64, device=dev) for k in ("t0", "qv0", "u0", "v0")}
+            grads["tke"] = torch.zeros((c.pver + 1, c.ncol), dtype=torch.float64, device=dev)
+            grads["pblh"] = torch.ones(c.ncol, dtype=torch.float64, device=dev)
+            grads["par_col"] = torch.zeros((3, c.ncol), dtype=torch.float64, device=dev)
+            grads["par_sum"] = torch.zeros(3, dtype=torch.float64, device=dev)
+            cd = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in cc.items()}
+            print(f"\n[adjoint pool] {c.ncol} columns", flush=True)
+            ctx.adjoint_into(inp, out, inp["cush"].clone(), cd, grads, dt=c.dt)
+            rc, _ = ctx.status()
+            assert rc == 0
+        for k in g1:
+            np.testing.assert_array_equal(grads[k].cpu().numpy(), g1[k], err_msg=k)
+    finally:
+        ctx.close()
+
+
 def test_tape_capacity_overflow_is_an_error_not_a_fault():
     ch = make_chunk(256, 35, 3, seed=63)
     rc, _, _ = _gpu_adjoint(ch, _cot(ch, 3), cap=4000)
