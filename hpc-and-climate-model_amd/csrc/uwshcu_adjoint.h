@@ -38,14 +38,7 @@ UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int* pages, TapePool
     for (int q = 0; q < nin; ++q) {
         if (q >= t.limit && !tape_grow(t, q)) break;
         t.n = q + 1;
-        if (TapeNode* top = t.nodes) {
-            TapeNode nd;
-            nd.da = 0.0;
-            nd.db = 0.0;
-            nd.a = -1;
-            nd.b = -1;
-            *(UWSHCU_GMEM TapeNode*)(top - q) = nd;
-        }
+        if (TapeNode* top = t.nodes) tape_store(top - q, 0.0, 0.0, -1, -1);
     }
 }
 

@@ -205,12 +205,14 @@ int uwshcu_merge_shallow_device(int pcols, int ncol, int pver, double latvap, co
  * pattern is the forward's own), writes the forward outputs exactly as
  * uwshcu_forward_device(UWSHCU_FP64) does, then sweeps each tape backwards from the
  * cotangents `cot` and writes the requested gradients `grad` (uwshcu_types.h).
- * A sizing pass first counts each column's tape (about 30 k nodes at 35 levels, 32
- * bytes each with the sweep's scratch); the call then synchronises `stream` once to
- * read the counts and records the columns in batches whose exact tapes fit the
- * workspace limit.  A column longer than uwshcu_ctx_set_tape_capacity (default: no
- * limit) sets UWSHCU_ERR_TAPE in uwshcu_status and gets no gradient.  The call
- * returns after its work is queued on `stream`, and its host buffers are released. */
+ * Tapes are recorded in pages of 2048 nodes taken from a pool in the workspace
+ * (about 23 k nodes per column at 35 levels, 32 bytes each with its adjoint), sized
+ * for 1.25 times the previous call's mean tape; columns whose share of the pool runs
+ * out are recorded again in launches sized from their counted lengths.  The call
+ * synchronises `stream` once per launch of columns, to read the pool's counters.  A
+ * column longer than the tape capacity (uwshcu_ctx_set_tape_capacity) sets
+ * UWSHCU_ERR_TAPE in uwshcu_status and gets no gradient.  The call returns after its
+ * work is queued on `stream`, and its host buffers are released. */
 int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,
     int pcols, int ncol, int pver, int ncnst, double dt,
     const double *ps0, const double *zs0, const double *p0, const double *z0, const double *dp0,
@@ -224,7 +226,8 @@ int uwshcu_adjoint_device(uwshcu_ctx *ctx, const uwshcu_params *par,
     double *cnt, double *cnb, const double *dpdry0, double *wu, int *diag,
     const uwshcu_cotangents *cot, const uwshcu_gradients *grad, void *stream);
 
-/* Upper bound on the adjoint's tape length in nodes per column (0: no bound). */
+/* Upper bound on the adjoint's tape length in nodes per column (0: the default,
+ * 4 (2048 + 512 ncnst)(pver + 1) nodes, about ten times the longest tapes measured). */
 int uwshcu_ctx_set_tape_capacity(uwshcu_ctx *ctx, long long nodes_per_column);
 
 /* Synchronise `stream` and return the status of the forward calls issued on it

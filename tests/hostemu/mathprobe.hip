@@ -1,9 +1,10 @@
 // mathprobe.hip -- TEST HARNESS: evaluates the portable FP64 functions of
 // uwshcu_math.h, sqrt and division (the operator, div_fast and div_rcp) on the GPU and
 // on the host, so that a test can check that both give identical bits for the same
-// inputs.
+// inputs; and the adjoint sweep's atomic FP64 addition (adj_add) on the GPU.
 #include <hip/hip_runtime.h>
 
+#include "uwshcu_adjoint.h"
 #include "uwshcu_math.h"
 
 using namespace uwshcu;
@@ -46,5 +47,25 @@ extern "C" int probe_gpu(int n, const double* x, const double* y, double* out) {
 
 extern "C" int probe_host(int n, const double* x, const double* y, double* out) {
     for (int i = 0; i < n; ++i) probe_one(x[i], y[i], out + NOUT * (size_t)i);
+    return 0;
+}
+
+// out[i] = x[i], then adj_add(&out[i], y[i]) on the GPU (one thread per element)
+__global__ void atomic_add_kernel(int n, const double* y, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adj_add((UWSHCU_GMEM double*)(out + i), y[i]);
+}
+
+extern "C" int probe_atomic_add_gpu(int n, const double* x, const double* y, double* out) {
+    double *dy = nullptr, *dout = nullptr;
+    if (hipMalloc(&dy, n * sizeof(double)) != hipSuccess) return 1;
+    if (hipMalloc(&dout, n * sizeof(double)) != hipSuccess) return 1;
+    if (hipMemcpy(dy, y, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 2;
+    if (hipMemcpy(dout, x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 2;
+    hipLaunchKernelGGL(atomic_add_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, dy, dout);
+    if (hipDeviceSynchronize() != hipSuccess) return 3;
+    if (hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return 4;
+    (void)hipFree(dy);
+    (void)hipFree(dout);
     return 0;
 }

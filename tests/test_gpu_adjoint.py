@@ -108,6 +108,34 @@ def test_adjoint_records_again_when_the_tape_pool_runs_out():
         ctx.close()
 
 
+def test_atomic_fp64_add_rounds_like_the_addition():
+    """The sweep adds far contributions with a no-return FP64 atomic add (adj_add).
+    Its result must be the bits of x + y: normal operands of any relative size,
+    operands a half ulp apart, subnormal results, and exact cancellations."""
+    import ctypes as C
+    import os
+    L = C.CDLL(os.path.join(os.path.dirname(__file__), "hostemu", "libmathprobe.so"))
+    rng = np.random.default_rng(21)
+    n = 1 << 20
+    e1, e2 = rng.integers(-1074, 60, n), rng.integers(-1074, 60, n)
+    x = np.ldexp(rng.uniform(-1, 1, n), e1)
+    y = np.ldexp(rng.uniform(-1, 1, n), e2)
+    k = rng.integers(0, 4, n)
+    half = k == 0                          # y near half an ulp of x
+    y[half] = np.ldexp(rng.uniform(-1, 1, half.sum()), np.maximum(-1074, np.frexp(x[half])[1] - 54))
+    sub = k == 1                           # subnormal sums
+    x[sub] = np.ldexp(rng.uniform(-1, 1, sub.sum()), -1060)
+    y[sub] = np.ldexp(rng.uniform(-1, 1, sub.sum()), -1070)
+    y[k == 2] = -x[k == 2]                 # exact cancellation
+    got = np.zeros(n)
+    pd = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert L.probe_atomic_add_gpu(n, pd(x), pd(y), pd(got)) == 0
+    want = x + y
+    diff = np.flatnonzero(got.view(np.int64) != want.view(np.int64))
+    assert (np.abs(want[sub]) < np.finfo(float).tiny).sum() > 1000
+    assert diff.size == 0, (diff.size, x[diff[:3]], y[diff[:3]])
+
+
 def test_tape_capacity_overflow_is_an_error_not_a_fault():
     ch = make_chunk(256, 35, 3, seed=63)
     rc, _, _ = _gpu_adjoint(ch, _cot(ch, 3), cap=4000)
