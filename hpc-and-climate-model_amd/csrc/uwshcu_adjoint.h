@@ -101,7 +101,10 @@ struct AdGradients {
 // step, the node's adjoint and its operands' window entries are
 // read together and written back unconditionally (an operand outside the window uses
 // the spare entry), so that a step waits for the window once.
-constexpr int kSweepBatch = 16;
+#ifndef UWSHCU_SWEEP_BATCH
+#define UWSHCU_SWEEP_BATCH 16
+#endif
+constexpr int kSweepBatch = UWSHCU_SWEEP_BATCH;
 static_assert(kTapePage % kSweepBatch == 0, "a sweep batch lies within one tape page");
 
 template <class Win>
