@@ -16,30 +16,32 @@ namespace uwshcu {
 // Start a column's tape: contiguous in nodes (cap nodes; nullptr = count only), or,
 // with pages != nullptr, paged from shard `shard` of pool with the column's page table
 // in pages.  The first nin nodes are the leaves (the independent variables).
-UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeNode* nodes, int* pages, TapePool* pool, int shard, int cap, int nin) {
+UWSHCU_HD inline void ad_begin(TapeCtx& t, TapeHot& h, TapeNode* nodes, int* pages, TapePool* pool, int shard, int cap,
+                               int nin) {
     t.pages = pages;
     t.pool = pool;
     t.adj = 0;
     t.cap = cap;
     t.overflow = 0;
     t.pad = 0;
-    t.n = 0;
+    h.n = 0;
     if (pages) {
-        t.nodes = nullptr;
-        t.limit = 0;
+        h.nodes = nullptr;
+        h.limit = 0;
         t.ctr = &pool->next[shard * 32];
         t.shard_base = shard * pool->per_shard;
     } else {
-        t.nodes = nodes ? nodes + (cap - 1) : nullptr;
-        t.limit = cap;
+        h.nodes = nodes ? nodes + (cap - 1) : nullptr;
+        h.limit = cap;
         t.ctr = nullptr;
         t.shard_base = 0;
     }
     for (int q = 0; q < nin; ++q) {
-        if (q >= t.limit && !tape_grow(t, q)) break;
-        t.n = q + 1;
-        if (TapeNode* top = t.nodes) tape_store(top - q, 0.0, 0.0, -1, -1);
+        if (q >= h.limit && !tape_grow(t, h, q)) break;
+        h.n = q + 1;
+        if (TapeNode* top = h.nodes) tape_store(top - q, 0.0, 0.0, -1, -1);
     }
+    t.hot = h;
 }
 
 // The sweep's accesses to a node's adjoint in the tape.  On the GPU they are agent-scope
@@ -114,12 +116,13 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
     const int KS = mkx + 1;
     // the header's fields in registers: the sweep's stores could alias the header
     const TapeCtx th = t;
+    const int n_nodes = th.hot.n;
     UWSHCU_GMEM double* const adjv = (UWSHCU_GMEM double*)adjv_;
     auto P = [&](int q) { return adjv + q; };
     const bool paged = th.pages != nullptr;
     const UWSHCU_GMEM int* const pages = (const UWSHCU_GMEM int*)th.pages;
     const UWSHCU_GMEM TapeNode* const pool = paged ? (const UWSHCU_GMEM TapeNode*)th.pool->base : nullptr;
-    const UWSHCU_GMEM TapeNode* const contig = (const UWSHCU_GMEM TapeNode*)th.nodes;   // biased: node q at contig - q
+    const UWSHCU_GMEM TapeNode* const contig = (const UWSHCU_GMEM TapeNode*)th.hot.nodes;   // biased: node q at contig - q
     // node q of a batch whose page (paged tapes) is pg lives at top(pg) - (q & qmask)
     const int qmask = paged ? kTapePage - 1 : 0x7fffffff;
     auto top = [&](int pg) { return paged ? pool + (size_t)pg * kTapePage + (kTapePage - 1) : contig; };
@@ -145,7 +148,7 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
         if (nd >= 0) adj_add(P(nd), cot.scal[q][col]);
     }
     const int nin = ad_inputs(mkx);
-    const int n = th.n;
+    const int n = n_nodes;
     if (S > 0) {
         constexpr int K = kSweepBatch;
         double da[K], db[K];

@@ -24,14 +24,17 @@ __global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n,
     if (j >= n) return;
     // the tape header lives in LDS: every recorded operation updates it
     TapeCtx& tape = *ad_tape();
+    TapeHot& hot = *ad_hot();
     int* const pt = pages + (size_t)j * maxp;
     const int nsh = pool->nshards;
-    ad_begin(tape, nullptr, pt, pool, nsh > 1 ? (int)(blockIdx.x % (unsigned)nsh) : 0, cap, ad_inputs(args.mkx));
+    ad_begin(tape, hot, nullptr, pt, pool, nsh > 1 ? (int)(blockIdx.x % (unsigned)nsh) : 0, cap,
+             ad_inputs(args.mkx));
     const ThermoT<AD> T{consts, s_estbl};
     const Column<AD, AD> col(args, T, args.col0 + j, j, &tape);
     col.template run_body<false>();
     col.ad_export(node_out);
-    if (tape.overflow == 0) tape.adj = (long long)atomicAdd(&pool->adj_next, (unsigned long long)tape.n);
+    tape.hot = hot;
+    if (tape.overflow == 0) tape.adj = (long long)atomicAdd(&pool->adj_next, (unsigned long long)tape.hot.n);
     else if (tape.overflow == 2) atomicAdd(&pool->retry, 1);
     else record_error(args.err, UWSHCU_ERR_TAPE, args.col0 + j);
     tape.pages = pt;

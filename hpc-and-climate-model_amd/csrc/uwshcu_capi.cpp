@@ -473,7 +473,7 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
             size_t need = 0;
             long long s1 = s0;
             while (s1 < n) {
-                const size_t pc = ((size_t)(hdr[s1].n < cap ? hdr[s1].n : cap) + kTapePage - 1) / kTapePage;
+                const size_t pc = ((size_t)(hdr[s1].hot.n < cap ? hdr[s1].hot.n : cap) + kTapePage - 1) / kTapePage;
                 if (s1 > s0 && need + pc > room(s1 + 1 - s0)) break;
                 need += pc;
                 ++s1;

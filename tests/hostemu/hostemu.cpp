@@ -238,9 +238,9 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
         for (int i = 0; i < ncol; ++i) {
             pool.next[0] = 0;
             if (g_paged) {
-                ad_begin(tapes[i], nullptr, pages.data(), &pool, 0, cap, nin);
+                ad_begin(tapes[i], tapes[i].hot, nullptr, pages.data(), &pool, 0, cap, nin);
             } else {
-                ad_begin(tapes[i], buf.data(), nullptr, nullptr, 0, cap, nin);
+                ad_begin(tapes[i], tapes[i].hot, buf.data(), nullptr, nullptr, 0, cap, nin);
             }
             std::fill(adj.begin(), adj.end(), 0.0);
             const Column<AD, AD> col(a, T, i, i);
@@ -256,7 +256,7 @@ extern "C" int hostemu_adjoint(const uwshcu_consts* c, const double* estbl, cons
             } else {
                 ad_reverse(tapes[i], adj.data(), nodes.data(), (size_t)ncol, i, pver, ncnst, (size_t)pcols, i, ct, g);
             }
-            tape_len[i] = tapes[i].overflow ? -1 : tapes[i].n;
+            tape_len[i] = tapes[i].overflow ? -1 : tapes[i].hot.n;
         }
     }
     return err[0];
