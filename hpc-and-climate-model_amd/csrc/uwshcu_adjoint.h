@@ -9,6 +9,8 @@
 #ifndef UWSHCU_ADJOINT_H
 #define UWSHCU_ADJOINT_H
 
+#include <type_traits>
+
 #include "uwshcu_column.h"
 
 namespace uwshcu {
@@ -149,6 +151,8 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
     }
     const int nin = ad_inputs(mkx);
     const int n = n_nodes;
+    // window entry of node x >= 0 (unsigned: a mask when S is a power of two)
+    auto wslot = [&](int x) { return (int)((unsigned)x % (unsigned)S); };
     if (S > 0) {
         constexpr int K = kSweepBatch;
         double da[K], db[K];
@@ -167,36 +171,43 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
             }
         };
         int q0 = (n - 1) | (K - 1);
-        for (int q = n - 1; q >= 0 && q > q0 - S; --q) win(q % S) = adj_load(P(q));
+        for (int q = n - 1; q >= 0 && q > q0 - S; --q) win(wslot(q)) = adj_load(P(q));
         if (n > nin) load_batch(q0, top(page_of(n - 1)));
         // page of the next batch, loaded a batch ahead of its use (clamped to node 0)
         int pg_next = page_of(q0 - K >= 0 ? q0 - K : 0);
         for (; q0 >= nin; q0 -= K) {
             const int lo = q0 - S;     // the window holds the nodes above lo
+            const int lo1 = lo > -1 ? lo : -1;   // an operand a is in the window iff a > lo1
             int fx[2 * K];             // far contributions of the batch: node (or -1), value
             double fc[2 * K];
+            // the steps of the batch; kAll: every node of the batch lies in [nin, n - 1]
+            auto steps = [&](auto all) {
+                constexpr bool kAll = decltype(all)::value;
 #pragma unroll
-            for (int u = 0; u < K; ++u) {
-                // a step outside [nin, n - 1] still runs, on the spare entry, so that every
-                // path consumes the batch's loads
-                const int q = q0 - u;
-                const bool act = q >= nin && q < n;
-                const int a = ia[u], b = ib[u];
-                const bool nearA = act && a >= 0 && a > lo, nearB = act && b >= 0 && b > lo;
-                const int sa = nearA ? a % S : S, sb = nearB ? b % S : S;
-                const double w = win(act ? q % S : S), va = win(sa), vb = win(sb);
-                const bool live = act && w != 0.0;
-                const double ca = w * da[u], cb = w * db[u];
-                fx[2 * u] = live && a >= 0 && !nearA ? a : -1;
-                fx[2 * u + 1] = live && b >= 0 && !nearB ? b : -1;
-                fc[2 * u] = ca;
-                fc[2 * u + 1] = cb;
-                const double va1 = live && nearA ? va + ca : va;
-                const double vb0 = nearA && sb == sa ? va1 : vb;    // a == b, both in the window
-                const double vb1 = live && nearB ? vb0 + cb : vb;
-                win(sa) = va1;
-                win(sb) = vb1;
-            }
+                for (int u = 0; u < K; ++u) {
+                    // a step outside [nin, n - 1] still runs, on the spare entry, so that
+                    // every path consumes the batch's loads
+                    const int q = q0 - u;
+                    const bool act = kAll || (q >= nin && q < n);
+                    const int a = ia[u], b = ib[u];
+                    const bool nearA = act && a > lo1, nearB = act && b > lo1;
+                    const int sa = nearA ? wslot(a) : S, sb = nearB ? wslot(b) : S;
+                    const double w = win(act ? wslot(q) : S), va = win(sa), vb = win(sb);
+                    const bool live = act && w != 0.0;
+                    const double ca = w * da[u], cb = w * db[u];
+                    fx[2 * u] = live && a >= 0 && !nearA ? a : -1;
+                    fx[2 * u + 1] = live && b >= 0 && !nearB ? b : -1;
+                    fc[2 * u] = ca;
+                    fc[2 * u + 1] = cb;
+                    const double va1 = live && nearA ? va + ca : va;
+                    const double vb0 = nearA && sb == sa ? va1 : vb;    // a == b, both in the window
+                    const double vb1 = live && nearB ? vb0 + cb : vb;
+                    win(sa) = va1;
+                    win(sb) = vb1;
+                }
+            };
+            if (q0 - (K - 1) >= nin && q0 < n) steps(std::true_type{});
+            else steps(std::false_type{});
             // the batch's far contributions, in step order, then the batch's nodes leave the
             // window and nodes lo - u enter, into their entries
 #pragma unroll
@@ -211,7 +222,7 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
             // written unconditionally (to the spare entry when nothing enters), so that no
             // load is left pending on some paths into the next batch
 #pragma unroll
-            for (int u = 0; u < K; ++u) win(q0 - u >= nin && lo - u >= 0 ? (q0 - u) % S : S) = enter[u];
+            for (int u = 0; u < K; ++u) win(q0 - u >= nin && lo - u >= 0 ? wslot(q0 - u) : S) = enter[u];
         }
     } else {
         for (int q = n - 1; q >= nin; --q) {
@@ -223,7 +234,7 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
         }
     }
     // the inputs' adjoints: the window holds nodes nin - S .. nin - 1, adjv the rest
-    auto A = [&](int q) { return (S > 0 && q >= nin - S) ? win(q % S) : adj_load(P(q)); };
+    auto A = [&](int q) { return (S > 0 && q >= nin - S) ? win(wslot(q)) : adj_load(P(q)); };
     for (int k = 1; k <= mkx; ++k) {
         if (g.t0) g.t0[row(k)] = A(ADI_T0 * KS + k);
         if (g.qv0) g.qv0[row(k)] = A(ADI_QV0 * KS + k);
