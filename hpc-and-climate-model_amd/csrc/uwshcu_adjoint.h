@@ -248,8 +248,10 @@ UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv_, const int* nod
         for (int j = 0; j < 3; ++j) g.par[(size_t)j * ldc + col] = A(ADI_PARAMS * KS + j);
 }
 
+// The window of a sweep without one (S = 0): never read, but the call must bind.
 struct NoWindow {
-    UWSHCU_HD double& operator()(int) const { return *(double*)nullptr; }
+    mutable double unused = 0.0;
+    UWSHCU_HD double& operator()(int) const { return unused; }
 };
 UWSHCU_HD inline void ad_reverse(const TapeCtx& t, double* adjv, const int* nodes, size_t wld, int slot, int mkx,
                                  int ncnst, size_t ldc, int col, const AdCotangents& cot, const AdGradients& g) {

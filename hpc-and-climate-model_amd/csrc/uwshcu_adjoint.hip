@@ -14,13 +14,23 @@ namespace uwshcu {
 // from the pool's adjoint array.  The column code is expanded in place: the compiler
 // then sees every array pointer come from the kernel arguments and issues global, not
 // flat, accesses.
-__global__ __launch_bounds__(64) void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
-                                                           const double* __restrict__ estbl, TapePool* pool, int* pages,
-                                                           int maxp, int cap, int* node_out) {
+// UWSHCU_AD_LANES columns per 64-lane workgroup (fewer columns per wavefront give more
+// wavefronts for a launch of few columns) and UWSHCU_AD_OCC wavefronts per SIMD.
+#ifndef UWSHCU_AD_LANES
+#define UWSHCU_AD_LANES 64
+#endif
+#ifndef UWSHCU_AD_OCC
+#define UWSHCU_AD_OCC 1
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(UWSHCU_AD_OCC)))
+void adjoint_record_kernel(FwdArgs args, int n, const uwshcu_consts* __restrict__ consts,
+                           const double* __restrict__ estbl, TapePool* pool, int* pages, int maxp, int cap,
+                           int* node_out) {
     __shared__ double s_estbl[UWSHCU_ESTBL_LEN];
     for (int t = threadIdx.x; t < UWSHCU_ESTBL_LEN; t += blockDim.x) s_estbl[t] = estbl[t];
     __syncthreads();
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)threadIdx.x >= UWSHCU_AD_LANES) return;
+    const int j = blockIdx.x * UWSHCU_AD_LANES + threadIdx.x;
     if (j >= n) return;
     // the tape header lives in LDS: every recorded operation updates it
     TapeCtx& tape = *ad_tape();
@@ -59,8 +69,8 @@ __global__ __launch_bounds__(256) void param_sum_kernel(const double* par, long 
 
 hipError_t launch_adjoint_record(const FwdArgs& args, int n, const uwshcu_consts* dconsts, const double* destbl,
                                  TapePool* pool, int* pages, int maxp, int cap, int* node_out, hipStream_t stream) {
-    hipLaunchKernelGGL(adjoint_record_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, args, n, dconsts, destbl, pool,
-                       pages, maxp, cap, node_out);
+    hipLaunchKernelGGL(adjoint_record_kernel, dim3((n + UWSHCU_AD_LANES - 1) / UWSHCU_AD_LANES), dim3(64), 0, stream,
+                       args, n, dconsts, destbl, pool, pages, maxp, cap, node_out);
     return hipGetLastError();
 }
 
