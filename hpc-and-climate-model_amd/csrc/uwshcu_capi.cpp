@@ -48,7 +48,7 @@ struct uwshcu_ctx {
     double* d_parcol = nullptr;
     size_t parcol_cols = 0;
     uwshcu::TapePool h_pool{};   // host copy of the recording pass's page-pool counters
-    double tape_est = 0;         // tape nodes per column the pool is sized for (0: the default)
+    double tape_ratio = 0;       // previous call's mean tape over its default estimate (0: none yet)
     // columns whose second convection pass runs in a launch of its own
     int* d_defer = nullptr;
     double* d_defer_state = nullptr;   // NRESUME x defer_cols doubles: their saved scalars
@@ -382,12 +382,14 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
     // Columns per launch and pool size: the pool is sized for a per-column estimate of
     // the tape length, plus half a page per column for their last pages.  The pool is
     // split into shards with a page counter each, one per four workgroups of 64 columns.  The estimate
-    // is 1.25 times the mean length of the previous call's tapes, or on a context's
-    // first call (600 + 150 ncnst)(pver + 1) nodes (1.7 times the mean tape of a
-    // 35-level chunk that is 72 % convecting).  A launch whose columns need more finds
+    // is (600 + 150 ncnst)(pver + 1) nodes (1.7 times the mean tape of a 35-level chunk
+    // that is 72 % convecting), scaled after a context's first call by 1.25 times the
+    // ratio of the previous call's mean tape to its own such estimate (so that a call
+    // with other levels or constituents scales it).  A launch whose columns need more finds
     // the pool empty; it is then recorded again in parts that fit, from the lengths its
     // columns counted.
-    const double est = ctx->tape_est > 0 ? ctx->tape_est : (double)(600 + 150 * ncnst) * KS;
+    const double est0 = (double)(600 + 150 * ncnst) * KS;
+    const double est = ctx->tape_ratio > 0 ? ctx->tape_ratio * est0 : est0;
     const double pages_per_col = est / kTapePage + 0.5;
     double recorded = 0;   // tape nodes of the columns recorded
     long long nb = (long long)((double)ctx->ws_limit / ((double)b_fix + pages_per_col * (double)b_page + 512.0));
@@ -486,7 +488,7 @@ int run_adjoint(uwshcu_ctx* ctx, const uwshcu_params* par, int pcols, int ncol, 
             s0 = s1;
         }
     }
-    ctx->tape_est = 1.25 * recorded / (double)ncol;
+    ctx->tape_ratio = 1.25 * recorded / ((double)ncol * est0);
     if (grad->par_sum && !hip_ok(launch_param_sum(parcol, pcols, ncol, grad->par_sum, stream))) return UWSHCU_ERR_HIP;
     return UWSHCU_OK;
 }

@@ -207,7 +207,8 @@ int uwshcu_merge_shallow_device(int pcols, int ncol, int pver, double latvap, co
  * cotangents `cot` and writes the requested gradients `grad` (uwshcu_types.h).
  * Tapes are recorded in pages of 2048 nodes taken from a pool in the workspace
  * (about 23 k nodes per column at 35 levels, 32 bytes each with its adjoint), sized
- * for 1.25 times the previous call's mean tape; columns whose share of the pool runs
+ * from the previous call's mean tape (scaled to this call's levels and constituents,
+ * with 25 % to spare); columns whose share of the pool runs
  * out are recorded again in launches sized from their counted lengths.  The call
  * synchronises `stream` once per launch of columns, to read the pool's counters.  A
  * column longer than the tape capacity (uwshcu_ctx_set_tape_capacity) sets
