@@ -75,12 +75,31 @@ def test_batched_adjoint_gives_the_same_gradients():
         np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
 
 
-def test_adjoint_records_again_when_the_tape_pool_runs_out():
-    """The recording takes tape pages from a pool sized for 1.25 times the previous
-    call's mean tape.  After a chunk that hardly convects, a convecting chunk under a
-    small workspace limit overruns the pool; its columns, which counted their lengths,
-    are recorded again in parts that fit, with the same gradients."""
+def _adjoint_on(ctx, c, cc):
+    """Gradients of one chunk's adjoint on the context `ctx`."""
     import torch
+    from uwshcu_amd import device
+    dev = torch.device("cuda:0")
+    inp = device.chunk_to_device(c, "cuda:0")
+    out = ctx.alloc_outputs(c.ncol, c.pver, c.ncnst, diag=True)
+    grads = {k: torch.zeros((c.pver, c.ncol), dtype=torch.float64, device=dev) for k in ("t0", "qv0", "u0", "v0")}
+    grads["tke"] = torch.zeros((c.pver + 1, c.ncol), dtype=torch.float64, device=dev)
+    grads["pblh"] = torch.ones(c.ncol, dtype=torch.float64, device=dev)
+    grads["par_col"] = torch.zeros((3, c.ncol), dtype=torch.float64, device=dev)
+    grads["par_sum"] = torch.zeros(3, dtype=torch.float64, device=dev)
+    cd = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in cc.items()}
+    print(f"\n[adjoint on one context] {c.ncol}x{c.pver}x{c.ncnst}", flush=True)
+    ctx.adjoint_into(inp, out, inp["cush"].clone(), cd, grads, dt=c.dt)
+    rc, _ = ctx.status()
+    assert rc == 0
+    return {k: v.cpu().numpy() for k, v in grads.items()}
+
+
+def test_adjoint_records_again_when_the_tape_pool_runs_out():
+    """The recording takes tape pages from a pool sized from the previous call's mean
+    tape.  After a chunk that hardly convects, a convecting chunk under a small
+    workspace limit overruns the pool; its columns, which counted their lengths, are
+    recorded again in parts that fit, with the same gradients."""
     from uwshcu_amd import device
     quiet = make_chunk(512, 35, 3, seed=64, inversion_frac=0.95)
     ch = make_chunk(700, 35, 3, seed=62, inversion_frac=0.1)
@@ -88,22 +107,32 @@ def test_adjoint_records_again_when_the_tape_pool_runs_out():
     _, _, g1 = _gpu_adjoint(ch, cot)
     ctx = device.Context(0, workspace_limit=600 << 20)
     try:
-        dev = torch.device("cuda:0")
-        for c, cc in ((quiet, _cot(quiet, 4)), (ch, cot)):
-            inp = device.chunk_to_device(c, "cuda:0")
-            out = ctx.alloc_outputs(c.ncol, c.pver, c.ncnst, diag=True)
-            grads = {k: torch.zeros((c.pver, c.ncol), dtype=torch.float64, device=dev) for k in ("t0", "qv0", "u0", "v0")}
-            grads["tke"] = torch.zeros((c.pver + 1, c.ncol), dtype=torch.float64, device=dev)
-            grads["pblh"] = torch.ones(c.ncol, dtype=torch.float64, device=dev)
-            grads["par_col"] = torch.zeros((3, c.ncol), dtype=torch.float64, device=dev)
-            grads["par_sum"] = torch.zeros(3, dtype=torch.float64, device=dev)
-            cd = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in cc.items()}
-            print(f"\n[adjoint pool] {c.ncol} columns", flush=True)
-            ctx.adjoint_into(inp, out, inp["cush"].clone(), cd, grads, dt=c.dt)
-            rc, _ = ctx.status()
-            assert rc == 0
+        _adjoint_on(ctx, quiet, _cot(quiet, 4))
+        g = _adjoint_on(ctx, ch, cot)
         for k in g1:
-            np.testing.assert_array_equal(grads[k].cpu().numpy(), g1[k], err_msg=k)
+            np.testing.assert_array_equal(g[k], g1[k], err_msg=k)
+    finally:
+        ctx.close()
+
+
+def test_adjoint_pool_estimate_follows_the_chunk_shape():
+    """One context takes chunks of 35 levels and 3 constituents, then 72 levels and 10,
+    then 35 and 3 again: the pool's estimate, learnt on one shape, is scaled to the
+    next one's levels and constituents, and every chunk's gradients equal those of a
+    fresh context."""
+    from uwshcu_amd import device
+    small = make_chunk(600, 35, 3, seed=66, inversion_frac=0.1)
+    big = make_chunk(160, 72, 10, seed=67, ptop=100.0, inversion_frac=0.1)
+    cs, cb = _cot(small, 5), _cot(big, 6)
+    fresh = {}
+    for name, c, cc in (("small", small, cs), ("big", big, cb)):
+        _, _, fresh[name] = _gpu_adjoint(c, cc, workspace_limit=2 << 30)
+    ctx = device.Context(0, workspace_limit=2 << 30)
+    try:
+        for name, c, cc in (("small", small, cs), ("big", big, cb), ("small", small, cs)):
+            g = _adjoint_on(ctx, c, cc)
+            for k in g:
+                np.testing.assert_array_equal(g[k], fresh[name][k], err_msg=f"{name} {k}")
     finally:
         ctx.close()
 
