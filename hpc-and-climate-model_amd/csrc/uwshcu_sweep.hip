@@ -9,14 +9,21 @@
 namespace uwshcu {
 
 // Backward sweep, one column per thread, with a 64-entry window of recent adjoints
-// (and its spare entry) per thread in LDS: 33 KB per workgroup of 64, so that the four
-// workgroups a CU needs to give each SIMD one wave fit its LDS.  Columns whose
-// recording did not complete are skipped.
+// (and its spare entry) per thread in LDS: 520 B per column, 33 KB per workgroup of 64,
+// so that the four workgroups a CU needs to give each SIMD one wave fit its LDS.  The
+// columns per workgroup (UWSHCU_SWEEP_LANES, 64 or 32) are a build-time parameter.
+// Columns whose recording did not complete are skipped.
 constexpr int kSweepWindow = 64;
-__global__ __launch_bounds__(64) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
-                                                          long long ldc, const TapeCtx* tapes, const int* node_out,
-                                                          double* adj, AdCotangents cot, AdGradients g) {
-    __shared__ double s_win[kSweepWindow + 1][64];
+#ifndef UWSHCU_SWEEP_LANES
+#define UWSHCU_SWEEP_LANES 64
+#endif
+constexpr int kSweepLanes = UWSHCU_SWEEP_LANES;
+static_assert(kSweepLanes == 64 || kSweepLanes == 32, "a sweep workgroup is one wavefront");
+__global__ __launch_bounds__(kSweepLanes) void adjoint_sweep_kernel(int n, int col0, int mkx, int ncnst, long long wld,
+                                                                   long long ldc, const TapeCtx* tapes,
+                                                                   const int* node_out, double* adj, AdCotangents cot,
+                                                                   AdGradients g) {
+    __shared__ double s_win[kSweepWindow + 1][kSweepLanes];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const TapeCtx& t = tapes[j];
@@ -40,7 +47,8 @@ hipError_t launch_adjoint_sweep(const FwdArgs& args, int n, double* adj, long lo
         if (blocks > 8192) blocks = 8192;
         hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, adj, nadj);
     }
-    hipLaunchKernelGGL(adjoint_sweep_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, n, args.col0, args.mkx,
+    hipLaunchKernelGGL(adjoint_sweep_kernel, dim3((n + kSweepLanes - 1) / kSweepLanes), dim3(kSweepLanes), 0, stream,
+                       n, args.col0, args.mkx,
                        args.ncnst, args.wld, (long long)args.ldc, (const TapeCtx*)args.ad.tapes, node_out, adj, cot, g);
     return hipGetLastError();
 }
