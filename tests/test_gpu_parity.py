@@ -155,12 +155,17 @@ def test_division_without_range_scaling_is_the_ieee_quotient():
             assert diff.size == 0, f"{nm} on the {label}: {diff.size} quotients differ, e.g. {x[diff[0]]!r} / {y[diff[0]]!r}"
 
 
-def test_c1_chunk_16x30():
-    _check(make_chunk(16, 30, 3, seed=0), "16x30x3")
+# "pbl": a dry-adiabatic mixed layer under the inversion; "baseline": the benchmark
+# configurations' stated column recipe (6.5 K/km from the surface, RH 0.85 falling to
+# 0.2 at 500 hPa, +3 K above the PBL on 25 % of the columns)
+@pytest.mark.parametrize("recipe", ["pbl", "baseline"])
+def test_c1_chunk_16x30(recipe):
+    _check(make_chunk(16, 30, 3, seed=0, recipe=recipe), f"16x30x3 {recipe}")
 
 
-def test_c2_chunk_32768x35():
-    _check(make_chunk(32768, 35, 3, seed=1), "32768x35x3")
+@pytest.mark.parametrize("recipe", ["pbl", "baseline"])
+def test_c2_chunk_32768x35(recipe):
+    _check(make_chunk(32768, 35, 3, seed=1, recipe=recipe), f"32768x35x3 {recipe}")
 
 
 def test_72_levels_10_constituents():
