@@ -46,11 +46,11 @@ def _gpu_adjoint(ch, cot, workspace_limit=None, cap=None):
     return rc, res, g
 
 
-@pytest.mark.parametrize("ncol,pver,ncnst", [(1024, 35, 3), (128, 72, 10)])
-def test_device_adjoint_equals_host_adjoint(ncol, pver, ncnst):
-    ch = make_chunk(ncol, pver, ncnst, seed=60 + pver, ptop=100.0 if pver > 60 else 200.0)
+@pytest.mark.parametrize("ncol,pver,ncnst,recipe", [(1024, 35, 3, "pbl"), (128, 72, 10, "pbl"), (1024, 30, 3, "baseline")])
+def test_device_adjoint_equals_host_adjoint(ncol, pver, ncnst, recipe):
+    ch = make_chunk(ncol, pver, ncnst, seed=60 + pver, ptop=100.0 if pver > 60 else 200.0, recipe=recipe)
     cot = _cot(ch, 1)
-    print(f"\n[adjoint {ncol}x{pver}x{ncnst}] device run", flush=True)
+    print(f"\n[adjoint {ncol}x{pver}x{ncnst} {recipe}] device run", flush=True)
     rc, out, g = _gpu_adjoint(ch, cot)
     assert rc == 0
     ref, rdiag, _, _ = oracle.run(ch, nthreads=8, portable=True)

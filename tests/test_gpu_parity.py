@@ -297,15 +297,16 @@ def test_host_pointer_entry_and_fortran_dropin_symbol():
         np.testing.assert_array_equal(outs[k], pm[k], err_msg=k)
 
 
-@pytest.mark.parametrize("ncol", [8192, 140000])
-def test_mixed_precision_pipeline(ncol):
+@pytest.mark.parametrize("ncol,recipe", [(8192, "pbl"), (140000, "pbl"), (8192, "baseline")])
+def test_mixed_precision_pipeline(ncol, recipe):
     """MIXED = fast FP64 pass + conservation/branch check + FP64 recompute of the listed
     columns.  Columns it changed equal the FP64 path bit for bit; the others equal the
     fast pass alone (MIXED_FAST_UNCHECKED).  Every output is within 1e-4 relative of
     the FP64 path in every column and 1e-6 in relative L2.  At 140000 columns the
-    passes run as the split large launch."""
+    passes run as the split large launch; the stated column recipe is run at its own
+    inversion fraction (0.25)."""
     from uwshcu_amd import _capi, device
-    ch = make_chunk(ncol, 35, 3, seed=33, inversion_frac=0.3)
+    ch = make_chunk(ncol, 35, 3, seed=33, inversion_frac=0.3 if recipe == "pbl" else 0.25, recipe=recipe)
     ctx = device.default_context()
     inp = device.chunk_to_device(ch)
     res = {}
@@ -318,7 +319,7 @@ def test_mixed_precision_pipeline(ncol):
     for k in f64:
         if k != "diag":
             changed |= (mix[k] != raw[k]).reshape(-1, ch.ncol).any(axis=0)
-    print(f"\n[mixed {ncol}x35x3] recomputed {nfix} columns, {int(changed.sum())} changed", flush=True)
+    print(f"\n[mixed {ncol}x35x3 {recipe}] recomputed {nfix} columns, {int(changed.sum())} changed", flush=True)
     assert changed.sum() <= nfix < 0.05 * ch.ncol
     for k in f64:
         if k == "diag":
@@ -326,4 +327,4 @@ def test_mixed_precision_pipeline(ncol):
         np.testing.assert_array_equal(mix[k][..., changed], f64[k][..., changed], err_msg=k)
         np.testing.assert_array_equal(mix[k][..., ~changed], raw[k][..., ~changed], err_msg=k)
     np.testing.assert_array_equal(mix["diag"], f64["diag"])
-    _assert_within_bound(mix, f64, f"mixed {ncol}")
+    _assert_within_bound(mix, f64, f"mixed {ncol} {recipe}")
